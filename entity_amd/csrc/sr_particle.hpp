@@ -1,0 +1,548 @@
+// sr_particle.hpp -- per-particle device routines of the SR hot path:
+// B-spline shapes, field interpolation, Boris/Vay momentum update, Cartesian
+// position update with boundary conditions, and the charge-conserving current
+// deposit.  Every routine is templated on a field accessor so that the same
+// code runs against HBM (FieldView) and against LDS-staged windows.
+//
+// Reference: src/kernels/pushers/sr.hpp, src/kernels/particle_shapes.hpp,
+// src/kernels/currents_deposit.hpp (line numbers at each routine).
+#pragma once
+
+#include "ntt_common.hpp"
+
+namespace ntt {
+
+enum PusherAlg : int { ALG_BORIS = 0, ALG_VAY = 1, ALG_PHOTON = 2 };
+
+// ---- shapes: particle_shapes.hpp:544-588 --------------------------------
+template <bool STAG, int O>
+__device__ __forceinline__ void shape_order(int i, float di, int& i_min, float* S) {
+  if constexpr (O == 1) {
+    if constexpr (!STAG) {
+      i_min = i;
+      S[0]  = ONE - di;
+      S[1]  = di;
+    } else {
+      if (di < HALF) {
+        i_min = i - 1;
+        S[0]  = HALF - di;
+      } else {
+        i_min = i;
+        S[0]  = THREE_HALFS - di;
+      }
+      S[1] = ONE - S[0];
+    }
+  } else if constexpr (O == 2) {
+    if constexpr (!STAG) {
+      if (di < HALF) {
+        i_min = i - 1;
+        S[0]  = HALF * ((HALF - di) * (HALF - di));
+        S[1]  = THREE_FOURTHS - di * di;
+      } else {
+        i_min = i;
+        S[0]  = HALF * ((THREE_HALFS - di) * (THREE_HALFS - di));
+        S[1]  = THREE_FOURTHS - (ONE - di) * (ONE - di);
+      }
+      S[2] = ONE - S[0] - S[1];
+    } else {
+      i_min = i - 1;
+      S[0]  = HALF * ((ONE - di) * (ONE - di));
+      S[2]  = HALF * (di * di);
+      S[1]  = ONE - S[0] - S[2];
+    }
+  }
+}
+
+// ---- merged deposit stencils: particle_shapes.hpp:934-1024 --------------
+// Returns i_min and the number of extra slots (i_max - i_min).
+template <int O>
+__device__ __forceinline__ void for_deposit(int i_init, float di_init, int i_fin, float di_fin,
+                                            int& i_min, int& di, float* iS, float* fS) {
+  int   a, b;
+  float iS_[O + 1], fS_[O + 1];
+  shape_order<false, O>(i_init, di_init, a, iS_);
+  shape_order<false, O>(i_fin, di_fin, b, fS_);
+  if (a < b) {
+    i_min = a;
+    di    = O + 1;
+#pragma unroll
+    for (int j = 0; j < O + 1; j++) iS[j] = iS_[j];
+    iS[O + 1] = ZERO;
+    fS[0]     = ZERO;
+#pragma unroll
+    for (int j = 0; j < O + 1; j++) fS[j + 1] = fS_[j];
+  } else if (a > b) {
+    i_min = b;
+    di    = O + 1;
+    iS[0] = ZERO;
+#pragma unroll
+    for (int j = 0; j < O + 1; j++) iS[j + 1] = iS_[j];
+#pragma unroll
+    for (int j = 0; j < O + 1; j++) fS[j] = fS_[j];
+    fS[O + 1] = ZERO;
+  } else {
+    i_min = a;
+    di    = O;
+#pragma unroll
+    for (int j = 0; j < O + 1; j++) iS[j] = iS_[j];
+    iS[O + 1] = ZERO;
+#pragma unroll
+    for (int j = 0; j < O + 1; j++) fS[j] = fS_[j];
+    fS[O + 1] = ZERO;
+  }
+}
+
+// ---- field accessors --------------------------------------------------------
+// Global (HBM) accessor over the ghost-extended em array.
+struct GlobalEM {
+  const float* __restrict__ d;
+  int32_t N1, N2;
+  int64_t cstride;
+  __device__ __forceinline__ float operator()(int c, int i, int j, int k) const {
+    return d[static_cast<int64_t>(i) +
+             static_cast<int64_t>(N1) * (static_cast<int64_t>(j) + static_cast<int64_t>(N2) * k) +
+             cstride * c];
+  }
+};
+
+// ---- interpolation, order 0 "direct": sr.hpp:905-1117 ------------------------
+template <class A>
+__device__ __forceinline__ float lin2(const A& F, int c, int ia, int ja, int k, float wx0,
+                                      float wx1, float wy0, float wy1) {
+  const float c000 = F(c, ia, ja, k), c100 = F(c, ia + 1, ja, k);
+  const float c010 = F(c, ia, ja + 1, k), c110 = F(c, ia + 1, ja + 1, k);
+  const float c00 = c000 * wx0 + c100 * wx1;
+  const float c10 = c010 * wx0 + c110 * wx1;
+  return c00 * wy0 + c10 * wy1;
+}
+
+template <class A>
+__device__ __forceinline__ float lin3(const A& F, int c, int ia, int ja, int ka, float wx0,
+                                      float wx1, float wy0, float wy1, float wz0, float wz1) {
+  const float c000 = F(c, ia, ja, ka), c100 = F(c, ia + 1, ja, ka);
+  const float c010 = F(c, ia, ja + 1, ka), c110 = F(c, ia + 1, ja + 1, ka);
+  const float c001 = F(c, ia, ja, ka + 1), c101 = F(c, ia + 1, ja, ka + 1);
+  const float c011 = F(c, ia, ja + 1, ka + 1), c111 = F(c, ia + 1, ja + 1, ka + 1);
+  const float c00 = c000 * wx0 + c100 * wx1;
+  const float c10 = c010 * wx0 + c110 * wx1;
+  const float c0  = c00 * wy0 + c10 * wy1;
+  const float c01 = c001 * wx0 + c101 * wx1;
+  const float c11 = c011 * wx0 + c111 * wx1;
+  const float c1  = c01 * wy0 + c11 * wy1;
+  return c0 * wz0 + c1 * wz1;
+}
+
+// ---- interpolation, B-spline orders: sr.hpp:1170-1366 ------------------------
+template <int O, class A>
+__device__ __forceinline__ float spl2(const A& F, int c, int imin, int jmin, const float* S1,
+                                      const float* S2) {
+  float e = ZERO;
+#pragma unroll
+  for (int b = 0; b < O + 1; b++) {
+    float c0 = ZERO;
+#pragma unroll
+    for (int a = 0; a < O + 1; a++) c0 += S1[a] * F(c, imin + a, jmin + b, 0);
+    e += c0 * S2[b];
+  }
+  return e;
+}
+
+template <int O, class A>
+__device__ __forceinline__ float spl3(const A& F, int c, int imin, int jmin, int kmin,
+                                      const float* S1, const float* S2, const float* S3) {
+  float e = ZERO;
+#pragma unroll
+  for (int g = 0; g < O + 1; g++) {
+    float c0 = ZERO;
+#pragma unroll
+    for (int b = 0; b < O + 1; b++) {
+      float c00 = ZERO;
+#pragma unroll
+      for (int a = 0; a < O + 1; a++) c00 += S1[a] * F(c, imin + a, jmin + b, kmin + g);
+      c0 += c00 * S2[b];
+    }
+    e += c0 * S3[g];
+  }
+  return e;
+}
+
+// Interpolates E and B at the particle.  (i, j, k) are ghost-extended cell
+// indices (particle index + ngh); k is ignored in 2D.
+template <int D, int O, class A>
+__device__ __forceinline__ void interpolate(const A& F, int i, int j, int k, float dx1,
+                                            float dx2, float dx3, float* e0, float* b0) {
+  if constexpr (O == 0) {
+    const int indx = static_cast<int>(dx1 + HALF);
+    const int indy = static_cast<int>(dx2 + HALF);
+    const float ponpmx = ONE - dx1, ponppx = dx1;
+    const float ponpmy = ONE - dx2, ponppy = dx2;
+    const float pondmx = static_cast<float>(indx + 1) - (dx1 + HALF), pondpx = ONE - pondmx;
+    const float pondmy = static_cast<float>(indy + 1) - (dx2 + HALF), pondpy = ONE - pondmy;
+    if constexpr (D == 2) {
+      e0[0] = lin2(F, 0, i - 1 + indx, j, 0, pondmx, pondpx, ponpmy, ponppy);
+      e0[1] = lin2(F, 1, i, j - 1 + indy, 0, ponpmx, ponppx, pondmy, pondpy);
+      e0[2] = lin2(F, 2, i, j, 0, ponpmx, ponppx, ponpmy, ponppy);
+      b0[0] = lin2(F, 3, i, j - 1 + indy, 0, ponpmx, ponppx, pondmy, pondpy);
+      b0[1] = lin2(F, 4, i - 1 + indx, j, 0, pondmx, pondpx, ponpmy, ponppy);
+      b0[2] = lin2(F, 5, i - 1 + indx, j - 1 + indy, 0, pondmx, pondpx, pondmy, pondpy);
+    } else {
+      const int   indz   = static_cast<int>(dx3 + HALF);
+      const float ponpmz = ONE - dx3, ponppz = dx3;
+      const float pondmz = static_cast<float>(indz + 1) - (dx3 + HALF), pondpz = ONE - pondmz;
+      e0[0] = lin3(F, 0, i - 1 + indx, j, k, pondmx, pondpx, ponpmy, ponppy, ponpmz, ponppz);
+      e0[1] = lin3(F, 1, i, j - 1 + indy, k, ponpmx, ponppx, pondmy, pondpy, ponpmz, ponppz);
+      e0[2] = lin3(F, 2, i, j, k - 1 + indz, ponpmx, ponppx, ponpmy, ponppy, pondmz, pondpz);
+      b0[0] = lin3(F, 3, i, j - 1 + indy, k - 1 + indz, ponpmx, ponppx, pondmy, pondpy, pondmz, pondpz);
+      b0[1] = lin3(F, 4, i - 1 + indx, j, k - 1 + indz, pondmx, pondpx, ponpmy, ponppy, pondmz, pondpz);
+      b0[2] = lin3(F, 5, i - 1 + indx, j - 1 + indy, k, pondmx, pondpx, pondmy, pondpy, ponpmz, ponppz);
+    }
+  } else {
+    float S1p[O + 1], S1d[O + 1], S2p[O + 1], S2d[O + 1];
+    int   ip, id, jp, jd;
+    shape_order<false, O>(i, dx1, ip, S1p);
+    shape_order<false, O>(j, dx2, jp, S2p);
+    shape_order<true, O>(i, dx1, id, S1d);
+    shape_order<true, O>(j, dx2, jd, S2d);
+    if constexpr (D == 2) {
+      e0[0] = spl2<O>(F, 0, id, jp, S1d, S2p);
+      e0[1] = spl2<O>(F, 1, ip, jd, S1p, S2d);
+      e0[2] = spl2<O>(F, 2, ip, jp, S1p, S2p);
+      b0[0] = spl2<O>(F, 3, ip, jd, S1p, S2d);
+      b0[1] = spl2<O>(F, 4, id, jp, S1d, S2p);
+      b0[2] = spl2<O>(F, 5, id, jd, S1d, S2d);
+    } else {
+      float S3p[O + 1], S3d[O + 1];
+      int   kp, kd;
+      shape_order<false, O>(k, dx3, kp, S3p);
+      shape_order<true, O>(k, dx3, kd, S3d);
+      e0[0] = spl3<O>(F, 0, id, jp, kp, S1d, S2p, S3p);
+      e0[1] = spl3<O>(F, 1, ip, jd, kp, S1p, S2d, S3p);
+      e0[2] = spl3<O>(F, 2, ip, jp, kd, S1p, S2p, S3d);
+      b0[0] = spl3<O>(F, 3, ip, jd, kd, S1p, S2d, S3d);
+      b0[1] = spl3<O>(F, 4, id, jp, kd, S1d, S2p, S3d);
+      b0[2] = spl3<O>(F, 5, id, jd, kp, S1d, S2d, S3p);
+    }
+  }
+}
+
+// Ghost-extended index range [lo, hi] of the interpolation stencil around
+// cell i (ghost-extended), used for bounds checks: orders 0-2 read at most
+// [i - 1, i + 2].
+template <int O>
+__device__ __forceinline__ void interp_reach(int& lo, int& hi) {
+  if constexpr (O == 0) { lo = -1; hi = 1; }
+  else if constexpr (O == 1) { lo = -1; hi = 1; }
+  else { lo = -1; hi = 2; }
+}
+
+// ---- momentum update: sr.hpp:337-437 -----------------------------------------
+__device__ __forceinline__ void boris(float coeff, float* u, float* e0, float* b0) {
+  float COEFF = coeff;
+  e0[0] *= COEFF; e0[1] *= COEFF; e0[2] *= COEFF;
+  float u0[3] = { u[0] + e0[0], u[1] + e0[1], u[2] + e0[2] };
+  COEFF *= ONE / sqrtf(ONE + (u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2]));
+  b0[0] *= COEFF; b0[1] *= COEFF; b0[2] *= COEFF;
+  COEFF = TWO / (ONE + (b0[0] * b0[0] + b0[1] * b0[1] + b0[2] * b0[2]));
+  const float u1[3] = { (u0[0] + (u0[1] * b0[2] - u0[2] * b0[1])) * COEFF,
+                        (u0[1] + (u0[2] * b0[0] - u0[0] * b0[2])) * COEFF,
+                        (u0[2] + (u0[0] * b0[1] - u0[1] * b0[0])) * COEFF };
+  u0[0] += (u1[1] * b0[2] - u1[2] * b0[1]) + e0[0];
+  u0[1] += (u1[2] * b0[0] - u1[0] * b0[2]) + e0[1];
+  u0[2] += (u1[0] * b0[1] - u1[1] * b0[0]) + e0[2];
+  u[0] = u0[0]; u[1] = u0[1]; u[2] = u0[2];
+}
+
+__device__ __forceinline__ void vay(float coeff, float* u, float* e0, float* b0) {
+  float COEFF = coeff;
+  e0[0] *= COEFF; e0[1] *= COEFF; e0[2] *= COEFF;
+  b0[0] *= COEFF; b0[1] *= COEFF; b0[2] *= COEFF;
+  COEFF = ONE / sqrtf(ONE + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+  const float u1[3] = { (u[0] + TWO * e0[0] + (u[1] * b0[2] - u[2] * b0[1]) * COEFF),
+                        (u[1] + TWO * e0[1] + (u[2] * b0[0] - u[0] * b0[2]) * COEFF),
+                        (u[2] + TWO * e0[2] + (u[0] * b0[1] - u[1] * b0[0]) * COEFF) };
+  COEFF = u1[0] * b0[0] + u1[1] * b0[1] + u1[2] * b0[2];
+  float COEFF2 = ONE + (u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]) -
+                 (b0[0] * b0[0] + b0[1] * b0[1] + b0[2] * b0[2]);
+  COEFF = ONE / sqrtf(INV_2 * (COEFF2 + sqrtf(COEFF2 * COEFF2 +
+                                              FOUR * (b0[0] * b0[0] + b0[1] * b0[1] +
+                                                      b0[2] * b0[2] + COEFF * COEFF))));
+  COEFF2 = ONE / (ONE + (b0[0] * COEFF) * (b0[0] * COEFF) + (b0[1] * COEFF) * (b0[1] * COEFF) +
+                  (b0[2] * COEFF) * (b0[2] * COEFF));
+  const float ub = u1[0] * b0[0] + u1[1] * b0[1] + u1[2] * b0[2];
+  u[0] = COEFF2 * (u1[0] + COEFF * ub * (b0[0] * COEFF) + u1[1] * b0[2] * COEFF -
+                   u1[2] * b0[1] * COEFF);
+  u[1] = COEFF2 * (u1[1] + COEFF * ub * (b0[1] * COEFF) + u1[2] * b0[0] * COEFF -
+                   u1[0] * b0[2] * COEFF);
+  u[2] = COEFF2 * (u1[2] + COEFF * ub * (b0[2] * COEFF) + u1[0] * b0[1] * COEFF -
+                   u1[1] * b0[0] * COEFF);
+}
+
+// ---- position update, Cartesian: sr.hpp:528-572 ------------------------------
+__device__ __forceinline__ void push_coord(int& i, float& dx, float v_dt) {
+  dx += v_dt;
+  i += static_cast<int>(dx >= ONE) - static_cast<int>(dx < ZERO);
+  dx -= static_cast<float>(dx >= ONE);
+  dx += static_cast<float>(dx < ZERO);
+}
+
+// ---- boundary conditions, Cartesian: sr.hpp:659-790 --------------------------
+struct PrtlBCs {
+  uint8_t lo[3], hi[3];
+  int32_t ni[3];
+};
+
+__device__ __forceinline__ void bc_dir(int& i, int& ip, float& dx, float& u, int16_t& tag, int ni,
+                                       uint8_t bmin, uint8_t bmax, bool allow_axis) {
+  bool invert = false;
+  if (i < 0) {
+    if (bmin == NTT_PRTL_BC_PERIODIC) { i += ni; ip += ni; }
+    else if (bmin == NTT_PRTL_BC_ABSORB || bmin == NTT_PRTL_BC_ATMOSPHERE) tag = NTT_TAG_DEAD;
+    else if (bmin == NTT_PRTL_BC_REFLECT) { i = 0; dx = ONE - dx; invert = true; }
+    else if (allow_axis && bmin == NTT_PRTL_BC_AXIS) { i = 0; dx = ONE - dx; }
+  } else if (i >= ni) {
+    if (bmax == NTT_PRTL_BC_PERIODIC) { i -= ni; ip -= ni; }
+    else if (bmax == NTT_PRTL_BC_ABSORB || bmax == NTT_PRTL_BC_ATMOSPHERE) tag = NTT_TAG_DEAD;
+    else if (bmax == NTT_PRTL_BC_REFLECT) { i = ni - 1; dx = ONE - dx; invert = true; }
+    else if (allow_axis && bmax == NTT_PRTL_BC_AXIS) { i = ni - 1; dx = ONE - dx; }
+  }
+  if (invert) u = -u;
+}
+
+// Per-particle state of one SR update, held in registers.
+struct PState {
+  int   i[3], ip[3];
+  float dx[3], dxp[3];
+  float u[3];
+  int16_t tag;
+};
+
+// One full SR update of a particle (alive check done by the caller):
+// interpolation, momentum update, position update, boundary conditions.
+// sr.hpp:117-332 (no-policy path) + 526-814.
+template <int D, int O, int ALG, class A>
+__device__ __forceinline__ void sr_update(const A& F, int ngh, float ndth, float dt, float dxm,
+                                          const PrtlBCs& bc, PState& s) {
+  if constexpr (ALG != ALG_PHOTON) {
+    float e0[3], b0[3];
+    interpolate<D, O>(F, s.i[0] + ngh, s.i[1] + ngh, D == 3 ? s.i[2] + ngh : 0, s.dx[0], s.dx[1],
+                      D == 3 ? s.dx[2] : 0.0f, e0, b0);
+    // transform_xyz<U, XYZ> (minkowski.h:282-289): in-plane components times dx
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      e0[d] *= dxm;
+      b0[d] *= dxm;
+    }
+    if constexpr (ALG == ALG_BORIS) boris(ndth, s.u, e0, b0);
+    else vay(ndth, s.u, e0, b0);
+  }
+  const float ux1 = s.u[0], ux2 = s.u[1], ux3 = s.u[2];
+  const float dt_inv_energy = (ALG != ALG_PHOTON)
+                                ? (dt / sqrtf(ONE + ux1 * ux1 + ux2 * ux2 + ux3 * ux3))
+                                : (dt / sqrtf(ux1 * ux1 + ux2 * ux2 + ux3 * ux3));
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    s.ip[d]  = s.i[d];
+    s.dxp[d] = s.dx[d];
+    // transform<d, XYZ, U>: u / dx (minkowski.h:277-281)
+    push_coord(s.i[d], s.dx[d], (s.u[d] / dxm) * dt_inv_energy);
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+    bc_dir(s.i[d], s.ip[d], s.dx[d], s.u[d], s.tag, bc.ni[d], bc.lo[d], bc.hi[d], d == 1);
+}
+
+// ---- current deposit: currents_deposit.hpp:108-761 ---------------------------
+// A "J sink" receives additions at ghost-extended indices (i, j, k, comp).
+
+// velocity for the out-of-plane 2D component: currents_deposit.hpp:112-166
+__device__ __forceinline__ float vp_out_of_plane(const float* u) {
+  const float inv_energy = ONE / sqrtf(ONE + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+  float v = u[2];
+  if (isnan(v) || isinf(v)) v = ZERO;
+  return v * inv_energy;
+}
+
+// zig-zag deposit, O = 0: currents_deposit.hpp:171-405
+template <int D, class Sink>
+__device__ __forceinline__ void deposit_zigzag(Sink& J, int G, const PState& s, float vz,
+                                               float coeff, float inv_dt) {
+  float W1[3], W2[3], F1[3], F2[3];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const int   i = s.i[d], ip = s.ip[d];
+    const float dxp_r = static_cast<float>(i == ip) * (s.dx[d] + s.dxp[d]) * INV_2;
+    const int   gt    = static_cast<int>(i > ip);
+    W1[d] = INV_2 * (dxp_r + s.dxp[d] + static_cast<float>(gt));
+    W2[d] = INV_2 * (s.dx[d] + dxp_r + static_cast<float>(gt + ip - i));
+    F1[d] = (static_cast<float>(gt) + dxp_r - s.dxp[d]) * coeff * inv_dt;
+    F2[d] = (static_cast<float>(i - ip - gt) + s.dx[d] - dxp_r) * coeff * inv_dt;
+  }
+  const int a = s.ip[0] + G, b = s.ip[1] + G, A = s.i[0] + G, B = s.i[1] + G;
+  if constexpr (D == 2) {
+    const float Fx3_1 = HALF * vz * coeff;
+    const float Fx3_2 = HALF * vz * coeff;
+    J.add(a, b, 0, 0, F1[0] * (ONE - W1[1]));
+    J.add(a, b + 1, 0, 0, F1[0] * W1[1]);
+    J.add(A, B, 0, 0, F2[0] * (ONE - W2[1]));
+    J.add(A, B + 1, 0, 0, F2[0] * W2[1]);
+    J.add(a, b, 0, 1, F1[1] * (ONE - W1[0]));
+    J.add(a + 1, b, 0, 1, F1[1] * W1[0]);
+    J.add(A, B, 0, 1, F2[1] * (ONE - W2[0]));
+    J.add(A + 1, B, 0, 1, F2[1] * W2[0]);
+    J.add(a, b, 0, 2, Fx3_1 * (ONE - W1[0]) * (ONE - W1[1]));
+    J.add(a + 1, b, 0, 2, Fx3_1 * W1[0] * (ONE - W1[1]));
+    J.add(a, b + 1, 0, 2, Fx3_1 * (ONE - W1[0]) * W1[1]);
+    J.add(a + 1, b + 1, 0, 2, Fx3_1 * W1[0] * W1[1]);
+    J.add(A, B, 0, 2, Fx3_2 * (ONE - W2[0]) * (ONE - W2[1]));
+    J.add(A + 1, B, 0, 2, Fx3_2 * W2[0] * (ONE - W2[1]));
+    J.add(A, B + 1, 0, 2, Fx3_2 * (ONE - W2[0]) * W2[1]);
+    J.add(A + 1, B + 1, 0, 2, Fx3_2 * W2[0] * W2[1]);
+  } else {
+    const int c = s.ip[2] + G, C = s.i[2] + G;
+    J.add(a, b, c, 0, F1[0] * (ONE - W1[1]) * (ONE - W1[2]));
+    J.add(a, b + 1, c, 0, F1[0] * W1[1] * (ONE - W1[2]));
+    J.add(a, b, c + 1, 0, F1[0] * (ONE - W1[1]) * W1[2]);
+    J.add(a, b + 1, c + 1, 0, F1[0] * W1[1] * W1[2]);
+    J.add(A, B, C, 0, F2[0] * (ONE - W2[1]) * (ONE - W2[2]));
+    J.add(A, B + 1, C, 0, F2[0] * W2[1] * (ONE - W2[2]));
+    J.add(A, B, C + 1, 0, F2[0] * (ONE - W2[1]) * W2[2]);
+    J.add(A, B + 1, C + 1, 0, F2[0] * W2[1] * W2[2]);
+    J.add(a, b, c, 1, F1[1] * (ONE - W1[0]) * (ONE - W1[2]));
+    J.add(a + 1, b, c, 1, F1[1] * W1[0] * (ONE - W1[2]));
+    J.add(a, b, c + 1, 1, F1[1] * (ONE - W1[0]) * W1[2]);
+    J.add(a + 1, b, c + 1, 1, F1[1] * W1[0] * W1[2]);
+    J.add(A, B, C, 1, F2[1] * (ONE - W2[0]) * (ONE - W2[2]));
+    J.add(A + 1, B, C, 1, F2[1] * W2[0] * (ONE - W2[2]));
+    J.add(A, B, C + 1, 1, F2[1] * (ONE - W2[0]) * W2[2]);
+    J.add(A + 1, B, C + 1, 1, F2[1] * W2[0] * W2[2]);
+    J.add(a, b, c, 2, F1[2] * (ONE - W1[0]) * (ONE - W1[1]));
+    J.add(a + 1, b, c, 2, F1[2] * W1[0] * (ONE - W1[1]));
+    J.add(a, b + 1, c, 2, F1[2] * (ONE - W1[0]) * W1[1]);
+    J.add(a + 1, b + 1, c, 2, F1[2] * W1[0] * W1[1]);
+    J.add(A, B, C, 2, F2[2] * (ONE - W2[0]) * (ONE - W2[1]));
+    J.add(A + 1, B, C, 2, F2[2] * W2[0] * (ONE - W2[1]));
+    J.add(A, B + 1, C, 2, F2[2] * (ONE - W2[0]) * W2[1]);
+    J.add(A + 1, B + 1, C, 2, F2[2] * W2[0] * W2[1]);
+  }
+}
+
+// Esirkepov deposit, O = 1, 2 in 2D: currents_deposit.hpp:476-555
+template <int O, class Sink>
+__device__ __forceinline__ void deposit_esirkepov_2d(Sink& J, int G, const PState& s, float vz,
+                                                     float coeff, float inv_dt) {
+  constexpr int M = O + 2;
+  float iS1[M], fS1[M], iS2[M], fS2[M];
+  int   i1_min, d1, i2_min, d2;
+  for_deposit<O>(s.ip[0], s.dxp[0], s.i[0], s.dx[0], i1_min, d1, iS1, fS1);
+  for_deposit<O>(s.ip[1], s.dxp[1], s.i[1], s.dx[1], i2_min, d2, iS2, fS2);
+  const float Qdt = coeff * inv_dt;
+  const float QVx3 = coeff * vz;
+  i1_min += G;
+  i2_min += G;
+  // jx1: prefix sum over i of -Q*Wx1
+  float jx1[M][M], jx2[M][M];
+#pragma unroll
+  for (int b = 0; b < M; ++b) jx1[0][b] = -Qdt * (HALF * (fS1[0] - iS1[0]) * (fS2[b] + iS2[b]));
+#pragma unroll
+  for (int a = 1; a < M; ++a)
+#pragma unroll
+    for (int b = 0; b < M; ++b)
+      jx1[a][b] = jx1[a - 1][b] - Qdt * (HALF * (fS1[a] - iS1[a]) * (fS2[b] + iS2[b]));
+#pragma unroll
+  for (int a = 0; a < M; ++a) jx2[a][0] = -Qdt * (HALF * (fS1[a] + iS1[a]) * (fS2[0] - iS2[0]));
+#pragma unroll
+  for (int b = 1; b < M; ++b)
+#pragma unroll
+    for (int a = 0; a < M; ++a)
+      jx2[a][b] = jx2[a][b - 1] - Qdt * (HALF * (fS1[a] + iS1[a]) * (fS2[b] - iS2[b]));
+#pragma unroll
+  for (int a = 0; a < M; ++a)
+#pragma unroll
+    for (int b = 0; b < M; ++b) {
+      if (a < d1 && b <= d2) J.add(i1_min + a, i2_min + b, 0, 0, jx1[a][b]);
+      if (a <= d1 && b < d2) J.add(i1_min + a, i2_min + b, 0, 1, jx2[a][b]);
+      if (a <= d1 && b <= d2) {
+        const float w3 = THIRD * (fS2[b] * (HALF * iS1[a] + fS1[a]) + iS2[b] * (HALF * fS1[a] + iS1[a]));
+        J.add(i1_min + a, i2_min + b, 0, 2, QVx3 * w3);
+      }
+    }
+}
+
+// Esirkepov deposit, O = 1, 2 in 3D: currents_deposit.hpp:557-754
+template <int O, class Sink>
+__device__ __forceinline__ void deposit_esirkepov_3d(Sink& J, int G, const PState& s, float coeff,
+                                                     float inv_dt) {
+  constexpr int M = O + 2;
+  float iS1[M], fS1[M], iS2[M], fS2[M], iS3[M], fS3[M];
+  int   i1_min, d1, i2_min, d2, i3_min, d3;
+  for_deposit<O>(s.ip[0], s.dxp[0], s.i[0], s.dx[0], i1_min, d1, iS1, fS1);
+  for_deposit<O>(s.ip[1], s.dxp[1], s.i[1], s.dx[1], i2_min, d2, iS2, fS2);
+  for_deposit<O>(s.ip[2], s.dxp[2], s.i[2], s.dx[2], i3_min, d3, iS3, fS3);
+  const float Qdt = coeff * inv_dt;
+  i1_min += G;
+  i2_min += G;
+  i3_min += G;
+  // jx1 (running prefix over a)
+#pragma unroll
+  for (int b = 0; b < M; ++b) {
+#pragma unroll
+    for (int g = 0; g < M; ++g) {
+      const float t23 = (iS2[b] * iS3[g] + fS2[b] * fS3[g]) + HALF * (iS3[g] * fS2[b] + iS2[b] * fS3[g]);
+      float acc = ZERO;
+#pragma unroll
+      for (int a = 0; a < M; ++a) {
+        acc = (a == 0 ? ZERO : acc) - Qdt * (THIRD * (fS1[a] - iS1[a]) * t23);
+        if (a < d1 && b <= d2 && g <= d3) J.add(i1_min + a, i2_min + b, i3_min + g, 0, acc);
+      }
+    }
+  }
+  // jx2 (running prefix over b)
+#pragma unroll
+  for (int a = 0; a < M; ++a) {
+#pragma unroll
+    for (int g = 0; g < M; ++g) {
+      const float t13 = iS1[a] * iS3[g] + fS1[a] * fS3[g] + HALF * (iS3[g] * fS1[a] + iS1[a] * fS3[g]);
+      float acc = ZERO;
+#pragma unroll
+      for (int b = 0; b < M; ++b) {
+        acc = (b == 0 ? ZERO : acc) - Qdt * (THIRD * (fS2[b] - iS2[b]) * t13);
+        if (a <= d1 && b < d2 && g <= d3) J.add(i1_min + a, i2_min + b, i3_min + g, 1, acc);
+      }
+    }
+  }
+  // jx3 (running prefix over g)
+#pragma unroll
+  for (int a = 0; a < M; ++a) {
+#pragma unroll
+    for (int b = 0; b < M; ++b) {
+      const float t12 = iS1[a] * iS2[b] + fS1[a] * fS2[b] + HALF * (iS1[a] * fS2[b] + iS2[b] * fS1[a]);
+      float acc = ZERO;
+#pragma unroll
+      for (int g = 0; g < M; ++g) {
+        acc = (g == 0 ? ZERO : acc) - Qdt * (THIRD * (fS3[g] - iS3[g]) * t12);
+        if (a <= d1 && b <= d2 && g < d3) J.add(i1_min + a, i2_min + b, i3_min + g, 2, acc);
+      }
+    }
+  }
+}
+
+template <int D, int O, class Sink>
+__device__ __forceinline__ void deposit_particle(Sink& J, int G, const PState& s, float coeff,
+                                                 float inv_dt) {
+  float vz = ZERO;
+  if constexpr (D == 2) vz = vp_out_of_plane(s.u);
+  if constexpr (O == 0) deposit_zigzag<D>(J, G, s, vz, coeff, inv_dt);
+  else if constexpr (D == 2) deposit_esirkepov_2d<O>(J, G, s, vz, coeff, inv_dt);
+  else deposit_esirkepov_3d<O>(J, G, s, coeff, inv_dt);
+}
+
+// Ghost-extended index range [lo, hi] (relative to the final cell i + G)
+// touched by the deposit of a particle that moved at most one cell.
+template <int O>
+__device__ __forceinline__ void deposit_reach(int& lo, int& hi) {
+  if constexpr (O == 0) { lo = -1; hi = 2; }
+  else if constexpr (O == 1) { lo = -2; hi = 2; }
+  else { lo = -3; hi = 3; }
+}
+
+}  // namespace ntt

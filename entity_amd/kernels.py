@@ -1,0 +1,198 @@
+"""Python mirrors of Entity's kernel functors, backed by libntt_hip.
+
+Each class is constructed with the arguments of the reference functor and
+launched by calling it; a launch covers the range the reference's driver
+passes to ``Kokkos::parallel_for`` (active particles, or active cells).
+Launches are asynchronous on the current torch stream; failures raise
+``NttError``.  There is no CPU fallback: without the built HIP library every
+launch raises.
+
+Reference functors (paths relative to the Entity source tree):
+* sr.Pusher_kernel           src/kernels/pushers/sr.hpp:63-332
+* DepositCurrents_kernel     src/kernels/currents_deposit.hpp:33-762
+* mink.Faraday_kernel        src/kernels/faraday_mink.hpp:25-167
+* mink.Ampere_kernel         src/kernels/ampere_mink.hpp:31-89
+* mink.CurrentsAmpere_kernel src/kernels/ampere_mink.hpp:96-215
+* DigitalFilter_kernel       src/kernels/digital_filter.hpp:58-389
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import torch
+
+from . import _abi
+from .containers import Fields, GridShape, Particles, field_abi
+
+
+def _stream(stream=None) -> int | None:
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
+
+
+@dataclass
+class PusherContext:
+    """kernel::sr::PusherContext (src/kernels/pushers/context.h:74-122)."""
+    pusher_flags: int
+    mass: float
+    charge: float
+    dt: float
+    omegaB0: float
+    ni1: int
+    ni2: int
+    ni3: int = 1
+
+    def abi(self) -> _abi.NttSrPusherCtx:
+        return _abi.NttSrPusherCtx(self.mass, self.charge, self.dt, self.omegaB0,
+                                   self.ni1, self.ni2, self.ni3, self.pusher_flags)
+
+
+@dataclass
+class PusherBoundaries:
+    """kernel::sr::PusherBoundaries<D> (context.h:124-178): PrtlBC per face."""
+    bc: list = field(default_factory=lambda: [(_abi.PRTL_BC_PERIODIC,) * 2] * 3)
+
+    def abi(self) -> _abi.NttPrtlBC:
+        s = _abi.NttPrtlBC()
+        for d in range(3):
+            lo, hi = self.bc[d] if d < len(self.bc) else (0, 0)
+            s.bc[d][0], s.bc[d][1] = lo, hi
+        return s
+
+
+@dataclass
+class Minkowski:
+    """metric::Minkowski<D> (src/metrics/minkowski.h:30-84): uniform cell size."""
+    dim: int
+    dx: float = 1.0
+
+    def abi(self) -> _abi.NttMinkowski:
+        return _abi.NttMinkowski(self.dim, self.dx)
+
+
+class _Launch:
+    strict = False
+
+    @property
+    def lib(self):
+        return _abi.load(self.strict)
+
+
+class sr:  # namespace kernel::sr
+    class Pusher_kernel(_Launch):
+        def __init__(self, ctx: PusherContext, bc: PusherBoundaries, particles: Particles,
+                     EB: torch.Tensor, shape: GridShape, metric: Minkowski,
+                     shape_order: int = 0, strict: bool = False):
+            self.ctx, self.bc, self.particles = ctx, bc, particles
+            self.EB, self.shape, self.metric = EB, shape, metric
+            self.shape_order, self.strict = shape_order, strict
+
+        def __call__(self, stream=None) -> None:
+            p = self.particles.abi()
+            f = field_abi(self.EB, self.shape)
+            st = self.lib.ntt_sr_push(C.byref(self.ctx.abi()), C.byref(self.bc.abi()),
+                                      C.byref(p), C.byref(f), C.byref(self.metric.abi()),
+                                      self.shape_order, _stream(stream))
+            _abi.check(st, "ntt_sr_push")
+
+
+class DepositCurrents_kernel(_Launch):
+    def __init__(self, J: torch.Tensor, shape: GridShape, particles: Particles,
+                 metric: Minkowski, charge: float, dt: float, shape_order: int = 0,
+                 strict: bool = False):
+        self.J, self.shape, self.particles, self.metric = J, shape, particles, metric
+        self.charge, self.dt, self.shape_order, self.strict = charge, dt, shape_order, strict
+
+    def __call__(self, stream=None) -> None:
+        p = self.particles.abi()
+        f = field_abi(self.J, self.shape)
+        st = self.lib.ntt_deposit_currents(C.byref(p), C.byref(f), C.byref(self.metric.abi()),
+                                           self.charge, self.dt, self.shape_order,
+                                           _stream(stream))
+        _abi.check(st, "ntt_deposit_currents")
+
+
+class mink:  # namespace kernel::mink
+    class Faraday_kernel(_Launch):
+        def __init__(self, EB: torch.Tensor, shape: GridShape, coeff1: float, coeff2: float,
+                     deltax=0.0, deltay=0.0, betaxy=0.0, betayx=0.0, deltaz=0.0, betaxz=0.0,
+                     betazx=0.0, betayz=0.0, betazy=0.0, strict: bool = False):
+            self.EB, self.shape, self.c1, self.c2, self.strict = EB, shape, coeff1, coeff2, strict
+            self.fd = _abi.NttFdCoeffs(deltax, deltay, deltaz, betaxy, betayx, betaxz, betazx,
+                                       betayz, betazy)
+
+        def __call__(self, stream=None) -> None:
+            f = field_abi(self.EB, self.shape)
+            _abi.check(self.lib.ntt_faraday_mink(C.byref(f), self.c1, self.c2, C.byref(self.fd),
+                                                 _stream(stream)), "ntt_faraday_mink")
+
+    class Ampere_kernel(_Launch):
+        def __init__(self, EB: torch.Tensor, shape: GridShape, coeff1: float, coeff2: float,
+                     strict: bool = False):
+            self.EB, self.shape, self.c1, self.c2, self.strict = EB, shape, coeff1, coeff2, strict
+
+        def __call__(self, stream=None) -> None:
+            f = field_abi(self.EB, self.shape)
+            _abi.check(self.lib.ntt_ampere_mink(C.byref(f), self.c1, self.c2, _stream(stream)),
+                       "ntt_ampere_mink")
+
+    class CurrentsAmpere_kernel(_Launch):
+        def __init__(self, E: torch.Tensor, J: torch.Tensor, shape: GridShape, coeff: float,
+                     ppc0: float, strict: bool = False):
+            self.E, self.J, self.shape = E, J, shape
+            self.coeff, self.ppc0, self.strict = coeff, ppc0, strict
+
+        def __call__(self, stream=None) -> None:
+            e, j = field_abi(self.E, self.shape), field_abi(self.J, self.shape)
+            _abi.check(self.lib.ntt_currents_ampere_mink(C.byref(e), C.byref(j), self.coeff,
+                                                         self.ppc0, _stream(stream)),
+                       "ntt_currents_ampere_mink")
+
+
+class DigitalFilter_kernel(_Launch):
+    def __init__(self, array: torch.Tensor, buffer: torch.Tensor, shape: GridShape,
+                 boundaries=None, strict: bool = False):
+        self.array, self.buffer, self.shape, self.strict = array, buffer, shape, strict
+        self.bc = None
+        if boundaries is not None:
+            self.bc = _abi.NttFldsBC()
+            for d, (lo, hi) in enumerate(boundaries):
+                self.bc.bc[d][0], self.bc.bc[d][1] = lo, hi
+
+    def __call__(self, stream=None) -> None:
+        a, b = field_abi(self.array, self.shape), field_abi(self.buffer, self.shape)
+        bc = C.byref(self.bc) if self.bc is not None else None
+        _abi.check(self.lib.ntt_digital_filter(C.byref(a), C.byref(b), bc, _stream(stream)),
+                   "ntt_digital_filter")
+
+
+def communicate_fields(t: torch.Tensor, shape: GridShape, comp_begin: int, comp_end: int,
+                       stream=None, strict: bool = False) -> None:
+    """Periodic ghost fill (Metadomain::CommunicateFields, single periodic domain)."""
+    f = field_abi(t, shape)
+    _abi.check(_abi.load(strict).ntt_communicate_fields(C.byref(f), comp_begin, comp_end,
+                                                        _stream(stream)),
+               "ntt_communicate_fields")
+
+
+def synchronize_currents(cur: torch.Tensor, buff: torch.Tensor, shape: GridShape,
+                         stream=None, strict: bool = False) -> None:
+    """Periodic current fold (Metadomain::SynchronizeFields(Comm::J))."""
+    j, b = field_abi(cur, shape), field_abi(buff, shape)
+    _abi.check(_abi.load(strict).ntt_synchronize_currents(C.byref(j), C.byref(b),
+                                                          _stream(stream)),
+               "ntt_synchronize_currents")
+
+
+def device_errors(strict: bool = False) -> int:
+    """Synchronize and return/clear the library's device error word."""
+    v = C.c_uint32(0)
+    _abi.check(_abi.load(strict).ntt_get_device_errors(C.byref(v)), "ntt_get_device_errors")
+    return int(v.value)
+
+
+__all__ = ["PusherContext", "PusherBoundaries", "Minkowski", "sr", "mink",
+           "DepositCurrents_kernel", "DigitalFilter_kernel", "communicate_fields",
+           "synchronize_currents", "device_errors", "Fields", "GridShape", "Particles"]

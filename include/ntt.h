@@ -1,0 +1,243 @@
+/*
+ * ntt.h -- C ABI of the MI355X-native Entity PIC hot path (libntt_hip.so).
+ *
+ * Every entry point is stream-ordered and asynchronous: it validates its
+ * arguments on the host, enqueues HIP kernels on `stream` (a hipStream_t, NULL
+ * for the null stream) and returns.  Device-side faults that the reference
+ * turns into `raise::KernelError` (invalid particle tag, a stencil that leaves
+ * the ghost-extended array) set bits of a device error word instead of
+ * aborting; read and clear it with ntt_get_device_errors().
+ *
+ * Pointers inside the structs below are caller-owned DEVICE pointers.  The
+ * library never allocates: scratch memory is passed in explicitly.
+ *
+ * Memory image (the reference's GPU layout, Kokkos LayoutLeft):
+ *   field(i1, i2, i3, c) = data[i1 + N1 * (i2 + N2 * (i3 + N3 * c))]
+ *   with N_d = n[d] + 2 * ngh for d < dim and N_d = 1 otherwise; active cells
+ *   are [ngh, ngh + n[d]) in every simulated direction.
+ *   em components: 0..2 = ex1..ex3 (Yee edges), 3..5 = bx1..bx3 (Yee faces);
+ *   cur components: 0..2 = jx1..jx3.
+ * Particles are a structure of arrays with an active prefix [0, npart).
+ *
+ * Reference interfaces replaced (paths relative to the Entity source tree):
+ *   ntt_particles_t         ntt::ParticleArrays / Particles<D,C>
+ *                           src/framework/containers/particles.h:47-71,78-320
+ *   ntt_field_t             ndfield_t<D,N> (Kokkos::View<real_t*** [N]>)
+ *                           src/global/arch/kokkos_aliases.h:85-109,
+ *                           src/framework/containers/fields.h:54-82
+ *   ntt_sr_pusher_ctx_t     kernel::sr::PusherContext
+ *                           src/kernels/pushers/context.h:74-122
+ *   ntt_prtl_bc_t           kernel::sr::PusherBoundaries<D>
+ *                           src/kernels/pushers/context.h:124-178
+ *   ntt_minkowski_t         metric::Minkowski<D> (dx, dx_inv)
+ *                           src/metrics/minkowski.h:30-84
+ */
+#ifndef NTT_H
+#define NTT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NTT_ABI_VERSION 1
+
+#if defined(__GNUC__) || defined(__clang__)
+#define NTT_API __attribute__((visibility("default")))
+#else
+#define NTT_API
+#endif
+
+typedef enum ntt_status {
+  NTT_OK              = 0,
+  NTT_ERR_ARG         = 1, /* invalid argument (null pointer, bad size, ...)  */
+  NTT_ERR_HIP         = 2, /* a HIP runtime call failed                       */
+  NTT_ERR_DEVICE      = 3, /* device error word is non-zero                   */
+  NTT_ERR_UNSUPPORTED = 4, /* valid request outside what this build supports  */
+  NTT_ERR_CAPACITY    = 5  /* workspace or capacity too small                 */
+} ntt_status;
+
+/* device error word bits (ntt_get_device_errors) */
+#define NTT_DEVERR_INVALID_TAG   0x1u /* tag neither alive nor dead (sr.hpp:118-122) */
+#define NTT_DEVERR_OUT_OF_BOUNDS 0x2u /* gather/deposit stencil outside the array     */
+#define NTT_DEVERR_PUSHER_FLAGS  0x4u /* invalid pusher algorithm flags               */
+
+/* ParticleTag (src/global/global.h:168-171) */
+#define NTT_TAG_DEAD  0
+#define NTT_TAG_ALIVE 1
+
+/* ParticlePusher flags (src/global/enums.h:317-324) */
+#define NTT_PUSHER_NONE   0u
+#define NTT_PUSHER_PHOTON (1u << 0)
+#define NTT_PUSHER_BORIS  (1u << 1)
+#define NTT_PUSHER_VAY    (1u << 2)
+#define NTT_PUSHER_GCA    (1u << 3)
+
+/* PrtlBC values (src/global/enums.h:185-195) */
+#define NTT_PRTL_BC_PERIODIC   1u
+#define NTT_PRTL_BC_ABSORB     2u
+#define NTT_PRTL_BC_ATMOSPHERE 3u
+#define NTT_PRTL_BC_REFLECT    5u
+#define NTT_PRTL_BC_HORIZON    6u
+#define NTT_PRTL_BC_AXIS       7u
+
+/* FldsBC values (src/global/enums.h:215-226) */
+#define NTT_FLDS_BC_PERIODIC  1u
+#define NTT_FLDS_BC_CONDUCTOR 8u
+
+typedef struct ntt_particles {
+  int32_t* i1;
+  int32_t* i2;
+  int32_t* i3;
+  int32_t* i1_prev;
+  int32_t* i2_prev;
+  int32_t* i3_prev;
+  float*   dx1;
+  float*   dx2;
+  float*   dx3;
+  float*   dx1_prev;
+  float*   dx2_prev;
+  float*   dx3_prev;
+  float*   ux1;
+  float*   ux2;
+  float*   ux3;
+  float*   weight;
+  float*   phi;   /* 2D axisymmetric (GR) only; may be NULL otherwise       */
+  int16_t* tag;
+  int64_t* pld_i; /* optional integer payload carried through sorts, or NULL */
+  int64_t  npart;
+  int64_t  maxnpart;
+} ntt_particles_t;
+
+typedef struct ntt_field {
+  float*  data;
+  int32_t dim;   /* 2 or 3                                         */
+  int32_t n[3];  /* active cells; n[d] = 1 for d >= dim            */
+  int32_t ngh;   /* ghost layers per side (2 for shape order <= 2) */
+  int32_t ncomp; /* 6 (em) or 3 (cur)                              */
+} ntt_field_t;
+
+typedef struct ntt_sr_pusher_ctx {
+  float   mass;
+  float   charge;
+  float   dt;
+  float   omegaB0;
+  int32_t ni1, ni2, ni3;
+  uint8_t pusher_flags; /* NTT_PUSHER_* */
+} ntt_sr_pusher_ctx_t;
+
+typedef struct ntt_prtl_bc {
+  uint8_t bc[3][2]; /* [direction][0 = min, 1 = max], NTT_PRTL_BC_* */
+} ntt_prtl_bc_t;
+
+typedef struct ntt_flds_bc {
+  uint8_t bc[3][2]; /* NTT_FLDS_BC_*; this build filters the periodic interior */
+} ntt_flds_bc_t;
+
+typedef struct ntt_minkowski {
+  int32_t dim;
+  float   dx; /* uniform cell size: (x1_max - x1_min) / nx1 */
+} ntt_minkowski_t;
+
+/* Extended-stencil Faraday coefficients (algorithms.fieldsolver.*,
+ * src/global/defaults.h:29-41; all zero = standard Yee). */
+typedef struct ntt_fd_coeffs {
+  float delta_x, delta_y, delta_z;
+  float beta_xy, beta_yx, beta_xz, beta_zx, beta_yz, beta_zy;
+} ntt_fd_coeffs_t;
+
+/* ------------------------------------------------------------------ misc */
+
+/* library ABI version (NTT_ABI_VERSION) */
+NTT_API int32_t ntt_abi_version(void);
+
+/* human-readable name of a status code */
+NTT_API const char* ntt_status_string(ntt_status status);
+
+/* Waits for the device, returns the error word (NTT_DEVERR_*) of the current
+ * device in *flags and clears it.  No reference counterpart: it replaces the
+ * Kokkos::abort of raise::KernelError (src/global/utils/error.h:101-107). */
+NTT_API ntt_status ntt_get_device_errors(uint32_t* flags);
+
+/* --------------------------------------------------- kernel-level entries */
+
+/* One launch of the SR pusher over [0, npart).
+ * Replaces kernel::sr::Pusher_kernel<Minkowski<D>, PusherPolicy<M>> with the
+ * no-policy path (src/kernels/pushers/sr.hpp:63-332; launch site
+ * src/engines/srpic/particle_pusher.h:152-160).  shape_order is the
+ * compile-time SHAPE_ORDER of the reference (0 = linear "direct"
+ * interpolation, 1, 2 = B-spline shapes). */
+NTT_API ntt_status ntt_sr_push(const ntt_sr_pusher_ctx_t* ctx,
+                       const ntt_prtl_bc_t*       bc,
+                       const ntt_particles_t*     prtls,
+                       const ntt_field_t*         em,
+                       const ntt_minkowski_t*     metric,
+                       int32_t                    shape_order,
+                       void*                      stream);
+
+/* Charge-conserving current deposit, accumulated (atomically) into `cur`
+ * including its ghost cells, exactly where the reference writes.
+ * Replaces kernel::DepositCurrents_kernel<SRPIC, Minkowski<D>, O>
+ * (src/kernels/currents_deposit.hpp:33-762): O = 0 zig-zag, 1..2 Esirkepov.
+ * The caller zeroes `cur` first, as srpic::CurrentsDeposit does
+ * (src/engines/srpic/currents.h:68). */
+NTT_API ntt_status ntt_deposit_currents(const ntt_particles_t* prtls,
+                                ntt_field_t*           cur,
+                                const ntt_minkowski_t* metric,
+                                float                  charge,
+                                float                  dt,
+                                int32_t                shape_order,
+                                void*                  stream);
+
+/* B += coeff * curl E over active cells (in place on em).
+ * Replaces kernel::mink::Faraday_kernel<D> (src/kernels/faraday_mink.hpp:25-167).
+ * fd may be NULL (standard Yee). */
+NTT_API ntt_status ntt_faraday_mink(ntt_field_t*           em,
+                            float                  coeff1,
+                            float                  coeff2,
+                            const ntt_fd_coeffs_t* fd,
+                            void*                  stream);
+
+/* E += coeff * curl B over active cells.
+ * Replaces kernel::mink::Ampere_kernel<D> (src/kernels/ampere_mink.hpp:31-89). */
+NTT_API ntt_status ntt_ampere_mink(ntt_field_t* em, float coeff1, float coeff2, void* stream);
+
+/* E += coeff * J, then J /= ppc0, over active cells.
+ * Replaces kernel::mink::CurrentsAmpere_kernel<D> (no external current)
+ * (src/kernels/ampere_mink.hpp:96-215). */
+NTT_API ntt_status ntt_currents_ampere_mink(ntt_field_t* em,
+                                    ntt_field_t* cur,
+                                    float        coeff,
+                                    float        ppc0,
+                                    void*        stream);
+
+/* One binomial pass array = K * buffer over active cells (periodic interior
+ * stencil; 3D keeps the reference's stencil as written).
+ * Replaces kernel::DigitalFilter_kernel<D, Cartesian>
+ * (src/kernels/digital_filter.hpp:58-389). bc may be NULL (periodic). */
+NTT_API ntt_status ntt_digital_filter(ntt_field_t*         array,
+                              const ntt_field_t*   buffer,
+                              const ntt_flds_bc_t* bc,
+                              void*                stream);
+
+/* Periodic ghost fill (copy the active edges into the ghost layers) of
+ * components [comp_begin, comp_end).  Replaces the single-domain periodic path
+ * of Metadomain::CommunicateFields (src/framework/domain/metadomain_comm.cpp:204-367,
+ * comm_nompi.hpp:28-63). */
+NTT_API ntt_status ntt_communicate_fields(ntt_field_t* fld,
+                                  int32_t      comp_begin,
+                                  int32_t      comp_end,
+                                  void*        stream);
+
+/* Periodic fold of ghost-deposited currents into their images (additive),
+ * using `buff` as the receive buffer.  Replaces Metadomain::SynchronizeFields
+ * for Comm::J (src/framework/domain/metadomain_comm.cpp:369-562). */
+NTT_API ntt_status ntt_synchronize_currents(ntt_field_t* cur, ntt_field_t* buff, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NTT_H */

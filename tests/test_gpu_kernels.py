@@ -1,0 +1,230 @@
+"""GPU parity of the kernel-level C ABI against the oracle.
+
+Inputs are seeded random fields (periodic ghosts) and particles, including
+particles sitting next to cell faces and the domain boundary, dead particles
+and particles that cross cells.
+
+* Strict-FP build (libntt_hip_strict.so, -ffp-contract=off): particle state
+  after the push is bit-identical to the oracle (integers and floats), and the
+  field kernels are bit-identical.
+* Product build (libntt_hip.so, FMA contraction): integer state (cell
+  indices, tags, prev indices) is identical except for particles whose offset
+  lies within 4 ulp of a cell face after the push, which are counted and
+  bounded; floats agree to a relative 2e-6 (FP32).
+* Deposits are compared with a tolerance of 2e-6 * max|J|, since atomic
+  accumulation order is not fixed (as with the reference's ScatterView).
+"""
+import numpy as np
+import pytest
+import torch
+
+from _helpers import G, field_array
+
+pytestmark = pytest.mark.gpu
+
+BORIS, VAY, PHOTON = 2, 4, 1
+ALGS = {"boris": BORIS, "vay": VAY, "photon": PHOTON}
+
+
+def make_fields(oracle, dim, n, seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    em = (rng.standard_normal(field_array(dim, n, 6).shape) * scale).astype(np.float32)
+    oracle.communicate(em, 0, 6, dim=dim)
+    return em
+
+
+def make_particles(dim, n, npart, seed, umax=1.5, dead_frac=0.05):
+    rng = np.random.default_rng(seed)
+    p = {}
+    for d in range(dim):
+        i = rng.integers(0, n[d], npart).astype(np.int32)
+        dx = rng.random(npart, dtype=np.float32)
+        # a slice of particles right next to cell faces and to the box edges
+        k = npart // 8
+        dx[:k] = np.float32(1.0) - np.float32(1e-7) * rng.integers(1, 4, k)
+        dx[k:2 * k] = np.float32(1e-7) * rng.integers(0, 4, k)
+        i[2 * k:3 * k] = rng.choice([0, n[d] - 1], k)
+        p[f"i{d+1}"] = i
+        p[f"dx{d+1}"] = dx
+        p[f"i{d+1}_prev"] = i.copy()
+        p[f"dx{d+1}_prev"] = dx.copy()
+    for d in range(3):
+        p[f"ux{d+1}"] = (rng.standard_normal(npart) * umax).astype(np.float32)
+    p["weight"] = rng.uniform(0.5, 1.5, npart).astype(np.float32)
+    tag = np.ones(npart, np.int16)
+    tag[rng.random(npart) < dead_frac] = 0
+    p["tag"] = tag
+    return p
+
+
+def to_device(p, dim):
+    from entity_amd.containers import Particles
+    npart = len(p["tag"])
+    prt = Particles(dim, npart, device="cuda")
+    prt.load(**{k: torch.from_numpy(np.ascontiguousarray(v)) for k, v in p.items()})
+    return prt
+
+
+def near_face(dx, ulps=4):
+    eps = np.float32(ulps) * np.finfo(np.float32).eps
+    return (dx < eps) | (dx > np.float32(1.0) - eps)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("order", [0, 1, 2])
+@pytest.mark.parametrize("alg", ["boris", "vay", "photon"])
+@pytest.mark.parametrize("strict", [True, False])
+def test_push_parity(hip, oracle, dim, order, alg, strict):
+    from entity_amd.containers import GridShape
+    from entity_amd.kernels import Minkowski, PusherBoundaries, PusherContext, device_errors, sr
+    if alg == "photon" and order != 0:
+        pytest.skip("photons do not interpolate fields")
+    n = (13, 10, 9)[:dim]
+    em_h = make_fields(oracle, dim, n, seed=dim * 10 + order)
+    p = make_particles(dim, n, 4096 if dim == 2 else 2048, seed=order + 7 * dim)
+    if alg == "photon":
+        p["ux1"][p["ux1"] == 0] = 1.0
+    dt = float(np.float32(0.45 / np.sqrt(dim)))
+    mass, charge, omegaB0 = 1.0, -1.0, 0.7
+    shape = GridShape(dim, n)
+    prt = to_device(p, dim)
+    em = torch.from_numpy(em_h).cuda()
+    ctx = PusherContext(ALGS[alg], mass, charge, dt, omegaB0, *n, *([1] * (3 - dim)))
+    sr.Pusher_kernel(ctx, PusherBoundaries(), prt, em, shape, Minkowski(dim), order,
+                     strict=strict)()
+    torch.cuda.synchronize()
+    assert device_errors(strict) == 0
+    out = prt.to_host()
+    q, rc = oracle.sr_push(p, em_h.copy(), dim=dim, mass=mass, charge=charge, dt=dt,
+                           omegaB0=omegaB0, pusher=ALGS[alg], order=order)
+    assert rc == 0
+    ints = [f"i{d+1}" for d in range(dim)] + [f"i{d+1}_prev" for d in range(dim)] + ["tag"]
+    floats = [f"dx{d+1}" for d in range(dim)] + [f"dx{d+1}_prev" for d in range(dim)] + \
+        ["ux1", "ux2", "ux3"]
+    if strict:
+        for k in ints + floats:
+            np.testing.assert_array_equal(out[k], q[k], err_msg=k)
+        return
+    # product build: near-face particles may take the other side of a face
+    flagged = np.zeros(len(q["tag"]), bool)
+    for d in range(dim):
+        flagged |= near_face(q[f"dx{d+1}"]) | near_face(out[f"dx{d+1}"])
+    mism = np.zeros_like(flagged)
+    for k in ints:
+        mism |= out[k] != q[k]
+    assert not (mism & ~flagged).any(), "integer mismatch away from cell faces"
+    assert mism.sum() <= max(8, len(mism) // 200), f"{mism.sum()} near-face crossings differ"
+    ok = ~mism
+    for k in floats:
+        np.testing.assert_allclose(out[k][ok], q[k][ok], rtol=2e-6, atol=2e-6, err_msg=k)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("order", [0, 1, 2])
+@pytest.mark.parametrize("strict", [True, False])
+def test_deposit_parity(hip, oracle, dim, order, strict):
+    from entity_amd.containers import GridShape
+    from entity_amd.kernels import DepositCurrents_kernel, Minkowski, device_errors
+    n = (13, 10, 9)[:dim]
+    rng = np.random.default_rng(100 + order + dim)
+    p = make_particles(dim, n, 8192 if dim == 2 else 4096, seed=3 * order + dim)
+    # previous state: one move of at most 0.45 cells, with periodic index shift
+    for d in range(dim):
+        disp = rng.uniform(-0.45, 0.45, len(p["tag"])).astype(np.float32)
+        x = p[f"dx{d+1}"] + disp
+        ip = p[f"i{d+1}"] + np.floor(x).astype(np.int32)
+        p[f"i{d+1}_prev"] = ip.astype(np.int32)
+        p[f"dx{d+1}_prev"] = (x - np.floor(x)).astype(np.float32)
+    dt = float(np.float32(0.4))
+    shape = GridShape(dim, n)
+    J = torch.zeros(shape.tensor_shape(3), device="cuda")
+    prt = to_device(p, dim)
+    DepositCurrents_kernel(J, shape, prt, Minkowski(dim), -1.0, dt, order, strict=strict)()
+    torch.cuda.synchronize()
+    assert device_errors(strict) == 0
+    J_ref = field_array(dim, n, 3)
+    assert oracle.deposit(p, J_ref, dim=dim, charge=-1.0, dt=dt, order=order) == 0
+    Jg = J.cpu().numpy()
+    scale = np.abs(J_ref).max()
+    assert scale > 0
+    np.testing.assert_allclose(Jg, J_ref, rtol=0, atol=2e-6 * scale * np.sqrt(len(p["tag"])) / 10)
+    # the totals agree tightly
+    np.testing.assert_allclose(Jg.astype(np.float64).sum(axis=tuple(range(1, dim + 1))),
+                               J_ref.astype(np.float64).sum(axis=tuple(range(1, dim + 1))),
+                               rtol=1e-4, atol=1e-3 * scale)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("strict", [True, False])
+@pytest.mark.parametrize("extended", [False, True])
+def test_fields_parity(hip, oracle, dim, strict, extended):
+    from entity_amd.containers import GridShape
+    from entity_amd.kernels import (DigitalFilter_kernel, communicate_fields, mink,
+                                    synchronize_currents)
+    n = (17, 12, 10)[:dim]
+    shape = GridShape(dim, n)
+    rng = np.random.default_rng(dim + 2 * extended)
+    em_h = rng.standard_normal(field_array(dim, n, 6).shape).astype(np.float32)
+    J_h = rng.standard_normal(field_array(dim, n, 3).shape).astype(np.float32)
+    em = torch.from_numpy(em_h.copy()).cuda()
+    J = torch.from_numpy(J_h.copy()).cuda()
+    buf = torch.zeros_like(J)
+    fdd = dict(delta_x=0.05, delta_y=-0.03, delta_z=0.02, beta_xy=0.01, beta_yx=0.04,
+               beta_xz=-0.02, beta_zx=0.03, beta_yz=0.015, beta_zy=-0.01) if extended else {}
+    c1, c2 = 0.21, (0.35 if dim == 2 else 0.0)
+    mink.Faraday_kernel(em, shape, c1, c2, strict=strict, **{
+        {"delta_x": "deltax", "delta_y": "deltay", "delta_z": "deltaz", "beta_xy": "betaxy",
+         "beta_yx": "betayx", "beta_xz": "betaxz", "beta_zx": "betazx", "beta_yz": "betayz",
+         "beta_zy": "betazy"}[k]: v for k, v in fdd.items()})()
+    oracle.faraday(em_h, dim=dim, coeff1=c1, coeff2=c2, fd=fdd or None)
+    mink.Ampere_kernel(em, shape, 0.3, 0.45, strict=strict)()
+    oracle.ampere(em_h, dim=dim, coeff1=0.3, coeff2=0.45)
+    communicate_fields(em, shape, 0, 6, strict=strict)
+    oracle.communicate(em_h, 0, 6, dim=dim)
+    synchronize_currents(J, buf, shape, strict=strict)
+    oracle.synchronize(J_h, np.zeros_like(J_h), dim=dim)
+    communicate_fields(J, shape, 0, 3, strict=strict)
+    oracle.communicate(J_h, 0, 3, dim=dim)
+    buf.copy_(J)
+    DigitalFilter_kernel(J, buf, shape, strict=strict)()
+    oracle.filter_pass(J_h, J_h.copy(), dim=dim)
+    mink.CurrentsAmpere_kernel(em, J, shape, -0.125, 16.0, strict=strict)()
+    oracle.currents_ampere(em_h, J_h, dim=dim, coeff=-0.125, ppc0=16.0)
+    torch.cuda.synchronize()
+    emg, Jg = em.cpu().numpy(), J.cpu().numpy()
+    if strict:
+        np.testing.assert_array_equal(emg, em_h)
+        np.testing.assert_array_equal(Jg, J_h)
+    else:
+        np.testing.assert_allclose(emg, em_h, rtol=1e-5, atol=1e-5)
+        np.testing.assert_allclose(Jg, J_h, rtol=1e-5, atol=1e-5)
+
+
+def test_invalid_tag_sets_device_error(hip, oracle):
+    from entity_amd.containers import GridShape
+    from entity_amd.kernels import Minkowski, PusherBoundaries, PusherContext, device_errors, sr
+    n = (8, 8)
+    em = torch.zeros(GridShape(2, n).tensor_shape(6), device="cuda")
+    p = make_particles(2, n, 64, seed=1, dead_frac=0)
+    p["tag"][5] = 3
+    prt = to_device(p, 2)
+    ctx = PusherContext(BORIS, 1.0, 1.0, 0.3, 1.0, 8, 8)
+    sr.Pusher_kernel(ctx, PusherBoundaries(), prt, em, GridShape(2, n), Minkowski(2), 1)()
+    from entity_amd import _abi
+    assert device_errors() & _abi.DEVERR_INVALID_TAG
+    assert device_errors() == 0  # cleared
+
+
+def test_bad_arguments_rejected(hip):
+    from entity_amd import _abi
+    from entity_amd.containers import GridShape
+    from entity_amd.kernels import Minkowski, PusherBoundaries, PusherContext, sr
+    n = (8, 8)
+    em = torch.zeros(GridShape(2, n).tensor_shape(6), device="cuda")
+    prt = to_device(make_particles(2, n, 16, seed=2), 2)
+    ctx = PusherContext(BORIS, 1.0, 1.0, 0.3, 1.0, 9, 8)  # ni1 disagrees with the field
+    with pytest.raises(_abi.NttError):
+        sr.Pusher_kernel(ctx, PusherBoundaries(), prt, em, GridShape(2, n), Minkowski(2), 1)()
+    ctx = PusherContext(BORIS, 1.0, 1.0, 0.3, 1.0, 8, 8)
+    with pytest.raises(_abi.NttError):
+        sr.Pusher_kernel(ctx, PusherBoundaries(), prt, em, GridShape(2, n), Minkowski(2), 5)()
