@@ -1,0 +1,140 @@
+// engine.hip -- host-side driver of one SRPIC step for a periodic box (the
+// counterpart of SRPICEngine::step_forward, src/engines/srpic/srpic.hpp:65-188,
+// with the drivers of src/engines/srpic/{fieldsolvers.h, currents.h,
+// particle_pusher.h}).  All launches are enqueued on one stream.
+#include <cmath>
+
+#include "internal.hpp"
+
+using namespace ntt;
+
+namespace {
+
+bool same_grid(const FieldView& a, const FieldView& b) {
+  return a.n1 == b.n1 && a.n2 == b.n2 && a.n3 == b.n3 && a.ngh == b.ngh && a.N3 == b.N3;
+}
+
+bool fd_nonzero(const ntt_fd_coeffs_t& f) {
+  return f.delta_x != 0 || f.delta_y != 0 || f.delta_z != 0 || f.beta_xy != 0 ||
+         f.beta_yx != 0 || f.beta_xz != 0 || f.beta_zx != 0 || f.beta_yz != 0 || f.beta_zy != 0;
+}
+
+}  // namespace
+
+extern "C" ntt_status ntt_faraday_mink(ntt_field_t*, float, float, const ntt_fd_coeffs_t*, void*);
+
+extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t* buff,
+                                  ntt_species_t* species, int32_t nspecies,
+                                  const ntt_sr_step_params_t* prm, const ntt_minkowski_t* metric,
+                                  void* stream) {
+  if (prm == nullptr || metric == nullptr || (nspecies > 0 && species == nullptr) || nspecies < 0)
+    return NTT_ERR_ARG;
+  FieldView E, J, Bf;
+  ntt_status s = make_field_view(em, 6, &E);
+  if (s != NTT_OK) return s;
+  s = make_field_view(cur, 3, &J);
+  if (s != NTT_OK) return s;
+  s = make_field_view(buff, 3, &Bf);
+  if (s != NTT_OK) return s;
+  if (!same_grid(E, J) || !same_grid(E, Bf) || J.d == Bf.d) return NTT_ERR_ARG;
+  const int D = em->dim;
+  if (metric->dim != D || !(metric->dx > 0.0f) || !(prm->dt > 0.0f)) return NTT_ERR_ARG;
+  if (E.ngh != 2) return NTT_ERR_UNSUPPORTED;
+  const int na[3] = { E.n1, E.n2, E.n3 };
+  for (int d = 0; d < D; ++d) {
+    if (na[d] < 2 * E.ngh) return NTT_ERR_ARG;
+    if (prm->prtl_bc.bc[d][0] != NTT_PRTL_BC_PERIODIC || prm->prtl_bc.bc[d][1] != NTT_PRTL_BC_PERIODIC)
+      return NTT_ERR_UNSUPPORTED;
+  }
+  if (prm->shape_order < 0 || prm->shape_order > 2 || prm->n_filters < 0) return NTT_ERR_ARG;
+  hipStream_t st = as_stream(stream);
+  const float dt = prm->dt;
+  // dx = sqrt(h_11) as the drivers compute it (fieldsolvers.h:50,110)
+  const float dxh = sqrtf(metric->dx * metric->dx);
+  const bool  ext = fd_nonzero(prm->fd);
+
+  // validate every species before enqueueing anything
+  struct Prep {
+    ParticleView pv;
+    PrtlBCs      bcs;
+    float        ndth;
+    int          alg;
+    bool         run, dep;
+  };
+  Prep prep[64];
+  if (nspecies > 64) return NTT_ERR_UNSUPPORTED;
+  for (int sp = 0; sp < nspecies; ++sp) {
+    Prep& p = prep[sp];
+    p.run = species[sp].pusher_flags != NTT_PUSHER_NONE && species[sp].prtls.npart > 0;
+    p.dep = p.run && prm->deposit_enable && species[sp].charge != 0.0f;
+    if (!p.run) continue;
+    ntt_sr_pusher_ctx_t ctx{ species[sp].mass, species[sp].charge, dt, prm->omegaB0,
+                             E.n1, D >= 2 ? E.n2 : 1, D >= 3 ? E.n3 : 1,
+                             species[sp].pusher_flags };
+    FieldView fv;
+    s = prepare_push(&ctx, &prm->prtl_bc, &species[sp].prtls, em, metric, prm->shape_order,
+                     &p.alg, &p.pv, &fv, &p.bcs, &p.ndth);
+    if (s != NTT_OK) return s;
+  }
+
+  // srpic.hpp:71-82: fill E and B ghosts at step 0
+  if (prm->step == 0) launch_ghost_fill(E, 0, 6, st);
+
+  auto faraday_half = [&]() -> ntt_status {
+    const float dT = HALF * prm->correction * dt;  // fieldsolvers.h:44-47
+    float c1, c2;
+    if (D == 2) { c1 = dT / (dxh * dxh); c2 = dT; }
+    else { c1 = dT / dxh; c2 = ZERO; }
+    if (!ext) {
+      launch_faraday_periodic(E, c1, c2, st);
+      return check_launch();
+    }
+    ntt_status r = ntt_faraday_mink(em, c1, c2, &prm->fd, stream);
+    if (r != NTT_OK) return r;
+    launch_ghost_fill(E, 3, 6, st);
+    return check_launch();
+  };
+
+  if (prm->fieldsolver_enable) {
+    s = faraday_half();
+    if (s != NTT_OK) return s;
+  }
+  // currents.h:68: J = 0 (a fill kernel rather than a memset, so that the
+  // step can be captured in a graph)
+  if (prm->deposit_enable) launch_zero(J.d, J.cstride * 3, st);
+  for (int sp = 0; sp < nspecies; ++sp) {
+    const Prep& p = prep[sp];
+    if (!p.run) continue;
+    s = launch_push_deposit_periodic(p.pv, E, J, D, prm->shape_order, p.alg, p.ndth, dt,
+                                     metric->dx, species[sp].charge, p.dep, st);
+    if (s != NTT_OK) return s;
+  }
+  if (prm->deposit_enable) {
+    // currents.h:89-119: n filter passes (ping-pong between cur and buff)
+    if (prm->n_filters == 0) {
+      launch_ghost_fill(J, 0, 3, st);
+    } else {
+      const FieldView* src = &J;
+      const FieldView* dst = &Bf;
+      for (int f = 0; f < prm->n_filters; ++f) {
+        launch_filter_periodic(*dst, *src, st);
+        const FieldView* t = src;
+        src = dst;
+        dst = t;
+      }
+      if (src != &J) launch_copy(J.d, Bf.d, J.cstride * 3, st);
+    }
+  }
+  if (prm->fieldsolver_enable) {
+    s = faraday_half();
+    if (s != NTT_OK) return s;
+    const float dT = ONE * prm->correction * dt;
+    float c1, c2;
+    if (D == 2) { c1 = dT / (dxh * dxh); c2 = dT; }
+    else { c1 = dT / dxh; c2 = ZERO; }
+    // fieldsolvers.h:157: coeff = -dt * q0 / (B0 * V0)
+    const float coeff = -dt * prm->q0 / (prm->B0 * prm->V0);
+    launch_ampere_currents_periodic(E, J, c1, c2, coeff, prm->ppc0, prm->deposit_enable != 0, st);
+  }
+  return check_launch();
+}

@@ -256,6 +256,139 @@ __global__ __launch_bounds__(BX* BY) void k_fold(FieldView J) {
   }
 }
 
+// ---- periodic-box variants used by the engine step ---------------------------
+// Neighbours are read at their periodic image inside the active region, so the
+// kernels do not depend on ghost cells; every thread writes its active cell
+// and all of that cell's ghost copies, so the array leaves the kernel with
+// consistent periodic ghosts (what the reference obtains with
+// CommunicateFields afterwards).  The arithmetic is that of the kernels above.
+struct PIdx {
+  // active-image index of x + off (x ghost-extended active index)
+  int G, n;
+  __device__ __forceinline__ int at(int x, int off) const {
+    int y = x - G + off;
+    y += (y < 0) ? n : 0;
+    y -= (y >= n) ? n : 0;
+    return y + G;
+  }
+};
+
+// writes v at (i, j, k) and at every ghost copy of that active cell
+__device__ __forceinline__ void store_with_ghosts(const FieldView& F, int i, int j, int k, int c,
+                                                  float v) {
+  const int G = F.ngh;
+  const int xi[2] = { i, (i - G < G) ? i + F.n1 : ((i - G >= F.n1 - G) ? i - F.n1 : -1) };
+  const int xj[2] = { j, (j - G < G) ? j + F.n2 : ((j - G >= F.n2 - G) ? j - F.n2 : -1) };
+  int       xk[2] = { k, -1 };
+  if (F.N3 > 1) xk[1] = (k - G < G) ? k + F.n3 : ((k - G >= F.n3 - G) ? k - F.n3 : -1);
+  for (int a = 0; a < 2; ++a) {
+    if (xi[a] < 0) continue;
+    for (int b = 0; b < 2; ++b) {
+      if (xj[b] < 0) continue;
+      for (int g = 0; g < 2; ++g) {
+        if (xk[g] < 0) continue;
+        F(xi[a], xj[b], xk[g], c) = v;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(BX* BY) void k_faraday_periodic(FieldView F, float c1, float c2) {
+  const CellIdx c = active_cell(F);
+  if (!c.ok) return;
+  const PIdx X{ F.ngh, F.n1 }, Y{ F.ngh, F.n2 }, Z{ F.ngh, F.n3 };
+  const int i = c.i, j = c.j, k = c.k;
+  const int ip = X.at(i, 1), jp = Y.at(j, 1);
+  if (F.N3 == 1) {
+    const float b1 = F(i, j, 0, 3) + c1 * (-(F(i, jp, 0, 2) - F(i, j, 0, 2)));
+    const float b2 = F(i, j, 0, 4) + c1 * (+(F(ip, j, 0, 2) - F(i, j, 0, 2)));
+    const float b3 = F(i, j, 0, 5) + c2 * (+(F(i, jp, 0, 0) - F(i, j, 0, 0)) - (F(ip, j, 0, 1) - F(i, j, 0, 1)));
+    store_with_ghosts(F, i, j, 0, 3, b1);
+    store_with_ghosts(F, i, j, 0, 4, b2);
+    store_with_ghosts(F, i, j, 0, 5, b3);
+  } else {
+    const int kp = Z.at(k, 1);
+    const float b1 = F(i, j, k, 3) + c1 * (+(F(i, j, kp, 1) - F(i, j, k, 1)) - (F(i, jp, k, 2) - F(i, j, k, 2)));
+    const float b2 = F(i, j, k, 4) + c1 * (+(F(ip, j, k, 2) - F(i, j, k, 2)) - (F(i, j, kp, 0) - F(i, j, k, 0)));
+    const float b3 = F(i, j, k, 5) + c1 * (+(F(i, jp, k, 0) - F(i, j, k, 0)) - (F(ip, j, k, 1) - F(i, j, k, 1)));
+    store_with_ghosts(F, i, j, k, 3, b1);
+    store_with_ghosts(F, i, j, k, 4, b2);
+    store_with_ghosts(F, i, j, k, 5, b3);
+  }
+}
+
+// Ampere (E += coeff * curl B) followed by the current term (E += J * coeff,
+// J /= ppc0) in one pass; with_j = false skips the current term.
+__global__ __launch_bounds__(BX* BY) void k_ampere_currents_periodic(FieldView F, FieldView J,
+                                                                     float c1, float c2,
+                                                                     float coeff, float ppc0,
+                                                                     int with_j) {
+  const CellIdx c = active_cell(F);
+  if (!c.ok) return;
+  const PIdx X{ F.ngh, F.n1 }, Y{ F.ngh, F.n2 }, Z{ F.ngh, F.n3 };
+  const int i = c.i, j = c.j, k = c.k;
+  const int im = X.at(i, -1), jm = Y.at(j, -1);
+  float e[3];
+  if (F.N3 == 1) {
+    e[0] = F(i, j, 0, 0) + c1 * (F(i, j, 0, 5) - F(i, jm, 0, 5));
+    e[1] = F(i, j, 0, 1) + c1 * (F(im, j, 0, 5) - F(i, j, 0, 5));
+    e[2] = F(i, j, 0, 2) + c2 * (F(i, jm, 0, 3) - F(i, j, 0, 3) + F(i, j, 0, 4) - F(im, j, 0, 4));
+  } else {
+    const int km = Z.at(k, -1);
+    e[0] = F(i, j, k, 0) + c1 * (F(i, j, km, 4) - F(i, j, k, 4) + F(i, j, k, 5) - F(i, jm, k, 5));
+    e[1] = F(i, j, k, 1) + c1 * (F(im, j, k, 5) - F(i, j, k, 5) + F(i, j, k, 3) - F(i, j, km, 3));
+    e[2] = F(i, j, k, 2) + c1 * (F(i, jm, k, 3) - F(i, j, k, 3) + F(i, j, k, 4) - F(im, j, k, 4));
+  }
+  for (int comp = 0; comp < 3; ++comp) {
+    if (with_j) {
+      const float jv = J(i, j, k, comp);
+      e[comp] += jv * coeff;
+      store_with_ghosts(J, i, j, k, comp, jv / ppc0);
+    }
+    store_with_ghosts(F, i, j, k, comp, e[comp]);
+  }
+}
+
+// one filter pass dst = K * src (periodic reads), 2D and 3D
+__global__ __launch_bounds__(BX* BY) void k_filter_periodic(FieldView Dst, FieldView Src) {
+  const CellIdx c = active_cell(Dst);
+  if (!c.ok) return;
+  const PIdx X{ Src.ngh, Src.n1 }, Y{ Src.ngh, Src.n2 }, Z{ Src.ngh, Src.n3 };
+  const int i = c.i, j = c.j, k = c.k;
+  const int im = X.at(i, -1), ip = X.at(i, 1), jm = Y.at(j, -1), jp = Y.at(j, 1);
+  for (int comp = 0; comp < 3; ++comp) {
+    float v;
+    if (Dst.N3 == 1) {
+      auto b = [&](int x, int y) -> float { return Src(x, y, 0, comp); };
+      v = INV_4 * b(i, j) + INV_8 * (b(im, j) + b(ip, j) + b(i, jm) + b(i, jp)) +
+          INV_16 * (b(im, jm) + b(ip, jp) + b(im, jp) + b(ip, jm));
+    } else {
+      const int km = Z.at(k, -1), kp = Z.at(k, 1);
+      auto b = [&](int x, int y, int z) -> float { return Src(x, y, z, comp); };
+      v = INV_8 * b(i, j, k) +
+          INV_16 * (b(im, j, k) + b(ip, j, k) + b(i, jm, k) + b(i, jp, k) + b(i, j, km) + b(i, j, kp)) +
+          INV_32 * (b(im, jm, k) + b(ip, jp, k) + b(im, jp, k) + b(ip, jm, k) + b(i, jm, km) +
+                    b(i, jp, kp) + b(i, j, km) + b(i, j, kp) + b(im, j, km) + b(ip, j, kp) +
+                    b(im, j, kp) + b(ip, j, km)) +
+          INV_64 * (b(im, jm, km) + b(ip, jp, kp) + b(im, jp, kp) + b(ip, jm, km) + b(im, jm, kp) +
+                    b(ip, jp, km) + b(im, jp, km) + b(ip, jm, kp));
+    }
+    store_with_ghosts(Dst, i, j, k, comp, v);
+  }
+}
+
+__global__ void k_fill_zero(float* __restrict__ d, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t k = i; k < n; k += stride) d[k] = ZERO;
+}
+
+__global__ void k_copy(float* __restrict__ dst, const float* __restrict__ src, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t k = i; k < n; k += stride) dst[k] = src[k];
+}
+
 ntt_status launch_ok() { return check_launch(); }
 
 }  // namespace
@@ -356,3 +489,46 @@ extern "C" ntt_status ntt_synchronize_currents(ntt_field_t* cur, ntt_field_t* bu
   k_fold<<<active_grid(J), dim3(BX, BY), 0, as_stream(stream)>>>(J);
   return launch_ok();
 }
+
+// ---- engine-internal launchers ------------------------------------------------
+namespace ntt {
+
+unsigned flat_grid(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return static_cast<unsigned>(g);
+}
+
+void launch_faraday_periodic(const FieldView& F, float c1, float c2, hipStream_t st) {
+  k_faraday_periodic<<<active_grid(F), dim3(BX, BY), 0, st>>>(F, c1, c2);
+}
+
+void launch_ampere_currents_periodic(const FieldView& F, const FieldView& J, float c1, float c2,
+                                     float coeff, float ppc0, bool with_j, hipStream_t st) {
+  k_ampere_currents_periodic<<<active_grid(F), dim3(BX, BY), 0, st>>>(F, J, c1, c2, coeff, ppc0,
+                                                                     with_j ? 1 : 0);
+}
+
+void launch_filter_periodic(const FieldView& dst, const FieldView& src, hipStream_t st) {
+  k_filter_periodic<<<active_grid(dst), dim3(BX, BY), 0, st>>>(dst, src);
+}
+
+void launch_ghost_fill(const FieldView& F, int c0, int c1, hipStream_t st) {
+  const dim3 grid((F.N1 + BX - 1) / BX, (F.N2 + BY - 1) / BY, F.N3);
+  k_ghost_fill<<<grid, dim3(BX, BY), 0, st>>>(F, c0, c1);
+}
+
+void launch_fold(const FieldView& J, hipStream_t st) {
+  k_fold<<<active_grid(J), dim3(BX, BY), 0, st>>>(J);
+}
+
+void launch_zero(float* d, int64_t n, hipStream_t st) {
+  k_fill_zero<<<flat_grid(n), 256, 0, st>>>(d, n);
+}
+
+void launch_copy(float* dst, const float* src, int64_t n, hipStream_t st) {
+  k_copy<<<flat_grid(n), 256, 0, st>>>(dst, src, n);
+}
+
+}  // namespace ntt

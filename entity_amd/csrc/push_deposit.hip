@@ -1,0 +1,393 @@
+// push_deposit.hip -- the hot path of one SR step for a periodic box: the
+// particle push (interpolation, Boris/Vay, position update, periodic wrap)
+// fused with the charge-conserving current deposit.
+//
+// Design (gfx950):
+// * Work is cut into chunks of 256 * PPT consecutive particles (the species
+//   arrays are kept sorted by spatial tile, so a chunk covers a compact
+//   box of cells).  One workgroup = one chunk.  Chunks are mapped so that
+//   each XCD (workgroups are dealt round-robin over the 8 XCDs) walks a
+//   contiguous range of chunks, whose windows overlap in its L2.
+// * The workgroup reduces the cell bounding box of its live particles.  If
+//   box + [-1, +2] cells (the reach of the order-0..2 gather and of the
+//   deposit of a particle that moves less than half a cell) fits the LDS
+//   window, the E/B values of that box are staged into LDS once, every
+//   particle gathers from LDS, and every deposit is accumulated with LDS
+//   float atomics into an LDS current window.  The window is then added to
+//   HBM with one hardware float atomic per non-zero entry.  Particles whose
+//   exact deposit stencil leaves the window (moves of half a cell or more)
+//   add their current straight to HBM.
+// * Chunks whose box does not fit (unsorted input) run the same code against
+//   HBM (gather from the ghost-extended em array, global atomics).
+// * Deposits use the pre-wrap cell indices and are folded onto the periodic
+//   image at the flush, which produces the folded current of
+//   Metadomain::SynchronizeFields directly (metadomain_comm.cpp:408-562).
+//
+// Reference: src/kernels/pushers/sr.hpp:117-814, src/kernels/currents_deposit.hpp:108-761.
+#include "sr_particle.hpp"
+
+namespace ntt {
+
+template <int D> struct Win;
+template <> struct Win<3> { static constexpr int W1 = 16, W2 = 8, W3 = 8; };
+template <> struct Win<2> { static constexpr int W1 = 32, W2 = 16, W3 = 1; };
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves   = kThreads / 64;
+
+// LDS window accessor for em (6 components)
+template <int D>
+struct LdsEM {
+  const float* w;  // [6][W1*W2*W3]
+  int o1, o2, o3;  // ghost-extended origin of the window
+  static constexpr int W1 = Win<D>::W1, W2 = Win<D>::W2, W3 = Win<D>::W3;
+  static constexpr int WC = W1 * W2 * W3;
+  __device__ __forceinline__ float operator()(int c, int i, int j, int k) const {
+    if constexpr (D == 2)
+      return w[c * WC + (i - o1) + W1 * (j - o2)];
+    else
+      return w[c * WC + (i - o1) + W1 * ((j - o2) + W2 * (k - o3))];
+  }
+};
+
+// LDS current window sink (3 components), LDS float atomics
+template <int D>
+struct LdsJ {
+  float* w;
+  int    o1, o2, o3;
+  static constexpr int W1 = Win<D>::W1, W2 = Win<D>::W2, W3 = Win<D>::W3;
+  static constexpr int WC = W1 * W2 * W3;
+  __device__ __forceinline__ void add(int i, int j, int k, int c, float v) {
+    if constexpr (D == 2)
+      atomicAdd(w + c * WC + (i - o1) + W1 * (j - o2), v);
+    else
+      atomicAdd(w + c * WC + (i - o1) + W1 * ((j - o2) + W2 * (k - o3)), v);
+  }
+};
+
+// HBM current sink folding ghost-extended indices onto the periodic image
+struct WrapJ {
+  float*  d;
+  int32_t n1, n2, n3, G, N1, N2;
+  int64_t cstride;
+  __device__ __forceinline__ static int wrap(int x, int n) {
+    x += (x < 0) ? n : 0;
+    x -= (x >= n) ? n : 0;
+    return x;
+  }
+  __device__ __forceinline__ void add(int i, int j, int k, int c, float v) {
+    const int a = wrap(i - G, n1) + G;
+    const int b = wrap(j - G, n2) + G;
+    const int g = (n3 > 1) ? wrap(k - G, n3) + G : 0;
+    unsafeAtomicAdd(d + static_cast<int64_t>(a) +
+                      static_cast<int64_t>(N1) * (static_cast<int64_t>(b) + static_cast<int64_t>(N2) * g) +
+                      cstride * c,
+                    v);
+  }
+};
+
+struct FusedArgs {
+  ParticleView P;
+  GlobalEM     F;    // ghost-extended em (periodic ghosts filled)
+  WrapJ        J;    // active current, folded
+  int32_t      ngh;
+  int32_t      n[3];
+  float        ndth, dt, dxm, charge, inv_dt;
+  int64_t      nchunks;
+  uint32_t*    err;
+};
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+template <int D>
+__device__ __forceinline__ void load_state(const ParticleView& P, int64_t p, PState& s) {
+  s.i[0]  = P.i1[p];
+  s.i[1]  = P.i2[p];
+  s.i[2]  = (D == 3) ? P.i3[p] : 0;
+  s.dx[0] = P.dx1[p];
+  s.dx[1] = P.dx2[p];
+  if constexpr (D == 3) s.dx[2] = P.dx3[p];
+  s.u[0] = P.ux1[p];
+  s.u[1] = P.ux2[p];
+  s.u[2] = P.ux3[p];
+}
+
+template <int D>
+__device__ __forceinline__ void store_state(const ParticleView& P, int64_t p, const PState& s) {
+  P.i1p[p]  = s.ip[0];
+  P.dx1p[p] = s.dxp[0];
+  P.i1[p]   = s.i[0];
+  P.dx1[p]  = s.dx[0];
+  P.i2p[p]  = s.ip[1];
+  P.dx2p[p] = s.dxp[1];
+  P.i2[p]   = s.i[1];
+  P.dx2[p]  = s.dx[1];
+  if constexpr (D == 3) {
+    P.i3p[p]  = s.ip[2];
+    P.dx3p[p] = s.dxp[2];
+    P.i3[p]   = s.i[2];
+    P.dx3[p]  = s.dx[2];
+  }
+  P.ux1[p] = s.u[0];
+  P.ux2[p] = s.u[1];
+  P.ux3[p] = s.u[2];
+}
+
+template <int D, int O, int ALG, int PPT>
+__global__ __launch_bounds__(kThreads) void k_push_deposit(FusedArgs a) {
+  constexpr int W1 = Win<D>::W1, W2 = Win<D>::W2, W3 = Win<D>::W3;
+  constexpr int WC = W1 * W2 * W3;
+  __shared__ float em_w[6 * WC];
+  __shared__ float j_w[3 * WC];
+  __shared__ int   red[2 * 3][kWaves];
+
+  // XCD-aware chunk mapping: blockIdx.x % 8 is the XCD; give each XCD a
+  // contiguous run of chunks
+  const int64_t nb   = a.nchunks;
+  const int64_t bid  = blockIdx.x;
+  const int64_t per  = (nb + 7) / 8;
+  const int64_t xcd  = bid % 8;
+  const int64_t slot = bid / 8;
+  const int64_t full = nb - 8 * (per - 1);  // XCDs owning `per` chunks
+  const int64_t chunk =
+    (xcd < full) ? xcd * per + slot : full * per + (xcd - full) * (per - 1) + slot;
+  if (slot >= ((xcd < full) ? per : per - 1)) return;
+
+  const int     tid    = threadIdx.x;
+  const int64_t base   = chunk * (kThreads * PPT);
+  const int     G      = a.ngh;
+  const bool    do_dep = a.charge != ZERO;
+
+  // ---- pass 1: cell indices and tags -> bounding box of live particles -----
+  // (pass 2 reads i and tag again; the re-read hits in L2)
+  int lo[3] = { INT_MAX, INT_MAX, INT_MAX }, hi[3] = { INT_MIN, INT_MIN, INT_MIN };
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    const int64_t p = base + k * kThreads + tid;
+    if (p < a.P.n) {
+      const int16_t tag = a.P.tag[p];
+      if (tag == NTT_TAG_ALIVE) {
+        int c[3] = { a.P.i1[p], a.P.i2[p], 0 };
+        if constexpr (D == 3) c[2] = a.P.i3[p];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          lo[d] = min(lo[d], c[d]);
+          hi[d] = max(hi[d], c[d]);
+        }
+      } else if (tag != NTT_TAG_DEAD) {
+        raise_device_error(a.err, NTT_DEVERR_INVALID_TAG);
+      }
+    }
+  }
+  const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const int l = wave_min(lo[d]), h = wave_max(hi[d]);
+    if (lane == 0) {
+      red[2 * d][wv]     = l;
+      red[2 * d + 1][wv] = h;
+    }
+  }
+  __syncthreads();
+  int  blo[3] = { 0, 0, 0 }, bhi[3] = { 0, 0, 0 };
+  bool any = true, fits = true;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    int l = red[2 * d][0], h = red[2 * d + 1][0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) {
+      l = min(l, red[2 * d][w]);
+      h = max(h, red[2 * d + 1][w]);
+    }
+    blo[d] = l;
+    bhi[d] = h;
+    if (l > h) any = false;
+  }
+  if (!any) return;  // no live particle in this chunk (uniform across the block)
+  // every live particle must sit inside the box [0, n) for the window to
+  // stay inside the ghost-extended array; anything else takes the HBM path
+  const int Wd[3] = { W1, W2, W3 };
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    if (bhi[d] - blo[d] + 4 > Wd[d]) fits = false;
+    if (blo[d] < 0 || bhi[d] >= a.n[d]) fits = false;
+  }
+
+  if (!fits) {
+    // ---- HBM path -------------------------------------------------------------
+#pragma unroll 1
+    for (int k = 0; k < PPT; ++k) {
+      const int64_t p = base + k * kThreads + tid;
+      if (p >= a.P.n || a.P.tag[p] != NTT_TAG_ALIVE) continue;
+      PState s;
+      s.tag = NTT_TAG_ALIVE;
+      load_state<D>(a.P, p, s);
+      bool ok = true;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const int g = s.i[d] + G;
+        if (g - 1 < 0 || g + 2 >= a.n[d] + 2 * G || !(s.dx[d] >= ZERO && s.dx[d] <= ONE)) ok = false;
+      }
+      if (!ok) {
+        raise_device_error(a.err, NTT_DEVERR_OUT_OF_BOUNDS);
+        continue;
+      }
+      sr_move<D, O, ALG>(a.F, G, a.ndth, a.dt, a.dxm, s);
+      if (do_dep) {
+        WrapJ js = a.J;
+        deposit_particle<D, O>(js, G, s, a.P.w[p] * a.charge, a.inv_dt);
+      }
+      sr_wrap_periodic<D>(a.n, s);
+      store_state<D>(a.P, p, s);
+    }
+    return;
+  }
+
+  // ---- window path ----------------------------------------------------------
+  // window origin (ghost-extended): box low corner - 1
+  const int o1 = blo[0] - 1 + G, o2 = blo[1] - 1 + G, o3 = (D == 3) ? blo[2] - 1 + G : 0;
+  const int e1 = bhi[0] - blo[0] + 4, e2 = bhi[1] - blo[1] + 4;
+  const int e3 = (D == 3) ? bhi[2] - blo[2] + 4 : 1;
+  const int ncell = e1 * e2 * e3;
+  // stage E, B (the box lies inside the ghost-extended array: o >= 1 and
+  // o + e - 1 <= n + 2 * G - 1 because 0 <= blo <= bhi <= n - 1 and G = 2)
+  for (int idx = tid; idx < 6 * ncell; idx += kThreads) {
+    const int c   = idx / ncell;
+    int       r   = idx - c * ncell;
+    const int l3  = r / (e1 * e2);
+    r -= l3 * e1 * e2;
+    const int l2  = r / e1;
+    const int l1  = r - l2 * e1;
+    em_w[c * WC + l1 + W1 * (l2 + W2 * l3)] = a.F(c, o1 + l1, o2 + l2, o3 + l3);
+  }
+  for (int idx = tid; idx < 3 * ncell; idx += kThreads) {
+    const int c  = idx / ncell;
+    int       r  = idx - c * ncell;
+    const int l3 = r / (e1 * e2);
+    r -= l3 * e1 * e2;
+    const int l2 = r / e1;
+    const int l1 = r - l2 * e1;
+    j_w[c * WC + l1 + W1 * (l2 + W2 * l3)] = ZERO;
+  }
+  __syncthreads();
+
+  const LdsEM<D> F{ em_w, o1, o2, o3 };
+  LdsJ<D>        jw{ j_w, o1, o2, o3 };
+#pragma unroll 1
+  for (int k = 0; k < PPT; ++k) {
+    const int64_t p = base + k * kThreads + tid;
+    if (p >= a.P.n || a.P.tag[p] != NTT_TAG_ALIVE) continue;
+    PState s;
+    s.tag = NTT_TAG_ALIVE;
+    load_state<D>(a.P, p, s);
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      if (!(s.dx[d] >= ZERO && s.dx[d] <= ONE)) ok = false;
+    if (!ok) {
+      raise_device_error(a.err, NTT_DEVERR_OUT_OF_BOUNDS);
+      continue;
+    }
+    const float w = a.P.w[p];
+    sr_move<D, O, ALG>(F, G, a.ndth, a.dt, a.dxm, s);
+    // exact deposit stencil inside the window?
+    bool inwin = true;
+    const int oo[3] = { o1, o2, o3 }, ee[3] = { e1, e2, e3 };
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      int l, h;
+      deposit_extent<O>(s.i[d], s.dx[d], s.ip[d], s.dxp[d], l, h);
+      if (l + G < oo[d] || h + G >= oo[d] + ee[d]) inwin = false;
+    }
+    if (do_dep) {
+      if (inwin) {
+        deposit_particle<D, O>(jw, G, s, w * a.charge, a.inv_dt);
+      } else {
+        WrapJ js = a.J;
+        deposit_particle<D, O>(js, G, s, w * a.charge, a.inv_dt);
+      }
+    }
+    sr_wrap_periodic<D>(a.n, s);
+    store_state<D>(a.P, p, s);
+  }
+  __syncthreads();
+  // flush the current window onto the periodic image in HBM
+  for (int idx = tid; idx < 3 * ncell; idx += kThreads) {
+    const int c  = idx / ncell;
+    int       r  = idx - c * ncell;
+    const int l3 = r / (e1 * e2);
+    r -= l3 * e1 * e2;
+    const int l2 = r / e1;
+    const int l1 = r - l2 * e1;
+    const float v = j_w[c * WC + l1 + W1 * (l2 + W2 * l3)];
+    if (v != ZERO) a.J.add(o1 + l1, o2 + l2, o3 + l3, c, v);
+  }
+}
+
+template <int D, int O, int ALG>
+void launch_fused(const FusedArgs& a0, hipStream_t st) {
+  constexpr int PPT = 4;
+  FusedArgs a   = a0;
+  a.nchunks     = (a.P.n + kThreads * PPT - 1) / (kThreads * PPT);
+  if (a.nchunks == 0) return;
+  k_push_deposit<D, O, ALG, PPT><<<static_cast<unsigned>(a.nchunks), kThreads, 0, st>>>(a);
+}
+
+template <int D, int ALG>
+void launch_fused_o(int order, const FusedArgs& a, hipStream_t st) {
+  if (order == 0) launch_fused<D, 0, ALG>(a, st);
+  else if (order == 1) launch_fused<D, 1, ALG>(a, st);
+  else launch_fused<D, 2, ALG>(a, st);
+}
+
+}  // namespace
+
+// Fused push + deposit of one species over a periodic box.  em must have its
+// periodic ghosts filled; the folded current is added to the active cells of
+// `cur` (the caller zeroes it first).
+ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView& em,
+                                        const FieldView& cur, int D, int order, int alg,
+                                        float ndth, float dt, float dxm, float charge,
+                                        bool deposit, hipStream_t st) {
+  if (pv.n == 0) return NTT_OK;
+  uint32_t* err = device_error_word();
+  if (err == nullptr) return NTT_ERR_HIP;
+  FusedArgs a;
+  a.P = pv;
+  a.F = GlobalEM{ em.d, em.N1, em.N2, em.cstride };
+  a.J = WrapJ{ cur.d, cur.n1, cur.n2, (D == 3) ? cur.n3 : 1, cur.ngh, cur.N1, cur.N2, cur.cstride };
+  a.ngh = em.ngh;
+  a.n[0] = em.n1;
+  a.n[1] = em.n2;
+  a.n[2] = (D == 3) ? em.n3 : 1;
+  a.ndth = ndth;
+  a.dt = dt;
+  a.dxm = dxm;
+  a.charge = deposit ? charge : ZERO;
+  a.inv_dt = ONE / dt;
+  a.err = err;
+  a.nchunks = 0;
+  if (D == 2) {
+    if (alg == ALG_BORIS) launch_fused_o<2, ALG_BORIS>(order, a, st);
+    else if (alg == ALG_VAY) launch_fused_o<2, ALG_VAY>(order, a, st);
+    else launch_fused<2, 0, ALG_PHOTON>(a, st);
+  } else {
+    if (alg == ALG_BORIS) launch_fused_o<3, ALG_BORIS>(order, a, st);
+    else if (alg == ALG_VAY) launch_fused_o<3, ALG_VAY>(order, a, st);
+    else launch_fused<3, 0, ALG_PHOTON>(a, st);
+  }
+  return check_launch();
+}
+
+}  // namespace ntt

@@ -117,15 +117,12 @@ __global__ __launch_bounds__(kBlock) void k_deposit(DepositArgs a) {
     s.u[1] = a.P.ux2[p];
     s.u[2] = a.P.ux3[p];
   }
-  // every write lies in [min(i, i_prev) - (O == 2), max(i, i_prev) + max(O, 1)]
-  // (ghost-extended); refuse particles whose stencil leaves the array
-  constexpr int lo_off = (O == 2) ? -1 : 0;
-  constexpr int hi_off = (O == 0) ? 1 : O;
+  // refuse particles whose stencil leaves the ghost-extended array
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    const int lo = min(s.i[d], s.ip[d]) + a.ngh + lo_off;
-    const int hi = max(s.i[d], s.ip[d]) + a.ngh + hi_off;
-    if (lo < 0 || hi >= a.N[d]) {
+    int lo, hi;
+    deposit_extent<O>(s.i[d], s.dx[d], s.ip[d], s.dxp[d], lo, hi);
+    if (lo + a.ngh < 0 || hi + a.ngh >= a.N[d]) {
       raise_device_error(a.err, NTT_DEVERR_OUT_OF_BOUNDS);
       return;
     }

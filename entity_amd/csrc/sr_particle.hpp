@@ -316,12 +316,12 @@ struct PState {
   int16_t tag;
 };
 
-// One full SR update of a particle (alive check done by the caller):
-// interpolation, momentum update, position update, boundary conditions.
-// sr.hpp:117-332 (no-policy path) + 526-814.
+// Momentum and position update of a particle (alive check done by the
+// caller): interpolation, momentum update, position update -- without the
+// boundary conditions.  sr.hpp:117-332 (no-policy path) + 526-572.
 template <int D, int O, int ALG, class A>
-__device__ __forceinline__ void sr_update(const A& F, int ngh, float ndth, float dt, float dxm,
-                                          const PrtlBCs& bc, PState& s) {
+__device__ __forceinline__ void sr_move(const A& F, int ngh, float ndth, float dt, float dxm,
+                                        PState& s) {
   if constexpr (ALG != ALG_PHOTON) {
     float e0[3], b0[3];
     interpolate<D, O>(F, s.i[0] + ngh, s.i[1] + ngh, D == 3 ? s.i[2] + ngh : 0, s.dx[0], s.dx[1],
@@ -346,9 +346,38 @@ __device__ __forceinline__ void sr_update(const A& F, int ngh, float ndth, float
     // transform<d, XYZ, U>: u / dx (minkowski.h:277-281)
     push_coord(s.i[d], s.dx[d], (s.u[d] / dxm) * dt_inv_energy);
   }
+}
+
+// boundaryConditions, Cartesian (sr.hpp:659-790)
+template <int D>
+__device__ __forceinline__ void sr_apply_bcs(const PrtlBCs& bc, PState& s) {
 #pragma unroll
   for (int d = 0; d < D; ++d)
     bc_dir(s.i[d], s.ip[d], s.dx[d], s.u[d], s.tag, bc.ni[d], bc.lo[d], bc.hi[d], d == 1);
+}
+
+// periodic wrap of i and i_prev in every direction (sr.hpp:663-668 and the
+// analogous branches), the only boundary the fused step path accepts
+template <int D>
+__device__ __forceinline__ void sr_wrap_periodic(const int32_t* ni, PState& s) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    if (s.i[d] < 0) {
+      s.i[d] += ni[d];
+      s.ip[d] += ni[d];
+    } else if (s.i[d] >= ni[d]) {
+      s.i[d] -= ni[d];
+      s.ip[d] -= ni[d];
+    }
+  }
+}
+
+// One full SR update of a particle: sr_move followed by the boundaries.
+template <int D, int O, int ALG, class A>
+__device__ __forceinline__ void sr_update(const A& F, int ngh, float ndth, float dt, float dxm,
+                                          const PrtlBCs& bc, PState& s) {
+  sr_move<D, O, ALG>(F, ngh, ndth, dt, dxm, s);
+  sr_apply_bcs<D>(bc, s);
 }
 
 // ---- current deposit: currents_deposit.hpp:108-761 ---------------------------
@@ -536,13 +565,21 @@ __device__ __forceinline__ void deposit_particle(Sink& J, int G, const PState& s
   else deposit_esirkepov_3d<O>(J, G, s, coeff, inv_dt);
 }
 
-// Ghost-extended index range [lo, hi] (relative to the final cell i + G)
-// touched by the deposit of a particle that moved at most one cell.
+// Exact index range [lo, hi] (particle indices, without ghosts) written by
+// the deposit along one direction: the zig-zag touches i_prev, i and the
+// node above each; Esirkepov spans [i_min, i_max] of for_deposit.
 template <int O>
-__device__ __forceinline__ void deposit_reach(int& lo, int& hi) {
-  if constexpr (O == 0) { lo = -1; hi = 2; }
-  else if constexpr (O == 1) { lo = -2; hi = 2; }
-  else { lo = -3; hi = 3; }
+__device__ __forceinline__ void deposit_extent(int i, float dx, int ip, float dxp, int& lo,
+                                               int& hi) {
+  if constexpr (O == 0) {
+    lo = min(i, ip);
+    hi = max(i, ip) + 1;
+  } else {
+    const int a = (O == 2 && dxp < HALF) ? ip - 1 : ip;
+    const int b = (O == 2 && dx < HALF) ? i - 1 : i;
+    lo = min(a, b);
+    hi = (a == b) ? a + O : lo + O + 1;
+  }
 }
 
 }  // namespace ntt

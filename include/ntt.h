@@ -236,6 +236,52 @@ NTT_API ntt_status ntt_communicate_fields(ntt_field_t* fld,
  * for Comm::J (src/framework/domain/metadomain_comm.cpp:369-562). */
 NTT_API ntt_status ntt_synchronize_currents(ntt_field_t* cur, ntt_field_t* buff, void* stream);
 
+/* ------------------------------------------------------ engine-level entries */
+
+/* One species of a domain (ntt::Particles + ParticleSpecies mass/charge/pusher,
+ * src/framework/containers/species.h:29-32). */
+typedef struct ntt_species {
+  ntt_particles_t prtls;
+  float           mass;
+  float           charge;
+  uint8_t         pusher_flags; /* NTT_PUSHER_*; NONE skips the species */
+} ntt_species_t;
+
+/* Scalars of one SRPIC step (engine params and SimulationParams the drivers
+ * read: src/engines/srpic/fieldsolvers.h:36-200, currents.h:64-119,
+ * particle_pusher.h:93-105). */
+typedef struct ntt_sr_step_params {
+  float           dt;                 /* CFL * dx / sqrt(D) (algorithms.cpp:21-22) */
+  float           correction;         /* algorithms.timestep.correction            */
+  float           omegaB0;            /* scales.omegaB0 = 1 / larmor0               */
+  float           q0, B0, V0, ppc0;   /* scales.q0, scales.B0, scales.V0, ppc0      */
+  int32_t         shape_order;        /* SHAPE_ORDER: 0, 1, 2                       */
+  int32_t         n_filters;          /* algorithms.current_filters                 */
+  int32_t         fieldsolver_enable; /* algorithms.fieldsolver.enable              */
+  int32_t         deposit_enable;     /* algorithms.deposit.enable                  */
+  int64_t         step;               /* step 0 fills the E/B ghosts first          */
+  ntt_fd_coeffs_t fd;                 /* extended Faraday stencil (zero = Yee)      */
+  ntt_prtl_bc_t   prtl_bc;            /* must be periodic in every direction        */
+} ntt_sr_step_params_t;
+
+/* One full SRPIC step of a periodic box, in the reference's phase order:
+ * Faraday(1/2) -> push + deposit of every species -> current fold -> n
+ * filter passes -> Faraday(1/2) -> Ampere -> current source (E += coeff J,
+ * J /= ppc0), leaving E, B and J with consistent periodic ghosts.
+ * Replaces SRPICEngine::step_forward (src/engines/srpic/srpic.hpp:65-188) for
+ * a single periodic domain.  Push and deposit run fused and LDS-windowed
+ * (the species arrays should be sorted by spatial tile, see ntt_sort_particles;
+ * unsorted input is handled, only slower).  `buff` is scratch (its contents
+ * afterwards are unspecified). */
+NTT_API ntt_status ntt_sr_step(ntt_field_t*                em,
+                               ntt_field_t*                cur,
+                               ntt_field_t*                buff,
+                               ntt_species_t*              species,
+                               int32_t                     nspecies,
+                               const ntt_sr_step_params_t* prm,
+                               const ntt_minkowski_t*      metric,
+                               void*                       stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -968,7 +968,8 @@ int sr_step(ort_field_t* em, ort_field_t* cur, ort_field_t* buff, ort_species_t*
   if (prm->step == 0) err |= communicate<R>(em, 0, 6);
   // srpic::Faraday coefficients (fieldsolvers.h:44-76)
   const R dt = static_cast<R>(prm->dt);
-  const R dxm = static_cast<R>(prm->dx);
+  // dx = sqrt(h_11) as the drivers compute it (fieldsolvers.h:50,110)
+  const R dxm = std::sqrt(static_cast<R>(prm->dx) * static_cast<R>(prm->dx));
   auto fd_coeffs = [&](R fraction, R& c1, R& c2) {
     const R dT = fraction * static_cast<R>(prm->correction) * dt;
     if (D == 2) { c1 = dT / (dxm * dxm); c2 = dT; }
