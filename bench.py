@@ -1,0 +1,252 @@
+#!/usr/bin/env python3
+"""Throughput of the SR PIC step (push + deposit + fields) on one MI355X.
+
+Workload (default): 3D periodic 512 x 256 x 256 cells, e-/e+ pair plasma at
+16 particles per cell per species (1,073,741,824 particles), theta = 0.3
+thermal spread, background B0 along z plus a 16-mode random-phase
+perturbation (|k| <= 4 * 2 pi / L, dB/B0 = 0.5), shape order 2, Vay pusher,
+2 filter passes, CFL 0.5.  Synthetic inputs are generated on the GPU.
+
+One step = ntt_sr_step (Faraday 1/2, fused push+deposit per species,
+current fold, filters, Faraday 1/2, Ampere + current source) plus the tile
+re-sort every --sort-interval steps, all inside the timed region.
+
+Prints one JSON line.  Also reports the fused push+deposit kernel against the
+HBM roofline (HIP events around every launch in the timed region) and a CPU
+baseline: the oracle (a sequential C++ restatement, not the reference binary)
+on a sub-box of the same workload, on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "particle-steps/s (push+deposit+fields) 3D SR; batched & adjoint vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cells", type=str, default="512,256,256")
+    ap.add_argument("--ppc", type=int, default=16, help="particles per cell per species")
+    ap.add_argument("--order", type=int, default=2)
+    ap.add_argument("--pusher", choices=["vay", "boris"], default="vay")
+    ap.add_argument("--filters", type=int, default=2)
+    ap.add_argument("--sort-interval", type=int, default=10)
+    ap.add_argument("--tile", type=str, default="4,4,4")
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--cpu-cells", type=str, default="64,64,64")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    return ap.parse_args()
+
+
+def init_fields(eng, seed=12345, nmodes=16, kmax=4, dB=0.5, B0=1.0):
+    """B = B0 z + curl A with A a sum of random-phase modes (discrete curl on
+    the Yee grid, so div B = 0 to round-off); E = 0."""
+    import numpy as np
+    import torch
+    from entity_amd.kernels import communicate_fields, mink
+
+    shape = eng.shape
+    n = shape.n
+    L = [float(x) for x in n]
+    G = shape.ngh
+    rng = np.random.default_rng(seed)
+    modes = []
+    while len(modes) < nmodes:
+        m = rng.integers(-kmax, kmax + 1, 3)
+        if 0 < np.sqrt((m ** 2).sum()) <= kmax:
+            modes.append(m)
+    em = eng.fields.em
+    em.zero_()
+    dev = em.device
+    axes = [torch.arange(e, device=dev, dtype=torch.float32) - G for e in shape.extents]
+    shifts = {0: (0.5, 0, 0), 1: (0, 0.5, 0), 2: (0, 0, 0.5)}
+    for c in range(3):
+        z, y, x = torch.meshgrid(axes[2] + shifts[c][2], axes[1] + shifts[c][1],
+                                 axes[0] + shifts[c][0], indexing="ij")
+        acc = torch.zeros_like(x)
+        for m in modes:
+            k = [2 * math.pi * m[d] / L[d] for d in range(3)]
+            amp = rng.standard_normal(3)
+            kk = np.array(k)
+            amp -= kk * (amp @ kk) / (kk @ kk)   # transverse polarization
+            ph = rng.uniform(0, 2 * math.pi)
+            acc += float(amp[c] / max(np.linalg.norm(kk), 1e-9)) * torch.cos(
+                k[0] * x + k[1] * y + k[2] * z + ph)
+        em[c].copy_(acc)
+        del x, y, z, acc
+    communicate_fields(em, shape, 0, 3)
+    mink.Faraday_kernel(em, shape, -1.0, 0.0)()   # B = curl A
+    em[:3].zero_()
+    act = (slice(None),) + tuple(slice(G, G + s) for s in reversed(n))
+    B = em[3:6][act]
+    rms = torch.sqrt((B.double() ** 2).sum(0).mean()).item()
+    em[3:6] *= dB * B0 / rms
+    em[5] += B0
+    communicate_fields(em, shape, 0, 6)
+    torch.cuda.synchronize()
+
+
+def cpu_baseline(args, cells, order, pusher_flag, nfilt):
+    """The oracle (sequential C++, FP32) on a sub-box of the workload."""
+    import numpy as np
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import ntt_oracle
+    dim = 3
+    n = cells
+    ppc = args.ppc
+    rng = np.random.default_rng(5)
+    ncell = int(np.prod(n))
+    npart = ncell * ppc
+    em = np.zeros((6,) + tuple(x + 4 for x in reversed(n)), np.float32)
+    em[5] = 1.0
+    em[3:5] += (rng.standard_normal(em[3:5].shape) * 0.3).astype(np.float32)
+    cur = np.zeros((3,) + em.shape[1:], np.float32)
+    buff = np.zeros_like(cur)
+    species = []
+    for s, q in enumerate((-1.0, 1.0)):
+        cell = np.repeat(np.arange(ncell), ppc)
+        p = {"i1": (cell % n[0]).astype(np.int32), "i2": ((cell // n[0]) % n[1]).astype(np.int32),
+             "i3": (cell // (n[0] * n[1])).astype(np.int32)}
+        for d in range(3):
+            p[f"dx{d+1}"] = rng.random(npart, dtype=np.float32)
+            p[f"i{d+1}_prev"] = p[f"i{d+1}"].copy()
+            p[f"dx{d+1}_prev"] = p[f"dx{d+1}"].copy()
+            p[f"ux{d+1}"] = (rng.standard_normal(npart) * math.sqrt(0.3)).astype(np.float32)
+        p["weight"] = np.ones(npart, np.float32)
+        p["tag"] = np.ones(npart, np.int16)
+        species.append({"p": ntt_oracle.normalize_particles(p, "f32"), "mass": 1.0,
+                        "charge": q, "pusher": pusher_flag})
+    dt = float(np.float32(0.5) * (np.float32(1.0) / np.sqrt(np.float32(3))))
+    prm = dict(dt=dt, correction=1.0, omegaB0=1.0 / 8, q0=1.0 / (2 * ppc * 64.0), B0=1.0 / 8,
+               V0=1.0, ppc0=2.0 * ppc, dx=1.0, shape_order=order, n_filters=nfilt,
+               fieldsolver_enable=1, deposit_enable=1)
+    t0 = time.perf_counter()
+    for k in range(args.cpu_steps):
+        prm["step"] = k
+        rc = ntt_oracle.sr_step(em, cur, buff, species, dim=3, prm=prm, bc=((1, 1),) * 3)
+        assert rc == 0, rc
+    el = time.perf_counter() - t0
+    return {"value": 2 * npart * args.cpu_steps / el, "unit": "particle-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{args.cpu_steps} full steps of a {n[0]}x{n[1]}x{n[2]} sub-box of the same "
+                      f"workload ({2 * npart} particles, order {order}, {nfilt} filter passes), "
+                      f"oracle C++ restatement in FP32, one core, {el:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                          "(multi-GPU decomposition is a design note, not a measured path)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from entity_amd import _abi
+    from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
+    from entity_amd.kernels import device_errors
+
+    cells = tuple(int(x) for x in args.cells.split(","))
+    tile = tuple(int(x) for x in args.tile.split(","))
+    dim = len(cells)
+    pusher = _abi.PUSHER_VAY if args.pusher == "vay" else _abi.PUSHER_BORIS
+    prm = SRParams(dim=dim, n=cells, cfl=0.5, shape_order=args.order, n_filters=args.filters,
+                   ppc0=2.0 * args.ppc, larmor0=8.0, skindepth0=8.0,
+                   sort_interval=args.sort_interval, tile=tile)
+    ncell = int(np.prod(cells))
+    npart = ncell * args.ppc
+    t_init = time.perf_counter()
+    eng = SRPICEngine(prm, [SpeciesSpec("e-", 1.0, -1.0, pusher),
+                            SpeciesSpec("e+", 1.0, 1.0, pusher)], npart)
+    eng.inject_uniform(args.ppc, temperature=0.3, seed=2024, position_seed=99)
+    init_fields(eng)
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t_init
+    print(f"# init: {2 * npart} particles on {cells} in {t_init:.1f} s", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        eng.step_forward()
+    torch.cuda.synchronize()
+    assert device_errors() == 0
+    lib = eng.lib
+    lib.ntt_profile_read(None, None, 1)
+    lib.ntt_profile_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        eng.step_forward()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    lib.ntt_profile_enable(0)
+    import ctypes as C
+    ms_pd, nl = C.c_double(0), C.c_int64(0)
+    _abi.check(lib.ntt_profile_read(C.byref(ms_pd), C.byref(nl), 1), "ntt_profile_read")
+    errs = device_errors()
+    ms_step = 1000.0 * el / args.steps
+    total_particles = 2 * npart
+    value = total_particles * args.steps / el
+    # algorithmic bytes of one fused push+deposit launch (one species):
+    # per particle 102 B (3D) / 78 B (2D), per cell 36 B (E/B read + J write)
+    b_p = 102 if dim == 3 else 78
+    bytes_launch = npart * b_p + ncell * 36
+    avg_ms = ms_pd.value / max(nl.value, 1)
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{dim}D periodic {'x'.join(map(str, cells))} cells, e-/e+ "
+                        f"{args.ppc} ppc per species ({total_particles} particles), theta=0.3, "
+                        f"B0 z + 16-mode dB/B0=0.5, shape order {args.order}, {args.pusher} pusher, "
+                        f"{args.filters} filter passes, CFL 0.5, re-sort every "
+                        f"{args.sort_interval} steps",
+            "cells": list(cells), "particles": total_particles, "shape_order": args.order,
+            "pusher": args.pusher, "filters": args.filters, "sort_interval": args.sort_interval,
+            "tile": list(tile),
+        },
+        "cell_updates_per_s": ncell * args.steps / el,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_push_deposit (fused push + Esirkepov deposit, one species per launch)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_ms": avg_ms,
+            "launches": int(nl.value),
+            "algorithmic_bytes_per_launch": bytes_launch,
+        },
+        "device_errors": errs,
+    }
+    if args.cpu_baseline:
+        ccells = tuple(int(x) for x in args.cpu_cells.split(","))
+        out["cpu_baseline"] = cpu_baseline(args, ccells, args.order, pusher, args.filters)
+        out["cpu_baseline"]["cores_available"] = os.cpu_count()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -85,6 +85,26 @@ class NttFdCoeffs(C.Structure):
                 ("beta_zx", C.c_float), ("beta_yz", C.c_float), ("beta_zy", C.c_float)]
 
 
+class NttSpecies(C.Structure):
+    _fields_ = [("prtls", NttParticles), ("mass", C.c_float), ("charge", C.c_float),
+                ("pusher_flags", C.c_uint8), ("tile_offsets", C.c_void_p), ("ntiles", C.c_int64)]
+
+
+class NttSrStepParams(C.Structure):
+    _fields_ = [("dt", C.c_float), ("correction", C.c_float), ("omegaB0", C.c_float),
+                ("q0", C.c_float), ("B0", C.c_float), ("V0", C.c_float), ("ppc0", C.c_float),
+                ("shape_order", C.c_int32), ("n_filters", C.c_int32),
+                ("fieldsolver_enable", C.c_int32), ("deposit_enable", C.c_int32),
+                ("step", C.c_int64), ("fd", NttFdCoeffs), ("prtl_bc", NttPrtlBC),
+                ("particles_per_thread", C.c_int32)]
+
+
+class NttPlasmaInit(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("position_seed", C.c_uint64), ("species_id", C.c_int32),
+                ("ppc", C.c_int32), ("temperature", C.c_float), ("drift_u", C.c_float * 3),
+                ("weight", C.c_float), ("tile", C.c_int32 * 3)]
+
+
 class NttError(RuntimeError):
     def __init__(self, status: int, what: str = ""):
         self.status = status
@@ -114,6 +134,18 @@ SIGNATURES: dict[str, tuple] = {
     "ntt_digital_filter": (C.c_int, [_P(NttField), _P(NttField), _P(NttFldsBC), C.c_void_p]),
     "ntt_communicate_fields": (C.c_int, [_P(NttField), C.c_int32, C.c_int32, C.c_void_p]),
     "ntt_synchronize_currents": (C.c_int, [_P(NttField), _P(NttField), C.c_void_p]),
+    "ntt_sr_step": (C.c_int, [_P(NttField), _P(NttField), _P(NttField), _P(NttSpecies), C.c_int32,
+                              _P(NttSrStepParams), _P(NttMinkowski), C.c_void_p]),
+    "ntt_sort_workspace_size": (C.c_int, [C.c_int64, C.c_int32, _P(C.c_int32), _P(C.c_int32),
+                                          _P(C.c_uint64)]),
+    "ntt_sort_particles": (C.c_int, [_P(NttParticles), C.c_int32, _P(C.c_int32), _P(C.c_int32),
+                                     C.c_void_p, C.c_uint64, C.c_void_p, _P(C.c_int64),
+                                     C.c_void_p]),
+    "ntt_sort_ntiles": (C.c_int, [C.c_int32, _P(C.c_int32), _P(C.c_int32), _P(C.c_int64)]),
+    "ntt_profile_enable": (C.c_int, [C.c_int32]),
+    "ntt_profile_read": (C.c_int, [_P(C.c_double), _P(C.c_int64), C.c_int32]),
+    "ntt_init_uniform_plasma": (C.c_int, [_P(NttParticles), C.c_int32, _P(C.c_int32),
+                                          _P(NttPlasmaInit), C.c_void_p]),
 }
 
 _LIBS: dict[str, C.CDLL] = {}

@@ -3,12 +3,47 @@
 // with the drivers of src/engines/srpic/{fieldsolvers.h, currents.h,
 // particle_pusher.h}).  All launches are enqueued on one stream.
 #include <cmath>
+#include <mutex>
+#include <vector>
 
 #include "internal.hpp"
 
 using namespace ntt;
 
 namespace {
+
+// launch-timing record of the fused push+deposit kernels
+struct Profile {
+  std::mutex                                  mu;
+  bool                                        on = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> spare;
+  double                                      ms = 0.0;
+  int64_t                                     launches = 0;
+};
+Profile g_prof;
+
+bool prof_begin(hipStream_t st, std::pair<hipEvent_t, hipEvent_t>* ev) {
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  if (!g_prof.on) return false;
+  if (!g_prof.spare.empty()) {
+    *ev = g_prof.spare.back();
+    g_prof.spare.pop_back();
+  } else {
+    if (hipEventCreate(&ev->first) != hipSuccess) return false;
+    if (hipEventCreate(&ev->second) != hipSuccess) {
+      (void)hipEventDestroy(ev->first);
+      return false;
+    }
+  }
+  if (hipEventRecord(ev->first, st) != hipSuccess) return false;
+  return true;
+}
+
+void prof_end(hipStream_t st, const std::pair<hipEvent_t, hipEvent_t>& ev) {
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  if (hipEventRecord(ev.second, st) == hipSuccess) g_prof.pending.push_back(ev);
+}
 
 bool same_grid(const FieldView& a, const FieldView& b) {
   return a.n1 == b.n1 && a.n2 == b.n2 && a.n3 == b.n3 && a.ngh == b.ngh && a.N3 == b.N3;
@@ -105,8 +140,13 @@ extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t
   for (int sp = 0; sp < nspecies; ++sp) {
     const Prep& p = prep[sp];
     if (!p.run) continue;
+    const int ppt = prm->particles_per_thread > 0 ? prm->particles_per_thread : 4;
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    const bool timed = prof_begin(st, &ev);
     s = launch_push_deposit_periodic(p.pv, E, J, D, prm->shape_order, p.alg, p.ndth, dt,
-                                     metric->dx, species[sp].charge, p.dep, st);
+                                     metric->dx, species[sp].charge, p.dep,
+                                     species[sp].tile_offsets, species[sp].ntiles, ppt, st);
+    if (timed) prof_end(st, ev);
     if (s != NTT_OK) return s;
   }
   if (prm->deposit_enable) {
@@ -137,4 +177,30 @@ extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t
     launch_ampere_currents_periodic(E, J, c1, c2, coeff, prm->ppc0, prm->deposit_enable != 0, st);
   }
   return check_launch();
+}
+
+extern "C" ntt_status ntt_profile_enable(int32_t on) {
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  g_prof.on = on != 0;
+  return NTT_OK;
+}
+
+extern "C" ntt_status ntt_profile_read(double* ms_total, int64_t* launches, int32_t reset) {
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  for (auto& ev : g_prof.pending) {
+    if (hipEventSynchronize(ev.second) != hipSuccess) return NTT_ERR_HIP;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ev.first, ev.second) != hipSuccess) return NTT_ERR_HIP;
+    g_prof.ms += ms;
+    g_prof.launches += 1;
+    g_prof.spare.push_back(ev);
+  }
+  g_prof.pending.clear();
+  if (ms_total) *ms_total = g_prof.ms;
+  if (launches) *launches = g_prof.launches;
+  if (reset) {
+    g_prof.ms = 0.0;
+    g_prof.launches = 0;
+  }
+  return NTT_OK;
 }

@@ -19,7 +19,8 @@ ntt_status launch_deposit(const ParticleView& pv, const FieldView& J, int D, int
 ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView& em,
                                         const FieldView& cur, int D, int order, int alg,
                                         float ndth, float dt, float dxm, float charge,
-                                        bool deposit, hipStream_t st);
+                                        bool deposit, const uint32_t* tile_offsets,
+                                        int64_t ntiles, int ppt, hipStream_t st);
 
 // fields.hip
 void launch_faraday_periodic(const FieldView& F, float c1, float c2, hipStream_t st);

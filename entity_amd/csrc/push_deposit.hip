@@ -3,11 +3,11 @@
 // fused with the charge-conserving current deposit.
 //
 // Design (gfx950):
-// * Work is cut into chunks of 256 * PPT consecutive particles (the species
-//   arrays are kept sorted by spatial tile, so a chunk covers a compact
-//   box of cells).  One workgroup = one chunk.  Chunks are mapped so that
-//   each XCD (workgroups are dealt round-robin over the 8 XCDs) walks a
-//   contiguous range of chunks, whose windows overlap in its L2.
+// * Work is cut into chunks of consecutive particles: the particle range of
+//   one spatial tile when the tile offsets of the last sort are given, else
+//   fixed chunks of 256 * ppt particles.  One workgroup = one chunk.  Chunks
+//   are mapped so that each XCD (workgroups are dealt round-robin over the 8
+//   XCDs) walks a contiguous range of chunks, whose windows overlap in its L2.
 // * The workgroup reduces the cell bounding box of its live particles.  If
 //   box + [-1, +2] cells (the reach of the order-0..2 gather and of the
 //   deposit of a particle that moves less than half a cell) fits the LDS
@@ -29,7 +29,7 @@
 namespace ntt {
 
 template <int D> struct Win;
-template <> struct Win<3> { static constexpr int W1 = 16, W2 = 8, W3 = 8; };
+template <> struct Win<3> { static constexpr int W1 = 12, W2 = 10, W3 = 10; };
 template <> struct Win<2> { static constexpr int W1 = 32, W2 = 16, W3 = 1; };
 
 namespace {
@@ -89,14 +89,16 @@ struct WrapJ {
 };
 
 struct FusedArgs {
-  ParticleView P;
-  GlobalEM     F;    // ghost-extended em (periodic ghosts filled)
-  WrapJ        J;    // active current, folded
-  int32_t      ngh;
-  int32_t      n[3];
-  float        ndth, dt, dxm, charge, inv_dt;
-  int64_t      nchunks;
-  uint32_t*    err;
+  ParticleView    P;
+  GlobalEM        F;     // ghost-extended em (periodic ghosts filled)
+  WrapJ           J;     // active current, folded
+  int32_t         ngh;
+  int32_t         n[3];
+  float           ndth, dt, dxm, charge, inv_dt;
+  int64_t         nchunks;
+  int32_t         ppt;   // particles per thread of a fixed chunk
+  const uint32_t* toff;  // tile offsets (nchunks + 1 entries) or null
+  uint32_t*       err;
 };
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -144,7 +146,7 @@ __device__ __forceinline__ void store_state(const ParticleView& P, int64_t p, co
   P.ux3[p] = s.u[2];
 }
 
-template <int D, int O, int ALG, int PPT>
+template <int D, int O, int ALG>
 __global__ __launch_bounds__(kThreads) void k_push_deposit(FusedArgs a) {
   constexpr int W1 = Win<D>::W1, W2 = Win<D>::W2, W3 = Win<D>::W3;
   constexpr int WC = W1 * W2 * W3;
@@ -164,18 +166,24 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(FusedArgs a) {
     (xcd < full) ? xcd * per + slot : full * per + (xcd - full) * (per - 1) + slot;
   if (slot >= ((xcd < full) ? per : per - 1)) return;
 
-  const int     tid    = threadIdx.x;
-  const int64_t base   = chunk * (kThreads * PPT);
-  const int     G      = a.ngh;
-  const bool    do_dep = a.charge != ZERO;
+  const int tid    = threadIdx.x;
+  const int G      = a.ngh;
+  const bool do_dep = a.charge != ZERO;
+  int64_t    pbeg, pend;
+  if (a.toff != nullptr) {
+    pbeg = a.toff[chunk];
+    pend = a.toff[chunk + 1];
+  } else {
+    pbeg = chunk * (static_cast<int64_t>(kThreads) * a.ppt);
+    pend = pbeg + static_cast<int64_t>(kThreads) * a.ppt;
+  }
+  pend = min(pend, a.P.n);
 
   // ---- pass 1: cell indices and tags -> bounding box of live particles -----
   // (pass 2 reads i and tag again; the re-read hits in L2)
   int lo[3] = { INT_MAX, INT_MAX, INT_MAX }, hi[3] = { INT_MIN, INT_MIN, INT_MIN };
-#pragma unroll
-  for (int k = 0; k < PPT; ++k) {
-    const int64_t p = base + k * kThreads + tid;
-    if (p < a.P.n) {
+  for (int64_t p = pbeg + tid; p < pend; p += kThreads) {
+    {
       const int16_t tag = a.P.tag[p];
       if (tag == NTT_TAG_ALIVE) {
         int c[3] = { a.P.i1[p], a.P.i2[p], 0 };
@@ -227,9 +235,8 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(FusedArgs a) {
   if (!fits) {
     // ---- HBM path -------------------------------------------------------------
 #pragma unroll 1
-    for (int k = 0; k < PPT; ++k) {
-      const int64_t p = base + k * kThreads + tid;
-      if (p >= a.P.n || a.P.tag[p] != NTT_TAG_ALIVE) continue;
+    for (int64_t p = pbeg + tid; p < pend; p += kThreads) {
+      if (a.P.tag[p] != NTT_TAG_ALIVE) continue;
       PState s;
       s.tag = NTT_TAG_ALIVE;
       load_state<D>(a.P, p, s);
@@ -285,9 +292,8 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(FusedArgs a) {
   const LdsEM<D> F{ em_w, o1, o2, o3 };
   LdsJ<D>        jw{ j_w, o1, o2, o3 };
 #pragma unroll 1
-  for (int k = 0; k < PPT; ++k) {
-    const int64_t p = base + k * kThreads + tid;
-    if (p >= a.P.n || a.P.tag[p] != NTT_TAG_ALIVE) continue;
+  for (int64_t p = pbeg + tid; p < pend; p += kThreads) {
+    if (a.P.tag[p] != NTT_TAG_ALIVE) continue;
     PState s;
     s.tag = NTT_TAG_ALIVE;
     load_state<D>(a.P, p, s);
@@ -336,12 +342,9 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(FusedArgs a) {
 }
 
 template <int D, int O, int ALG>
-void launch_fused(const FusedArgs& a0, hipStream_t st) {
-  constexpr int PPT = 4;
-  FusedArgs a   = a0;
-  a.nchunks     = (a.P.n + kThreads * PPT - 1) / (kThreads * PPT);
+void launch_fused(const FusedArgs& a, hipStream_t st) {
   if (a.nchunks == 0) return;
-  k_push_deposit<D, O, ALG, PPT><<<static_cast<unsigned>(a.nchunks), kThreads, 0, st>>>(a);
+  k_push_deposit<D, O, ALG><<<static_cast<unsigned>(a.nchunks), kThreads, 0, st>>>(a);
 }
 
 template <int D, int ALG>
@@ -359,7 +362,8 @@ void launch_fused_o(int order, const FusedArgs& a, hipStream_t st) {
 ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView& em,
                                         const FieldView& cur, int D, int order, int alg,
                                         float ndth, float dt, float dxm, float charge,
-                                        bool deposit, hipStream_t st) {
+                                        bool deposit, const uint32_t* tile_offsets,
+                                        int64_t ntiles, int ppt, hipStream_t st) {
   if (pv.n == 0) return NTT_OK;
   uint32_t* err = device_error_word();
   if (err == nullptr) return NTT_ERR_HIP;
@@ -377,7 +381,15 @@ ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView&
   a.charge = deposit ? charge : ZERO;
   a.inv_dt = ONE / dt;
   a.err = err;
-  a.nchunks = 0;
+  a.toff = tile_offsets;
+  a.ppt = ppt < 1 ? 1 : ppt;
+  if (tile_offsets != nullptr) {
+    a.nchunks = ntiles;
+  } else {
+    const int64_t per = static_cast<int64_t>(kThreads) * a.ppt;
+    a.nchunks = (pv.n + per - 1) / per;
+  }
+  if (a.nchunks > 0x7FFFFFFF) return NTT_ERR_UNSUPPORTED;
   if (D == 2) {
     if (alg == ALG_BORIS) launch_fused_o<2, ALG_BORIS>(order, a, st);
     else if (alg == ALG_VAY) launch_fused_o<2, ALG_VAY>(order, a, st);

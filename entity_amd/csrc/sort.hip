@@ -1,0 +1,287 @@
+// sort.hip -- spatial sort of a species by tile (counting sort), which also
+// compacts dead particles to the tail.
+//
+// Replaces Particles::SortSpatially (BinSort by cell key, bins unsorted
+// inside) and Particles::RemoveDead (dead particles moved past the live ones,
+// npart reduced), src/framework/containers/particles_sort.cpp:103-253, with
+// the key of sort::PositionToTileIndex (src/global/utils/sorting.h:65-137):
+// tiles of tile[0] x tile[1] (x tile[2]) cells, numbered x fastest; dead
+// particles get the key one past the last tile.
+//
+// Steps: key + histogram (atomics) -> exclusive scan of the histogram ->
+// scatter of particle indices (atomic cursor per key, so the order inside a
+// tile is the arrival order) -> gather of every SoA array through a scratch
+// buffer.  All scratch lives in the caller's workspace.
+#include "internal.hpp"
+
+namespace ntt {
+namespace {
+
+constexpr int kB = 256;
+constexpr int kScanItems = 4;                 // per thread in the block scan
+constexpr int kScanTile  = kB * kScanItems;   // elements per scan block
+
+struct TileKey {
+  int32_t n[3], t[3], nt[3];
+  int32_t dim;
+  uint32_t dead_key;
+  __device__ __forceinline__ uint32_t operator()(int i1, int i2, int i3) const {
+    // particles outside [0, n) are clamped to the edge tile
+    i1 = min(max(i1, 0), n[0] - 1);
+    i2 = min(max(i2, 0), n[1] - 1);
+    const uint32_t a = i1 / t[0], b = i2 / t[1];
+    if (dim == 2) return a + static_cast<uint32_t>(nt[0]) * b;
+    i3 = min(max(i3, 0), n[2] - 1);
+    const uint32_t c = i3 / t[2];
+    return a + static_cast<uint32_t>(nt[0]) * (b + static_cast<uint32_t>(nt[1]) * c);
+  }
+};
+
+__global__ void k_zero_u32(uint32_t* d, int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kB)
+    d[i] = 0;
+}
+
+__global__ __launch_bounds__(kB) void k_keys(const int32_t* __restrict__ i1,
+                                             const int32_t* __restrict__ i2,
+                                             const int32_t* __restrict__ i3,
+                                             const int16_t* __restrict__ tag, int64_t n,
+                                             TileKey K, uint32_t* __restrict__ keys,
+                                             uint32_t* __restrict__ counts) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (p >= n) return;
+  uint32_t k;
+  if (tag[p] == NTT_TAG_DEAD) k = K.dead_key;
+  else k = K(i1[p], i2[p], K.dim == 3 ? i3[p] : 0);
+  keys[p] = k;
+  atomicAdd(counts + k, 1u);
+}
+
+// block-level exclusive scan of kScanTile elements; writes block totals
+__global__ __launch_bounds__(kB) void k_scan_block(uint32_t* __restrict__ data, int64_t n,
+                                                   uint32_t* __restrict__ block_sums) {
+  __shared__ uint32_t s[kB];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+  uint32_t v[kScanItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + threadIdx.x * kScanItems + k;
+    v[k] = (i < n) ? data[i] : 0u;
+    sum += v[k];
+  }
+  s[threadIdx.x] = sum;
+  __syncthreads();
+  // Hillis-Steele inclusive scan of the per-thread sums
+  for (int off = 1; off < kB; off <<= 1) {
+    const uint32_t t = (threadIdx.x >= off) ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  uint32_t run = s[threadIdx.x] - sum;  // exclusive prefix of this thread
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + threadIdx.x * kScanItems + k;
+    if (i < n) data[i] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == kB - 1) block_sums[blockIdx.x] = s[kB - 1];
+}
+
+__global__ __launch_bounds__(kB) void k_scan_add(uint32_t* __restrict__ data, int64_t n,
+                                                 const uint32_t* __restrict__ block_prefix) {
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+  const uint32_t add = block_prefix[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + threadIdx.x * kScanItems + k;
+    if (i < n) data[i] += add;
+  }
+}
+
+__global__ __launch_bounds__(kB) void k_scatter(const uint32_t* __restrict__ keys, int64_t n,
+                                                uint32_t* __restrict__ cursor,
+                                                uint32_t* __restrict__ perm) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t pos = atomicAdd(cursor + keys[p], 1u);
+  perm[pos] = static_cast<uint32_t>(p);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kB) void k_gather(T* __restrict__ dst, const T* __restrict__ src,
+                                               const uint32_t* __restrict__ perm, int64_t n) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (p >= n) return;
+  dst[p] = src[perm[p]];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kB) void k_copy_t(T* __restrict__ dst, const T* __restrict__ src,
+                                               int64_t n) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (p >= n) return;
+  dst[p] = src[p];
+}
+
+// exclusive scan of n uint32 in place; tmp holds the block sums of every level
+ntt_status scan_exclusive(uint32_t* data, int64_t n, uint32_t* tmp, hipStream_t st) {
+  const int64_t nb = (n + kScanTile - 1) / kScanTile;
+  k_scan_block<<<static_cast<unsigned>(nb), kB, 0, st>>>(data, n, tmp);
+  if (nb > 1) {
+    ntt_status s = scan_exclusive(tmp, nb, tmp + nb, st);
+    if (s != NTT_OK) return s;
+    k_scan_add<<<static_cast<unsigned>(nb), kB, 0, st>>>(data, n, tmp);
+  }
+  return check_launch();
+}
+
+int64_t scan_tmp_elems(int64_t n) {
+  int64_t total = 0;
+  while (n > 1) {
+    n = (n + kScanTile - 1) / kScanTile;
+    total += n;
+  }
+  return total + 1;
+}
+
+struct Layout {
+  int64_t keys, perm, counts, cursor, scan, tmp, total;  // byte offsets
+  int64_t nkeys;
+};
+
+int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+ntt_status make_layout(int64_t npart, int32_t dim, const int32_t* n, const int32_t* tile,
+                       TileKey* K, Layout* L) {
+  if (dim != 2 && dim != 3) return NTT_ERR_UNSUPPORTED;
+  if (npart < 0 || npart > 0xFFFFFFFFll) return NTT_ERR_ARG;
+  int64_t ntiles = 1;
+  for (int d = 0; d < dim; ++d) {
+    if (n[d] < 1 || tile[d] < 1) return NTT_ERR_ARG;
+    K->n[d]  = n[d];
+    K->t[d]  = tile[d];
+    K->nt[d] = (n[d] + tile[d] - 1) / tile[d];
+    ntiles *= K->nt[d];
+  }
+  for (int d = dim; d < 3; ++d) { K->n[d] = 1; K->t[d] = 1; K->nt[d] = 1; }
+  if (ntiles >= 0xFFFFFFFFll) return NTT_ERR_ARG;
+  K->dim      = dim;
+  K->dead_key = static_cast<uint32_t>(ntiles);
+  L->nkeys    = ntiles + 1;
+  int64_t off = 0;
+  L->keys   = off; off = align256(off + 4 * npart);
+  L->perm   = off; off = align256(off + 4 * npart);
+  L->counts = off; off = align256(off + 4 * L->nkeys);
+  L->cursor = off; off = align256(off + 4 * L->nkeys);
+  L->scan   = off; off = align256(off + 4 * scan_tmp_elems(L->nkeys));
+  L->tmp    = off; off = align256(off + 8 * npart);
+  L->total  = off;
+  return NTT_OK;
+}
+
+unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kB - 1) / kB); }
+
+template <typename T>
+void permute(T* arr, T* tmp, const uint32_t* perm, int64_t n, hipStream_t st) {
+  if (arr == nullptr || n == 0) return;
+  k_gather<T><<<blocks_for(n), kB, 0, st>>>(tmp, arr, perm, n);
+  k_copy_t<T><<<blocks_for(n), kB, 0, st>>>(arr, tmp, n);
+}
+
+}  // namespace
+}  // namespace ntt
+
+using namespace ntt;
+
+extern "C" ntt_status ntt_sort_workspace_size(int64_t maxnpart, int32_t dim, const int32_t n[3],
+                                              const int32_t tile[3], uint64_t* bytes) {
+  if (n == nullptr || tile == nullptr || bytes == nullptr) return NTT_ERR_ARG;
+  TileKey K;
+  Layout  L;
+  ntt_status s = make_layout(maxnpart, dim, n, tile, &K, &L);
+  if (s != NTT_OK) return s;
+  *bytes = static_cast<uint64_t>(L.total);
+  return NTT_OK;
+}
+
+extern "C" ntt_status ntt_sort_ntiles(int32_t dim, const int32_t n[3], const int32_t tile[3],
+                                      int64_t* ntiles) {
+  if (n == nullptr || tile == nullptr || ntiles == nullptr) return NTT_ERR_ARG;
+  TileKey K;
+  Layout  L;
+  ntt_status s = make_layout(0, dim, n, tile, &K, &L);
+  if (s != NTT_OK) return s;
+  *ntiles = L.nkeys - 1;
+  return NTT_OK;
+}
+
+extern "C" ntt_status ntt_sort_particles(ntt_particles_t* prtls, int32_t dim, const int32_t n[3],
+                                         const int32_t tile[3], void* workspace,
+                                         uint64_t workspace_bytes, uint32_t* tile_offsets,
+                                         int64_t* npart_alive, void* stream) {
+  if (prtls == nullptr || n == nullptr || tile == nullptr) return NTT_ERR_ARG;
+  ParticleView pv;
+  ntt_status s = make_particle_view(prtls, dim, true, &pv);
+  if (s != NTT_OK) return s;
+  TileKey K;
+  Layout  L;
+  s = make_layout(pv.n, dim, n, tile, &K, &L);
+  if (s != NTT_OK) return s;
+  if (pv.n == 0) {
+    if (tile_offsets != nullptr) {
+      k_zero_u32<<<blocks_for(L.nkeys) > 4096 ? 4096 : blocks_for(L.nkeys), kB, 0,
+                   as_stream(stream)>>>(tile_offsets, L.nkeys);
+      s = check_launch();
+      if (s != NTT_OK) return s;
+    }
+    if (npart_alive) *npart_alive = 0;
+    return NTT_OK;
+  }
+  if (workspace == nullptr || static_cast<int64_t>(workspace_bytes) < L.total)
+    return NTT_ERR_CAPACITY;
+  hipStream_t st   = as_stream(stream);
+  char*       ws   = static_cast<char*>(workspace);
+  uint32_t*   keys = reinterpret_cast<uint32_t*>(ws + L.keys);
+  uint32_t*   perm = reinterpret_cast<uint32_t*>(ws + L.perm);
+  uint32_t*   cnt  = reinterpret_cast<uint32_t*>(ws + L.counts);
+  uint32_t*   cur  = reinterpret_cast<uint32_t*>(ws + L.cursor);
+  uint32_t*   scn  = reinterpret_cast<uint32_t*>(ws + L.scan);
+  void*       tmp  = ws + L.tmp;
+  const int64_t np = pv.n;
+
+  k_zero_u32<<<blocks_for(L.nkeys) > 4096 ? 4096 : blocks_for(L.nkeys), kB, 0, st>>>(cnt, L.nkeys);
+  k_keys<<<blocks_for(np), kB, 0, st>>>(pv.i1, pv.i2, pv.i3, pv.tag, np, K, keys, cnt);
+  s = scan_exclusive(cnt, L.nkeys, scn, st);
+  if (s != NTT_OK) return s;
+  if (hipMemcpyAsync(cur, cnt, 4 * L.nkeys, hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return NTT_ERR_HIP;
+  k_scatter<<<blocks_for(np), kB, 0, st>>>(keys, np, cur, perm);
+  if (tile_offsets != nullptr &&
+      hipMemcpyAsync(tile_offsets, cnt, 4 * L.nkeys, hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return NTT_ERR_HIP;
+
+  int32_t* i32[6] = { prtls->i1, prtls->i2, prtls->i3, prtls->i1_prev, prtls->i2_prev, prtls->i3_prev };
+  float*   f32[10] = { prtls->dx1, prtls->dx2, prtls->dx3, prtls->dx1_prev, prtls->dx2_prev,
+                       prtls->dx3_prev, prtls->ux1, prtls->ux2, prtls->ux3, prtls->weight };
+  for (int k = 0; k < 6; ++k)
+    if ((k % 3) < dim) permute<int32_t>(i32[k], static_cast<int32_t*>(tmp), perm, np, st);
+  for (int k = 0; k < 10; ++k)
+    if (k >= 6 || (k % 3) < dim) permute<float>(f32[k], static_cast<float*>(tmp), perm, np, st);
+  permute<float>(prtls->phi, static_cast<float*>(tmp), perm, np, st);
+  permute<int16_t>(prtls->tag, static_cast<int16_t*>(tmp), perm, np, st);
+  permute<int64_t>(prtls->pld_i, static_cast<int64_t*>(tmp), perm, np, st);
+  s = check_launch();
+  if (s != NTT_OK) return s;
+  if (npart_alive != nullptr) {
+    // live particles = exclusive prefix at the dead key
+    uint32_t alive = 0;
+    if (hipMemcpyAsync(&alive, cnt + (L.nkeys - 1), 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+      return NTT_ERR_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return NTT_ERR_HIP;
+    *npart_alive = alive;
+  }
+  return NTT_OK;
+}

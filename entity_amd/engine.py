@@ -1,0 +1,220 @@
+"""SRPIC engine for one periodic domain on one GPU.
+
+Mirrors the reference's engine layer for the hot path:
+* ``SimulationParams``-style scalars derived as the reference derives them
+  (src/framework/parameters/parameters.cpp:47-78, algorithms.cpp:18-35,
+  grid.cpp:39-51): dt = CFL * dx / sqrt(D), B0 = omegaB0 = 1 / larmor0,
+  V0 = dx^D, n0 = ppc0 / V0, q0 = V0 / (ppc0 * skindepth0^2).
+* ``SRPICEngine.step_forward`` -- SRPICEngine::step_forward
+  (src/engines/srpic/srpic.hpp:65-188) through ``ntt_sr_step``.
+* ``SRPICEngine.sort`` -- Metadomain::SortParticles
+  (src/framework/domain/metadomain_sort.cpp:15-34) with a tile key, which also
+  compacts dead particles (RemoveDead).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import torch
+
+from . import _abi
+from .containers import Fields, GridShape, Particles
+
+
+@dataclass
+class SpeciesSpec:
+    label: str
+    mass: float
+    charge: float
+    pusher: int = _abi.PUSHER_BORIS
+
+
+@dataclass
+class SRParams:
+    """Hot-path keys of the reference's input (input.example.toml:300-420)."""
+    dim: int
+    n: tuple
+    extent: tuple = None            # ((x1min, x1max), ...); default [0, n_d]
+    cfl: float = 0.5
+    correction: float = 1.0
+    larmor0: float = 1.0
+    skindepth0: float = 1.0
+    ppc0: float = 16.0
+    shape_order: int = 1
+    n_filters: int = 0
+    fieldsolver_enable: bool = True
+    deposit_enable: bool = True
+    fd: dict = field(default_factory=dict)
+    sort_interval: int = 0          # 0: never re-sort
+    tile: tuple = None              # sort / generation tile
+
+    def __post_init__(self):
+        if self.extent is None:
+            self.extent = tuple((0.0, float(x)) for x in self.n)
+        if self.tile is None:
+            self.tile = (8, 8) if self.dim == 2 else (4, 4, 4)
+
+    # derived scalars (single precision, as the FP32 reference build holds them)
+    @property
+    def dx(self) -> float:
+        return (self.extent[0][1] - self.extent[0][0]) / self.n[0]
+
+    @property
+    def dt(self) -> float:
+        # dt = CFL * dxMin, dxMin = dx / sqrt(D), in single precision
+        # (algorithms.cpp:21-22, minkowski.h:82-84)
+        import numpy as np
+        dx_min = np.float32(self.dx) / np.sqrt(np.float32(self.dim))
+        return float(np.float32(self.cfl) * np.float32(dx_min))
+
+    @property
+    def omegaB0(self) -> float:
+        return 1.0 / self.larmor0
+
+    @property
+    def B0(self) -> float:
+        return 1.0 / self.larmor0
+
+    @property
+    def V0(self) -> float:
+        return self.dx ** self.dim
+
+    @property
+    def q0(self) -> float:
+        return self.V0 / (self.ppc0 * self.skindepth0 ** 2)
+
+
+class SRPICEngine:
+    def __init__(self, params: SRParams, species: list[SpeciesSpec], maxnpart: int | list,
+                 *, device="cuda", with_payload=False, strict=False):
+        self.params = params
+        self.device = torch.device(device)
+        self.shape = GridShape(params.dim, tuple(params.n))
+        self.fields = Fields(self.shape, device=self.device)
+        if not isinstance(maxnpart, (list, tuple)):
+            maxnpart = [maxnpart] * len(species)
+        self.species = [
+            Particles(params.dim, m, device=self.device, label=s.label, mass=s.mass,
+                      charge=s.charge, pusher=s.pusher, with_payload=with_payload)
+            for s, m in zip(species, maxnpart)]
+        self.step = 0
+        self.time = 0.0
+        self._sort_ws = None
+        self.lib = _abi.load(strict)
+        # tile offsets of every species (device uint32, ntiles + 1), or None
+        self.tile_offsets: list = [None] * len(self.species)
+        self.ntiles = _ntiles(self.lib, params)
+
+    # -- initial conditions ---------------------------------------------------
+    def inject_uniform(self, ppc: int, *, temperature: float, drifts=None, seed=1,
+                       position_seed=7, weight=1.0):
+        """Fills every species with ppc particles per cell (co-located across
+        species); drifts[s] is a 3-vector of drift 4-velocity for species s."""
+        n = (C.c_int32 * 3)(*(list(self.params.n) + [1] * (3 - self.params.dim)))
+        for s, sp in enumerate(self.species):
+            init = _abi.NttPlasmaInit()
+            init.seed = seed
+            init.position_seed = position_seed
+            init.species_id = s
+            init.ppc = ppc
+            init.temperature = temperature
+            d = (drifts[s] if drifts is not None else (0.0, 0.0, 0.0))
+            init.drift_u[0], init.drift_u[1], init.drift_u[2] = d
+            init.weight = weight
+            t = list(self.params.tile) + [1] * (3 - self.params.dim)
+            init.tile[0], init.tile[1], init.tile[2] = t
+            abi = sp.abi()
+            _abi.check(self.lib.ntt_init_uniform_plasma(C.byref(abi), self.params.dim, n,
+                                                         C.byref(init), _stream()),
+                       "ntt_init_uniform_plasma")
+            sp.npart = abi.npart
+            # the generator lays particles out tile by tile: ppc * cells_per_tile each
+            per_tile = ppc * math.prod(self.params.tile[: self.params.dim])
+            self.tile_offsets[s] = (torch.arange(self.ntiles + 1, device=self.device,
+                                                 dtype=torch.int64) * per_tile
+                                    ).to(torch.int32)
+
+    # -- one step -----------------------------------------------------------------
+    def step_params(self) -> _abi.NttSrStepParams:
+        p = self.params
+        s = _abi.NttSrStepParams()
+        s.dt, s.correction, s.omegaB0 = p.dt, p.correction, p.omegaB0
+        s.q0, s.B0, s.V0, s.ppc0 = p.q0, p.B0, p.V0, p.ppc0
+        s.shape_order, s.n_filters = p.shape_order, p.n_filters
+        s.fieldsolver_enable, s.deposit_enable = int(p.fieldsolver_enable), int(p.deposit_enable)
+        s.step = self.step
+        for k, v in p.fd.items():
+            setattr(s.fd, k, v)
+        for d in range(3):
+            s.prtl_bc.bc[d][0] = s.prtl_bc.bc[d][1] = _abi.PRTL_BC_PERIODIC if d < p.dim else 0
+        return s
+
+    def species_abi(self):
+        arr = (_abi.NttSpecies * len(self.species))()
+        for k, sp in enumerate(self.species):
+            arr[k].prtls = sp.abi()
+            arr[k].mass, arr[k].charge, arr[k].pusher_flags = sp.mass, sp.charge, sp.pusher
+            off = self.tile_offsets[k]
+            arr[k].tile_offsets = off.data_ptr() if off is not None else None
+            arr[k].ntiles = self.ntiles if off is not None else 0
+        return arr
+
+    def step_forward(self, stream=None) -> None:
+        p = self.params
+        if p.sort_interval and self.step > 0 and self.step % p.sort_interval == 0:
+            self.sort(stream=stream)
+        em, cur, buff = (self.fields.abi(x) for x in ("em", "cur", "buff"))
+        sp = self.species_abi()
+        prm = self.step_params()
+        metric = _abi.NttMinkowski(p.dim, p.dx)
+        _abi.check(self.lib.ntt_sr_step(C.byref(em), C.byref(cur), C.byref(buff), sp,
+                                        len(self.species), C.byref(prm), C.byref(metric),
+                                        _stream(stream)), "ntt_sr_step")
+        self.step += 1
+        self.time += p.dt
+
+    # -- spatial sort ----------------------------------------------------------------
+    def sort(self, stream=None, compact: bool = False) -> None:
+        p = self.params
+        n = (C.c_int32 * 3)(*(list(p.n) + [1] * (3 - p.dim)))
+        tile = (C.c_int32 * 3)(*(list(p.tile) + [1] * (3 - p.dim)))
+        need = max(_sort_ws_bytes(self.lib, sp.maxnpart, p.dim, n, tile) for sp in self.species)
+        if self._sort_ws is None or self._sort_ws.numel() < need:
+            self._sort_ws = torch.empty(need, dtype=torch.uint8, device=self.device)
+        for k, sp in enumerate(self.species):
+            abi = sp.abi()
+            alive = C.c_int64(0)
+            if self.tile_offsets[k] is None:
+                self.tile_offsets[k] = torch.zeros(self.ntiles + 1, dtype=torch.int32,
+                                                   device=self.device)
+            _abi.check(self.lib.ntt_sort_particles(C.byref(abi), p.dim, n, tile,
+                                                   self._sort_ws.data_ptr(),
+                                                   self._sort_ws.numel(),
+                                                   self.tile_offsets[k].data_ptr(),
+                                                   C.byref(alive) if compact else None,
+                                                   _stream(stream)), "ntt_sort_particles")
+            if compact:
+                sp.npart = int(alive.value)
+
+
+def _ntiles(lib, p: SRParams) -> int:
+    n = (C.c_int32 * 3)(*(list(p.n) + [1] * (3 - p.dim)))
+    tile = (C.c_int32 * 3)(*(list(p.tile) + [1] * (3 - p.dim)))
+    v = C.c_int64(0)
+    _abi.check(lib.ntt_sort_ntiles(p.dim, n, tile, C.byref(v)), "ntt_sort_ntiles")
+    return int(v.value)
+
+
+def _sort_ws_bytes(lib, maxnpart, dim, n, tile) -> int:
+    b = C.c_uint64(0)
+    _abi.check(lib.ntt_sort_workspace_size(maxnpart, dim, n, tile, C.byref(b)),
+               "ntt_sort_workspace_size")
+    return int(b.value)
+
+
+def _stream(stream=None):
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)

@@ -244,7 +244,10 @@ typedef struct ntt_species {
   ntt_particles_t prtls;
   float           mass;
   float           charge;
-  uint8_t         pusher_flags; /* NTT_PUSHER_*; NONE skips the species */
+  uint8_t         pusher_flags; /* NTT_PUSHER_*; NONE skips the species             */
+  const uint32_t* tile_offsets; /* device: first particle of every tile (ntiles + 1
+                                   entries) from the last ntt_sort_particles, or NULL */
+  int64_t         ntiles;
 } ntt_species_t;
 
 /* Scalars of one SRPIC step (engine params and SimulationParams the drivers
@@ -262,6 +265,8 @@ typedef struct ntt_sr_step_params {
   int64_t         step;               /* step 0 fills the E/B ghosts first          */
   ntt_fd_coeffs_t fd;                 /* extended Faraday stencil (zero = Yee)      */
   ntt_prtl_bc_t   prtl_bc;            /* must be periodic in every direction        */
+  int32_t         particles_per_thread; /* chunk = 256 * this when a species has no
+                                           tile offsets (0: 4)                        */
 } ntt_sr_step_params_t;
 
 /* One full SRPIC step of a periodic box, in the reference's phase order:
@@ -281,6 +286,74 @@ NTT_API ntt_status ntt_sr_step(ntt_field_t*                em,
                                const ntt_sr_step_params_t* prm,
                                const ntt_minkowski_t*      metric,
                                void*                       stream);
+
+/* Launch timing of the fused push+deposit kernels that ntt_sr_step enqueues:
+ * while enabled, every such launch is bracketed by HIP events recorded on its
+ * stream; ntt_profile_read waits for them and returns the summed elapsed
+ * milliseconds and the number of launches (reset != 0 clears the record).
+ * No reference counterpart (the reference wraps whole phases in wall-clock
+ * timers, src/global/utils/timer.h:51-148). */
+NTT_API ntt_status ntt_profile_enable(int32_t on);
+NTT_API ntt_status ntt_profile_read(double* ms_total, int64_t* launches, int32_t reset);
+
+/* Spatial sort of one species by tile, dead particles last.
+ * Replaces Particles::SortSpatially + RemoveDead
+ * (src/framework/containers/particles_sort.cpp:103-253) with the tile key of
+ * sort::PositionToTileIndex (src/global/utils/sorting.h:65-137): tiles of
+ * tile[0] x tile[1] (x tile[2]) cells numbered x fastest.  Every SoA array
+ * (prev arrays, phi and pld_i included when non-NULL) is permuted.  The order
+ * inside a tile is unspecified.  `workspace` must hold
+ * ntt_sort_workspace_size() bytes.  When npart_alive is non-NULL the call
+ * waits for the stream and returns the number of live particles (which lead
+ * the arrays afterwards); the caller may then shrink npart to it.  When
+ * tile_offsets (device, ntiles + 1 entries) is non-NULL it receives the index
+ * of the first particle of every tile, for ntt_species_t.tile_offsets. */
+NTT_API ntt_status ntt_sort_workspace_size(int64_t       maxnpart,
+                                           int32_t       dim,
+                                           const int32_t n[3],
+                                           const int32_t tile[3],
+                                           uint64_t*     bytes);
+NTT_API ntt_status ntt_sort_particles(ntt_particles_t* prtls,
+                                      int32_t          dim,
+                                      const int32_t    n[3],
+                                      const int32_t    tile[3],
+                                      void*            workspace,
+                                      uint64_t         workspace_bytes,
+                                      uint32_t*        tile_offsets,
+                                      int64_t*         npart_alive,
+                                      void*            stream);
+
+/* number of tiles of the key used by ntt_sort_particles (tile_offsets of the
+ * sort then holds ntiles + 1 entries, the last one counting the live particles) */
+NTT_API ntt_status ntt_sort_ntiles(int32_t dim, const int32_t n[3], const int32_t tile[3],
+                                   int64_t* ntiles);
+
+/* Seeded synthetic plasma: ppc particles per cell over the n[] box, thermal
+ * 4-velocities (Gaussian in u for temperature < 0.5, Sobol rejection above,
+ * as archetypes' JuttnerSinge, src/archetypes/energy_dist.h:79-135) boosted
+ * along one principal axis by the flipping method (:136-170).  Particles are
+ * laid out tile by tile (tiles of tile[] cells, cells visited round-robin
+ * inside a tile), i.e. already sorted for ntt_sort_particles with the same
+ * tile.  Species sharing position_seed are co-located particle by particle.
+ * Sets prtls->npart.  No reference counterpart as an interface: it stands in
+ * for the reference's injectors (src/kernels/injectors.hpp) to produce the
+ * benchmark inputs. */
+typedef struct ntt_plasma_init {
+  uint64_t seed;          /* velocity stream seed                   */
+  uint64_t position_seed; /* position stream seed                   */
+  int32_t  species_id;    /* separates the velocity streams          */
+  int32_t  ppc;           /* particles per cell                      */
+  float    temperature;   /* theta, in units of m c^2                */
+  float    drift_u[3];    /* drift 4-velocity (one non-zero entry)   */
+  float    weight;
+  int32_t  tile[3];       /* generation tile; must divide n          */
+} ntt_plasma_init_t;
+
+NTT_API ntt_status ntt_init_uniform_plasma(ntt_particles_t*         prtls,
+                                           int32_t                  dim,
+                                           const int32_t            n[3],
+                                           const ntt_plasma_init_t* init,
+                                           void*                    stream);
 
 #ifdef __cplusplus
 }
