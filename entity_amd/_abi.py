@@ -135,7 +135,9 @@ SIGNATURES: dict[str, tuple] = {
     "ntt_communicate_fields": (C.c_int, [_P(NttField), C.c_int32, C.c_int32, C.c_void_p]),
     "ntt_synchronize_currents": (C.c_int, [_P(NttField), _P(NttField), C.c_void_p]),
     "ntt_sr_step": (C.c_int, [_P(NttField), _P(NttField), _P(NttField), _P(NttSpecies), C.c_int32,
-                              _P(NttSrStepParams), _P(NttMinkowski), C.c_void_p]),
+                              _P(NttSrStepParams), _P(NttMinkowski), C.c_void_p, C.c_uint64,
+                              C.c_void_p]),
+    "ntt_sr_step_workspace_size": (C.c_int, [_P(NttField), _P(C.c_uint64)]),
     "ntt_sort_workspace_size": (C.c_int, [C.c_int64, C.c_int32, _P(C.c_int32), _P(C.c_int32),
                                           _P(C.c_uint64)]),
     "ntt_sort_particles": (C.c_int, [_P(NttParticles), C.c_int32, _P(C.c_int32), _P(C.c_int32),

@@ -58,10 +58,19 @@ bool fd_nonzero(const ntt_fd_coeffs_t& f) {
 
 extern "C" ntt_status ntt_faraday_mink(ntt_field_t*, float, float, const ntt_fd_coeffs_t*, void*);
 
+extern "C" ntt_status ntt_sr_step_workspace_size(const ntt_field_t* cur, uint64_t* bytes) {
+  if (bytes == nullptr) return NTT_ERR_ARG;
+  FieldView J;
+  ntt_status s = make_field_view(cur, 3, &J);
+  if (s != NTT_OK) return s;
+  *bytes = static_cast<uint64_t>(J.cstride) * 3 * sizeof(unsigned long long);
+  return NTT_OK;
+}
+
 extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t* buff,
                                   ntt_species_t* species, int32_t nspecies,
                                   const ntt_sr_step_params_t* prm, const ntt_minkowski_t* metric,
-                                  void* stream) {
+                                  void* workspace, uint64_t workspace_bytes, void* stream) {
   if (prm == nullptr || metric == nullptr || (nspecies > 0 && species == nullptr) || nspecies < 0)
     return NTT_ERR_ARG;
   FieldView E, J, Bf;
@@ -72,6 +81,10 @@ extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t
   s = make_field_view(buff, 3, &Bf);
   if (s != NTT_OK) return s;
   if (!same_grid(E, J) || !same_grid(E, Bf) || J.d == Bf.d) return NTT_ERR_ARG;
+  const uint64_t ws_need = static_cast<uint64_t>(J.cstride) * 3 * sizeof(unsigned long long);
+  if (prm != nullptr && prm->deposit_enable && (workspace == nullptr || workspace_bytes < ws_need))
+    return NTT_ERR_CAPACITY;
+  unsigned long long* acc = static_cast<unsigned long long*>(workspace);
   const int D = em->dim;
   if (metric->dim != D || !(metric->dx > 0.0f) || !(prm->dt > 0.0f)) return NTT_ERR_ARG;
   if (E.ngh != 2) return NTT_ERR_UNSUPPORTED;
@@ -134,26 +147,26 @@ extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t
     s = faraday_half();
     if (s != NTT_OK) return s;
   }
-  // currents.h:68: J = 0 (a fill kernel rather than a memset, so that the
-  // step can be captured in a graph)
-  if (prm->deposit_enable) launch_zero(J.d, J.cstride * 3, st);
+  // currents.h:68: J = 0 -- here the fixed-point accumulator (a fill kernel
+  // rather than a memset, so that the step can be captured in a graph)
+  if (prm->deposit_enable) launch_zero_u64(acc, J.cstride * 3, st);
   for (int sp = 0; sp < nspecies; ++sp) {
     const Prep& p = prep[sp];
     if (!p.run) continue;
     const int ppt = prm->particles_per_thread > 0 ? prm->particles_per_thread : 4;
     std::pair<hipEvent_t, hipEvent_t> ev;
     const bool timed = prof_begin(st, &ev);
-    s = launch_push_deposit_periodic(p.pv, E, J, D, prm->shape_order, p.alg, p.ndth, dt,
+    s = launch_push_deposit_periodic(p.pv, E, J, acc, D, prm->shape_order, p.alg, p.ndth, dt,
                                      metric->dx, species[sp].charge, p.dep,
                                      species[sp].tile_offsets, species[sp].ntiles, ppt, st);
     if (timed) prof_end(st, ev);
     if (s != NTT_OK) return s;
   }
   if (prm->deposit_enable) {
+    // folded current (active cells + periodic ghosts) from the accumulator
+    launch_j_from_fixed(J, acc, st);
     // currents.h:89-119: n filter passes (ping-pong between cur and buff)
-    if (prm->n_filters == 0) {
-      launch_ghost_fill(J, 0, 3, st);
-    } else {
+    if (prm->n_filters > 0) {
       const FieldView* src = &J;
       const FieldView* dst = &Bf;
       for (int f = 0; f < prm->n_filters; ++f) {

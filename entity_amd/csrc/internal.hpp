@@ -17,7 +17,8 @@ ntt_status launch_deposit(const ParticleView& pv, const FieldView& J, int D, int
 
 // push_deposit.hip
 ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView& em,
-                                        const FieldView& cur, int D, int order, int alg,
+                                        const FieldView& cur, unsigned long long* acc,
+                                        int D, int order, int alg,
                                         float ndth, float dt, float dxm, float charge,
                                         bool deposit, const uint32_t* tile_offsets,
                                         int64_t ntiles, int ppt, hipStream_t st);
@@ -31,5 +32,8 @@ void launch_ghost_fill(const FieldView& F, int c0, int c1, hipStream_t st);
 void launch_fold(const FieldView& J, hipStream_t st);
 void launch_zero(float* d, int64_t n, hipStream_t st);
 void launch_copy(float* dst, const float* src, int64_t n, hipStream_t st);
+void launch_zero_u64(unsigned long long* d, int64_t n, hipStream_t st);
+// J (active cells and their periodic ghost copies) = acc * 2^-32
+void launch_j_from_fixed(const FieldView& J, const unsigned long long* acc, hipStream_t st);
 
 }  // namespace ntt

@@ -82,6 +82,15 @@ struct GlobalJ {
   float*  d;
   int32_t N1, N2;
   int64_t cstride;
+  int     i0 = 0, j0 = 0, k0 = 0;
+  __device__ __forceinline__ void set_origin(int i, int j, int k) {
+    i0 = i;
+    j0 = j;
+    k0 = k;
+  }
+  __device__ __forceinline__ void add_rel(int a, int b, int g, int c, float v) {
+    add(i0 + a, j0 + b, k0 + g, c, v);
+  }
   __device__ __forceinline__ void add(int i, int j, int k, int c, float v) {
     unsafeAtomicAdd(d + static_cast<int64_t>(i) +
                       static_cast<int64_t>(N1) * (static_cast<int64_t>(j) + static_cast<int64_t>(N2) * k) +

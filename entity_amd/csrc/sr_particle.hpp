@@ -381,7 +381,9 @@ __device__ __forceinline__ void sr_update(const A& F, int ngh, float ndth, float
 }
 
 // ---- current deposit: currents_deposit.hpp:108-761 ---------------------------
-// A "J sink" receives additions at ghost-extended indices (i, j, k, comp).
+// A "J sink" receives additions at ghost-extended indices: add(i, j, k, comp, v)
+// at absolute indices, or set_origin(i0, j0, k0) once per particle followed by
+// add_rel(a, b, g, comp, v) at (i0 + a, j0 + b, k0 + g).
 
 // velocity for the out-of-plane 2D component: currents_deposit.hpp:112-166
 __device__ __forceinline__ float vp_out_of_plane(const float* u) {
@@ -468,6 +470,7 @@ __device__ __forceinline__ void deposit_esirkepov_2d(Sink& J, int G, const PStat
   const float QVx3 = coeff * vz;
   i1_min += G;
   i2_min += G;
+  J.set_origin(i1_min, i2_min, 0);
   // jx1: prefix sum over i of -Q*Wx1
   float jx1[M][M], jx2[M][M];
 #pragma unroll
@@ -488,11 +491,11 @@ __device__ __forceinline__ void deposit_esirkepov_2d(Sink& J, int G, const PStat
   for (int a = 0; a < M; ++a)
 #pragma unroll
     for (int b = 0; b < M; ++b) {
-      if (a < d1 && b <= d2) J.add(i1_min + a, i2_min + b, 0, 0, jx1[a][b]);
-      if (a <= d1 && b < d2) J.add(i1_min + a, i2_min + b, 0, 1, jx2[a][b]);
+      if (a < d1 && b <= d2) J.add_rel(a, b, 0, 0, jx1[a][b]);
+      if (a <= d1 && b < d2) J.add_rel(a, b, 0, 1, jx2[a][b]);
       if (a <= d1 && b <= d2) {
         const float w3 = THIRD * (fS2[b] * (HALF * iS1[a] + fS1[a]) + iS2[b] * (HALF * fS1[a] + iS1[a]));
-        J.add(i1_min + a, i2_min + b, 0, 2, QVx3 * w3);
+        J.add_rel(a, b, 0, 2, QVx3 * w3);
       }
     }
 }
@@ -511,6 +514,7 @@ __device__ __forceinline__ void deposit_esirkepov_3d(Sink& J, int G, const PStat
   i1_min += G;
   i2_min += G;
   i3_min += G;
+  J.set_origin(i1_min, i2_min, i3_min);
   // jx1 (running prefix over a)
 #pragma unroll
   for (int b = 0; b < M; ++b) {
@@ -521,7 +525,7 @@ __device__ __forceinline__ void deposit_esirkepov_3d(Sink& J, int G, const PStat
 #pragma unroll
       for (int a = 0; a < M; ++a) {
         acc = (a == 0 ? ZERO : acc) - Qdt * (THIRD * (fS1[a] - iS1[a]) * t23);
-        if (a < d1 && b <= d2 && g <= d3) J.add(i1_min + a, i2_min + b, i3_min + g, 0, acc);
+        if (a < d1 && b <= d2 && g <= d3) J.add_rel(a, b, g, 0, acc);
       }
     }
   }
@@ -535,7 +539,7 @@ __device__ __forceinline__ void deposit_esirkepov_3d(Sink& J, int G, const PStat
 #pragma unroll
       for (int b = 0; b < M; ++b) {
         acc = (b == 0 ? ZERO : acc) - Qdt * (THIRD * (fS2[b] - iS2[b]) * t13);
-        if (a <= d1 && b < d2 && g <= d3) J.add(i1_min + a, i2_min + b, i3_min + g, 1, acc);
+        if (a <= d1 && b < d2 && g <= d3) J.add_rel(a, b, g, 1, acc);
       }
     }
   }
@@ -549,7 +553,7 @@ __device__ __forceinline__ void deposit_esirkepov_3d(Sink& J, int G, const PStat
 #pragma unroll
       for (int g = 0; g < M; ++g) {
         acc = (g == 0 ? ZERO : acc) - Qdt * (THIRD * (fS3[g] - iS3[g]) * t12);
-        if (a <= d1 && b <= d2 && g < d3) J.add(i1_min + a, i2_min + b, i3_min + g, 2, acc);
+        if (a <= d1 && b <= d2 && g < d3) J.add_rel(a, b, g, 2, acc);
       }
     }
   }

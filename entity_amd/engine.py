@@ -106,6 +106,12 @@ class SRPICEngine:
         # tile offsets of every species (device uint32, ntiles + 1), or None
         self.tile_offsets: list = [None] * len(self.species)
         self.ntiles = _ntiles(self.lib, params)
+        # fixed-point current accumulator used by ntt_sr_step
+        ws = C.c_uint64(0)
+        cur = self.fields.abi("cur")
+        _abi.check(self.lib.ntt_sr_step_workspace_size(C.byref(cur), C.byref(ws)),
+                   "ntt_sr_step_workspace_size")
+        self._step_ws = torch.empty(int(ws.value), dtype=torch.uint8, device=self.device)
 
     # -- initial conditions ---------------------------------------------------
     def inject_uniform(self, ppc: int, *, temperature: float, drifts=None, seed=1,
@@ -171,6 +177,7 @@ class SRPICEngine:
         metric = _abi.NttMinkowski(p.dim, p.dx)
         _abi.check(self.lib.ntt_sr_step(C.byref(em), C.byref(cur), C.byref(buff), sp,
                                         len(self.species), C.byref(prm), C.byref(metric),
+                                        self._step_ws.data_ptr(), self._step_ws.numel(),
                                         _stream(stream)), "ntt_sr_step")
         self.step += 1
         self.time += p.dt

@@ -277,7 +277,9 @@ typedef struct ntt_sr_step_params {
  * a single periodic domain.  Push and deposit run fused and LDS-windowed
  * (the species arrays should be sorted by spatial tile, see ntt_sort_particles;
  * unsorted input is handled, only slower).  `buff` is scratch (its contents
- * afterwards are unspecified). */
+ * afterwards are unspecified).  The deposit accumulates the current as 64-bit
+ * fixed point (value * 2^32) in `workspace` (ntt_sr_step_workspace_size
+ * bytes), so J does not depend on the order of the atomic additions. */
 NTT_API ntt_status ntt_sr_step(ntt_field_t*                em,
                                ntt_field_t*                cur,
                                ntt_field_t*                buff,
@@ -285,7 +287,12 @@ NTT_API ntt_status ntt_sr_step(ntt_field_t*                em,
                                int32_t                     nspecies,
                                const ntt_sr_step_params_t* prm,
                                const ntt_minkowski_t*      metric,
+                               void*                       workspace,
+                               uint64_t                    workspace_bytes,
                                void*                       stream);
+
+/* bytes of the workspace ntt_sr_step needs for a current field of this shape */
+NTT_API ntt_status ntt_sr_step_workspace_size(const ntt_field_t* cur, uint64_t* bytes);
 
 /* Launch timing of the fused push+deposit kernels that ntt_sr_step enqueues:
  * while enabled, every such launch is bracketed by HIP events recorded on its

@@ -236,9 +236,8 @@ def test_sort_orders_by_tile_and_compacts(hip):
     assert sp.npart == int((before["tag"] == 1).sum())
     k = after["pld_i"]
     # same particles, each carried with all its arrays
-    order = np.argsort(before["pld_i"])
-    inv = np.empty_like(order)
-    inv[before["pld_i"][order]] = np.arange(len(order))
+    inv = np.empty(len(before["pld_i"]), dtype=np.int64)
+    inv[before["pld_i"]] = np.arange(len(before["pld_i"]))   # id -> position before
     src = inv[k]
     for name, arr in before.items():
         np.testing.assert_array_equal(after[name], arr[src], err_msg=name)
@@ -262,6 +261,8 @@ def test_generator_statistics(hip):
     cells = a["i1"] + 64 * a["i2"]
     assert (np.bincount(cells, minlength=64 * 64) == 16).all()
     assert ((a["dx1"] >= 0) & (a["dx1"] < 1)).all()
-    # drift: mean u_x ~ +-0.5 * gamma-ish; temperature ~ spread of u_y
-    assert abs(a["ux1"].mean() - 0.55) < 0.05 and abs(b["ux1"].mean() + 0.55) < 0.05
+    # drift: the lab-frame mean of u_x of a Juttner gas boosted by u_d is about
+    # u_d <gamma'> + beta Gamma <u'_x^2 / gamma'> ~ 0.575 + 0.045 = 0.62 for
+    # theta = 0.1, u_d = 0.5; the transverse spread stays sqrt(theta)
+    assert abs(a["ux1"].mean() - 0.62) < 0.03 and abs(b["ux1"].mean() + 0.62) < 0.03
     assert abs(a["ux2"].std() - np.sqrt(0.1)) < 0.02
