@@ -5,7 +5,8 @@ Workload (default): 3D periodic 512 x 256 x 256 cells, e-/e+ pair plasma at
 16 particles per cell per species (1,073,741,824 particles), theta = 0.3
 thermal spread, background B0 along z plus a 16-mode random-phase
 perturbation (|k| <= 4 * 2 pi / L, dB/B0 = 0.5), shape order 2, Vay pusher,
-2 filter passes, CFL 0.5.  Synthetic inputs are generated on the GPU.
+2 filter passes, CFL 0.5, tile re-sort every 4 steps.  Synthetic inputs are
+generated on the GPU.
 
 One step = ntt_sr_step (Faraday 1/2, fused push+deposit per species,
 current fold, filters, Faraday 1/2, Ampere + current source) plus the tile
@@ -43,7 +44,7 @@ def parse():
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--pusher", choices=["vay", "boris"], default="vay")
     ap.add_argument("--filters", type=int, default=2)
-    ap.add_argument("--sort-interval", type=int, default=10)
+    ap.add_argument("--sort-interval", type=int, default=4)
     ap.add_argument("--tile", type=str, default="4,4,4")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-cells", type=str, default="64,64,64")

@@ -8,19 +8,22 @@
 //   fixed chunks of 256 * ppt particles.  One workgroup = one chunk.  Chunks
 //   are mapped so that each XCD (workgroups are dealt round-robin over the 8
 //   XCDs) walks a contiguous range of chunks, whose windows overlap in its L2.
-// * The workgroup reduces the cell bounding box of its live particles.  If
-//   box + [-1, +2] cells (the reach of the order-0..2 gather and of the
-//   deposit of a particle that moves less than half a cell) fits the LDS
-//   window, the E/B values of that box are staged into LDS once, every
-//   particle gathers from LDS, and every deposit is accumulated into an LDS
-//   current window as 64-bit fixed point (value * 2^32, rounded) with integer
-//   LDS atomics: on gfx950 a 64-bit integer LDS atomic costs ~7 clocks per
-//   wavefront instruction against ~190 for the float one.  The window is then
-//   added to an HBM fixed-point accumulator with one 64-bit integer atomic per
-//   non-zero entry.  Integer sums do not depend on the order of the additions,
-//   so the deposited current is bit-reproducible.  Particles whose exact
-//   deposit stencil leaves the window (moves of half a cell or more) add their
-//   current straight to the HBM accumulator.
+// * The workgroup reduces the cell bounding box of its live particles, in
+//   the plain frame and in the frame rotated by half the box per dimension
+//   (a tile whose particles straddle the periodic boundary is compact in the
+//   rotated one), and keeps the smaller.  If box + [-1, +2] cells (the reach
+//   of the order-0..2 gather and of the deposit of a particle that moves at
+//   most one cell) fits the LDS window, the E/B values of that box (periodic
+//   images of the active cells) are staged into LDS once, every particle
+//   gathers from LDS, and every deposit is accumulated into an LDS current
+//   window as 64-bit fixed point (units of 2^-32) with integer LDS atomics:
+//   on gfx950 a 64-bit integer LDS atomic costs ~7 clocks per wavefront
+//   instruction against ~190 for the float one.  The window is then added to
+//   an HBM fixed-point accumulator with one 64-bit integer atomic per
+//   non-zero entry.  Integer sums do not depend on the order of the
+//   additions, so the deposited current is bit-reproducible.  Particles
+//   whose exact deposit stencil leaves the window add their current straight
+//   to the HBM accumulator.
 // * Chunks whose box does not fit (unsorted input) run the same code against
 //   HBM (gather from the ghost-extended em array, global atomics).
 // * Deposits use the pre-wrap cell indices and are folded onto the periodic
@@ -99,17 +102,21 @@ struct WrapJ {
   int32_t             n1, n2, n3, G, N1, N2;
   int64_t             cstride;
   __device__ __forceinline__ static int wrap(int x, int n) {
-    x += (x < 0) ? n : 0;
-    x -= (x >= n) ? n : 0;
+    while (x < 0) x += n;
+    while (x >= n) x -= n;
     return x;
+  }
+  // linear index of ghost-extended (a, b, g) in component c (no wrap)
+  __device__ __forceinline__ int64_t linear(int a, int b, int g, int c) const {
+    return static_cast<int64_t>(a) +
+           static_cast<int64_t>(N1) * (static_cast<int64_t>(b) + static_cast<int64_t>(N2) * g) +
+           cstride * c;
   }
   __device__ __forceinline__ int64_t index(int i, int j, int k, int c) const {
     const int a = wrap(i - G, n1) + G;
     const int b = wrap(j - G, n2) + G;
     const int g = (n3 > 1) ? wrap(k - G, n3) + G : 0;
-    return static_cast<int64_t>(a) +
-           static_cast<int64_t>(N1) * (static_cast<int64_t>(b) + static_cast<int64_t>(N2) * g) +
-           cstride * c;
+    return linear(a, b, g, c);
   }
   int i0 = 0, j0 = 0, k0 = 0;
   __device__ __forceinline__ void set_origin(int i, int j, int k) {
@@ -125,6 +132,9 @@ struct WrapJ {
   }
   __device__ __forceinline__ void add_fixed(int i, int j, int k, int c, unsigned long long v) {
     atomicAdd(d + index(i, j, k, c), v);
+  }
+  __device__ __forceinline__ void add_fixed_at(int64_t idx, unsigned long long v) {
+    atomicAdd(d + idx, v);
   }
 };
 
@@ -186,13 +196,34 @@ __device__ __forceinline__ void store_state(const ParticleView& P, int64_t p, co
   P.ux3[p] = s.u[2];
 }
 
+// Periodic frame of one dimension for a chunk.  A chunk whose particles sit
+// on both sides of the periodic boundary (e.g. cells n-1 and 0) is compact
+// in the frame rotated by h = n/2: cells c >= n - h are taken as c - n.  The
+// push and the deposit then run on these unwrapped cell indices, and the
+// periodic wrap that follows the push maps them back into [0, n) with the
+// same i / i_prev shifts as for a particle that crossed the boundary.
+__device__ __forceinline__ int rotated(int c, int n) {
+  const int h = n >> 1;
+  const int r = c + h;
+  return r >= n ? r - n : r;
+}
+__device__ __forceinline__ int wrap_any(int x, int n) {
+  while (x < 0) x += n;
+  while (x >= n) x -= n;
+  return x;
+}
+
 template <int D, int O, int ALG>
 __global__ __launch_bounds__(kThreads) void k_push_deposit(FusedArgs a) {
   constexpr int W1 = Win<D>::W1, W2 = Win<D>::W2, W3 = Win<D>::W3;
   constexpr int WC = W1 * W2 * W3;
+  constexpr int WM = (W1 > W2 ? W1 : W2) > W3 ? (W1 > W2 ? W1 : W2) : W3;
   __shared__ float              em_w[6 * WC];
   __shared__ unsigned long long j_w[3 * WC];
-  __shared__ int   red[2 * 3][kWaves];
+  __shared__ int                red[4 * 3 + 1][kWaves];
+  // ghost-extended index (em) and active index (current) of every window
+  // row/column, per dimension, after the periodic wrap
+  __shared__ int gtab[3][WM];
 
   // XCD-aware chunk mapping: blockIdx.x % 8 is the XCD; give each XCD a
   // contiguous run of chunks
@@ -219,58 +250,76 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(FusedArgs a) {
   }
   pend = min(pend, a.P.n);
 
-  // ---- pass 1: cell indices and tags -> bounding box of live particles -----
-  // (pass 2 reads i and tag again; the re-read hits in L2)
+  // ---- pass 1: bounding box of the live particles' cells, in the plain and
+  // in the rotated frame; cells outside [0, n) send the chunk to the HBM
+  // path, which reports them (pass 2 reads i and tag again; the re-read hits
+  // in L2)
   int lo[3] = { INT_MAX, INT_MAX, INT_MAX }, hi[3] = { INT_MIN, INT_MIN, INT_MIN };
+  int rlo[3] = { INT_MAX, INT_MAX, INT_MAX }, rhi[3] = { INT_MIN, INT_MIN, INT_MIN };
+  int bad = 0;
   for (int64_t p = pbeg + tid; p < pend; p += kThreads) {
-    {
-      const int16_t tag = a.P.tag[p];
-      if (tag == NTT_TAG_ALIVE) {
-        int c[3] = { a.P.i1[p], a.P.i2[p], 0 };
-        if constexpr (D == 3) c[2] = a.P.i3[p];
+    const int16_t tag = a.P.tag[p];
+    if (tag == NTT_TAG_ALIVE) {
+      int c[3] = { a.P.i1[p], a.P.i2[p], 0 };
+      if constexpr (D == 3) c[2] = a.P.i3[p];
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          lo[d] = min(lo[d], c[d]);
-          hi[d] = max(hi[d], c[d]);
-        }
-      } else if (tag != NTT_TAG_DEAD) {
-        raise_device_error(a.err, NTT_DEVERR_INVALID_TAG);
+      for (int d = 0; d < D; ++d) {
+        if (c[d] < 0 || c[d] >= a.n[d]) bad = 1;
+        const int r = rotated(c[d], a.n[d]);
+        lo[d]  = min(lo[d], c[d]);
+        hi[d]  = max(hi[d], c[d]);
+        rlo[d] = min(rlo[d], r);
+        rhi[d] = max(rhi[d], r);
       }
+    } else if (tag != NTT_TAG_DEAD) {
+      raise_device_error(a.err, NTT_DEVERR_INVALID_TAG);
     }
   }
   const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     const int l = wave_min(lo[d]), h = wave_max(hi[d]);
+    const int rl = wave_min(rlo[d]), rh = wave_max(rhi[d]);
     if (lane == 0) {
-      red[2 * d][wv]     = l;
-      red[2 * d + 1][wv] = h;
+      red[4 * d][wv]     = l;
+      red[4 * d + 1][wv] = h;
+      red[4 * d + 2][wv] = rl;
+      red[4 * d + 3][wv] = rh;
     }
   }
+  {
+    const int b = wave_max(bad);
+    if (lane == 0) red[12][wv] = b;
+  }
   __syncthreads();
-  int  blo[3] = { 0, 0, 0 }, bhi[3] = { 0, 0, 0 };
+  int  ulo[3] = { 0, 0, 0 }, uhi[3] = { 0, 0, 0 };
+  bool rot[3] = { false, false, false };
   bool any = true, fits = true;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    int l = red[2 * d][0], h = red[2 * d + 1][0];
+    int l = red[4 * d][0], h = red[4 * d + 1][0];
+    int rl = red[4 * d + 2][0], rh = red[4 * d + 3][0];
 #pragma unroll
     for (int w = 1; w < kWaves; ++w) {
-      l = min(l, red[2 * d][w]);
-      h = max(h, red[2 * d + 1][w]);
+      l  = min(l, red[4 * d][w]);
+      h  = max(h, red[4 * d + 1][w]);
+      rl = min(rl, red[4 * d + 2][w]);
+      rh = max(rh, red[4 * d + 3][w]);
     }
-    blo[d] = l;
-    bhi[d] = h;
     if (l > h) any = false;
+    rot[d] = (rh - rl) < (h - l);
+    const int hshift = a.n[d] >> 1;
+    ulo[d] = rot[d] ? rl - hshift : l;
+    uhi[d] = rot[d] ? rh - hshift : h;
   }
   if (!any) return;  // no live particle in this chunk (uniform across the block)
-  // every live particle must sit inside the box [0, n) for the window to
-  // stay inside the ghost-extended array; anything else takes the HBM path
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w)
+    if (red[12][w]) fits = false;
   const int Wd[3] = { W1, W2, W3 };
 #pragma unroll
-  for (int d = 0; d < D; ++d) {
-    if (bhi[d] - blo[d] + 4 > Wd[d]) fits = false;
-    if (blo[d] < 0 || bhi[d] >= a.n[d]) fits = false;
-  }
+  for (int d = 0; d < D; ++d)
+    if (uhi[d] - ulo[d] + 4 > Wd[d]) fits = false;
 
   if (!fits) {
     // ---- HBM path -------------------------------------------------------------
@@ -302,31 +351,35 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(FusedArgs a) {
   }
 
   // ---- window path ----------------------------------------------------------
-  // window origin (ghost-extended): box low corner - 1
-  const int o1 = blo[0] - 1 + G, o2 = blo[1] - 1 + G, o3 = (D == 3) ? blo[2] - 1 + G : 0;
-  const int e1 = bhi[0] - blo[0] + 4, e2 = bhi[1] - blo[1] + 4;
-  const int e3 = (D == 3) ? bhi[2] - blo[2] + 4 : 1;
-  const int ncell = e1 * e2 * e3;
-  // stage E, B (the box lies inside the ghost-extended array: o >= 1 and
-  // o + e - 1 <= n + 2 * G - 1 because 0 <= blo <= bhi <= n - 1 and G = 2)
-  for (int idx = tid; idx < 6 * ncell; idx += kThreads) {
-    const int c   = idx / ncell;
-    int       r   = idx - c * ncell;
-    const int l3  = r / (e1 * e2);
-    r -= l3 * e1 * e2;
-    const int l2  = r / e1;
-    const int l1  = r - l2 * e1;
-    em_w[c * WC + l1 + W1 * (l2 + W2 * l3)] = a.F(c, o1 + l1, o2 + l2, o3 + l3);
+  // window origin (ghost-extended, unwrapped frame): box low corner - 1
+  const int o1 = ulo[0] - 1 + G, o2 = ulo[1] - 1 + G, o3 = (D == 3) ? ulo[2] - 1 + G : 0;
+  const int e1 = uhi[0] - ulo[0] + 4, e2 = uhi[1] - ulo[1] + 4;
+  const int e3 = (D == 3) ? uhi[2] - ulo[2] + 4 : 1;
+  {
+    const int oo[3] = { o1, o2, o3 }, ee[3] = { e1, e2, e3 };
+    for (int idx = tid; idx < D * WM; idx += kThreads) {
+      const int d = idx / WM, l = idx - d * WM;
+      if (l < ee[d]) gtab[d][l] = wrap_any(oo[d] + l - G, a.n[d]);
+    }
   }
-  for (int idx = tid; idx < 3 * ncell; idx += kThreads) {
-    const int c  = idx / ncell;
-    int       r  = idx - c * ncell;
-    const int l3 = r / (e1 * e2);
-    r -= l3 * e1 * e2;
-    const int l2 = r / e1;
-    const int l1 = r - l2 * e1;
-    j_w[c * WC + l1 + W1 * (l2 + W2 * l3)] = 0ull;
+  __syncthreads();
+  // stage E, B of the box (periodic images of the active cells) and zero the
+  // current window; rows of constant width W1 keep the index math to
+  // constant divisions
+  const int rows = e3 * W2 * W1;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    for (int idx = tid; idx < rows; idx += kThreads) {
+      const int l3 = idx / (W1 * W2);
+      const int r  = idx - l3 * (W1 * W2);
+      const int l2 = r / W1, l1 = r - l2 * W1;
+      if (l1 < e1 && l2 < e2) {
+        const int g3 = (D == 3) ? gtab[2][l3] + G : 0;
+        em_w[c * WC + idx] = a.F(c, gtab[0][l1] + G, gtab[1][l2] + G, g3);
+      }
+    }
   }
+  for (int idx = tid; idx < 3 * WC; idx += kThreads) j_w[idx] = 0ull;
   __syncthreads();
 
   const LdsEM<D> F{ em_w, o1, o2, o3 };
@@ -339,8 +392,10 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(FusedArgs a) {
     load_state<D>(a.P, p, s);
     bool ok = true;
 #pragma unroll
-    for (int d = 0; d < D; ++d)
+    for (int d = 0; d < D; ++d) {
       if (!(s.dx[d] >= ZERO && s.dx[d] <= ONE)) ok = false;
+      if (rot[d] && s.i[d] >= a.n[d] - (a.n[d] >> 1)) s.i[d] -= a.n[d];
+    }
     if (!ok) {
       raise_device_error(a.err, NTT_DEVERR_OUT_OF_BOUNDS);
       continue;
@@ -369,17 +424,23 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(FusedArgs a) {
     sr_wrap_periodic<D>(a.n, s);
     store_state<D>(a.P, p, s);
   }
+  if (!do_dep) return;
   __syncthreads();
   // flush the current window onto the periodic image of the HBM accumulator
-  for (int idx = tid; idx < 3 * ncell; idx += kThreads) {
-    const int c  = idx / ncell;
-    int       r  = idx - c * ncell;
-    const int l3 = r / (e1 * e2);
-    r -= l3 * e1 * e2;
-    const int l2 = r / e1;
-    const int l1 = r - l2 * e1;
-    const unsigned long long v = j_w[c * WC + l1 + W1 * (l2 + W2 * l3)];
-    if (v != 0ull) a.J.add_fixed(o1 + l1, o2 + l2, o3 + l3, c, v);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    for (int idx = tid; idx < rows; idx += kThreads) {
+      const int l3 = idx / (W1 * W2);
+      const int r  = idx - l3 * (W1 * W2);
+      const int l2 = r / W1, l1 = r - l2 * W1;
+      if (l1 < e1 && l2 < e2) {
+        const unsigned long long v = j_w[c * WC + idx];
+        if (v != 0ull) {
+          const int g3 = (D == 3) ? gtab[2][l3] + G : 0;
+          a.J.add_fixed_at(a.J.linear(gtab[0][l1] + G, gtab[1][l2] + G, g3, c), v);
+        }
+      }
+    }
   }
 }
 

@@ -8,10 +8,11 @@
 // tiles of tile[0] x tile[1] (x tile[2]) cells, numbered x fastest; dead
 // particles get the key one past the last tile.
 //
-// Steps: key + histogram (atomics) -> exclusive scan of the histogram ->
-// scatter of particle indices (atomic cursor per key, so the order inside a
-// tile is the arrival order) -> gather of every SoA array through a scratch
-// buffer.  All scratch lives in the caller's workspace.
+// Steps: key + histogram (wavefront-aggregated atomics) -> exclusive scan of
+// the histogram -> scatter of particle indices (aggregated atomic cursor per
+// key, so the order inside a tile is the arrival order) -> gather of the SoA
+// arrays, four per pass, through a scratch buffer.  All scratch lives in the
+// caller's workspace.
 #include "internal.hpp"
 
 namespace ntt {
@@ -43,19 +44,46 @@ __global__ void k_zero_u32(uint32_t* d, int64_t n) {
     d[i] = 0;
 }
 
+// Wavefront-aggregated atomic increment: the lanes of a wavefront that
+// share a key are served by one atomic of their leader (a nearly sorted
+// species has a handful of distinct keys per wavefront).  Returns the old
+// value plus the lane's rank among the lanes with the same key.
+__device__ __forceinline__ uint32_t aggregated_increment(uint32_t* counters, uint32_t key,
+                                                         bool active) {
+  uint32_t result = 0;
+  uint64_t pending = __ballot(active);
+  while (pending != 0ull) {
+    const int      leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+    const uint32_t lkey   = __shfl(key, leader);
+    const uint64_t same   = __ballot(active && key == lkey) & pending;
+    const uint32_t rank =
+      __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(same >> 32),
+                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(same), 0u));
+    uint32_t base = 0;
+    if ((threadIdx.x & 63) == leader)
+      base = atomicAdd(counters + lkey, static_cast<uint32_t>(__popcll(same)));
+    base = __shfl(base, leader);
+    if ((same >> (threadIdx.x & 63)) & 1ull) result = base + rank;
+    pending &= ~same;
+  }
+  return result;
+}
+
 __global__ __launch_bounds__(kB) void k_keys(const int32_t* __restrict__ i1,
                                              const int32_t* __restrict__ i2,
                                              const int32_t* __restrict__ i3,
                                              const int16_t* __restrict__ tag, int64_t n,
                                              TileKey K, uint32_t* __restrict__ keys,
                                              uint32_t* __restrict__ counts) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
-  if (p >= n) return;
-  uint32_t k;
-  if (tag[p] == NTT_TAG_DEAD) k = K.dead_key;
-  else k = K(i1[p], i2[p], K.dim == 3 ? i3[p] : 0);
-  keys[p] = k;
-  atomicAdd(counts + k, 1u);
+  const int64_t p      = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  const bool    active = p < n;
+  uint32_t      k      = 0;
+  if (active) {
+    if (tag[p] == NTT_TAG_DEAD) k = K.dead_key;
+    else k = K(i1[p], i2[p], K.dim == 3 ? i3[p] : 0);
+    keys[p] = k;
+  }
+  aggregated_increment(counts, k, active);
 }
 
 // block-level exclusive scan of kScanTile elements; writes block totals
@@ -104,26 +132,55 @@ __global__ __launch_bounds__(kB) void k_scan_add(uint32_t* __restrict__ data, in
 __global__ __launch_bounds__(kB) void k_scatter(const uint32_t* __restrict__ keys, int64_t n,
                                                 uint32_t* __restrict__ cursor,
                                                 uint32_t* __restrict__ perm) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t pos = atomicAdd(cursor + keys[p], 1u);
-  perm[pos] = static_cast<uint32_t>(p);
+  const int64_t  p      = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  const bool     active = p < n;
+  const uint32_t key    = active ? keys[p] : 0u;
+  const uint32_t pos    = aggregated_increment(cursor, key, active);
+  if (active) perm[pos] = static_cast<uint32_t>(p);
 }
 
-template <typename T>
-__global__ __launch_bounds__(kB) void k_gather(T* __restrict__ dst, const T* __restrict__ src,
-                                               const uint32_t* __restrict__ perm, int64_t n) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
-  if (p >= n) return;
-  dst[p] = src[perm[p]];
-}
+// up to kBatch particle arrays permuted per pass, each staged in a slot of
+// 8 bytes per particle of the scratch buffer
+constexpr int kBatch = 4;
+struct ArrayBatch {
+  void*   arr[kBatch];
+  int32_t bytes[kBatch];  // element size: 2, 4 or 8
+  int32_t count;
+};
 
 template <typename T>
-__global__ __launch_bounds__(kB) void k_copy_t(T* __restrict__ dst, const T* __restrict__ src,
-                                               int64_t n) {
+__device__ __forceinline__ void gather_one(void* dst, const void* src, int64_t to, int64_t from) {
+  static_cast<T*>(dst)[to] = static_cast<const T*>(src)[from];
+}
+
+__global__ __launch_bounds__(kB) void k_gather_batch(ArrayBatch b, char* __restrict__ tmp,
+                                                     const uint32_t* __restrict__ perm,
+                                                     int64_t n) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
   if (p >= n) return;
-  dst[p] = src[p];
+  const int64_t q = perm[p];
+#pragma unroll
+  for (int k = 0; k < kBatch; ++k) {
+    if (k >= b.count) break;
+    void* slot = tmp + 8 * n * k;
+    if (b.bytes[k] == 2) gather_one<int16_t>(slot, b.arr[k], p, q);
+    else if (b.bytes[k] == 4) gather_one<int32_t>(slot, b.arr[k], p, q);
+    else gather_one<int64_t>(slot, b.arr[k], p, q);
+  }
+}
+
+__global__ __launch_bounds__(kB) void k_copy_batch(ArrayBatch b, const char* __restrict__ tmp,
+                                                   int64_t n) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (p >= n) return;
+#pragma unroll
+  for (int k = 0; k < kBatch; ++k) {
+    if (k >= b.count) break;
+    const void* slot = tmp + 8 * n * k;
+    if (b.bytes[k] == 2) gather_one<int16_t>(b.arr[k], slot, p, p);
+    else if (b.bytes[k] == 4) gather_one<int32_t>(b.arr[k], slot, p, p);
+    else gather_one<int64_t>(b.arr[k], slot, p, p);
+  }
 }
 
 // exclusive scan of n uint32 in place; tmp holds the block sums of every level
@@ -177,18 +234,29 @@ ntt_status make_layout(int64_t npart, int32_t dim, const int32_t* n, const int32
   L->counts = off; off = align256(off + 4 * L->nkeys);
   L->cursor = off; off = align256(off + 4 * L->nkeys);
   L->scan   = off; off = align256(off + 4 * scan_tmp_elems(L->nkeys));
-  L->tmp    = off; off = align256(off + 8 * npart);
+  L->tmp    = off; off = align256(off + 8 * kBatch * npart);
   L->total  = off;
   return NTT_OK;
 }
 
 unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kB - 1) / kB); }
 
-template <typename T>
-void permute(T* arr, T* tmp, const uint32_t* perm, int64_t n, hipStream_t st) {
-  if (arr == nullptr || n == 0) return;
-  k_gather<T><<<blocks_for(n), kB, 0, st>>>(tmp, arr, perm, n);
-  k_copy_t<T><<<blocks_for(n), kB, 0, st>>>(arr, tmp, n);
+// applies perm to every listed array (dst[p] = src[perm[p]]), kBatch at a time
+void permute_all(void* const* arrs, const int32_t* bytes, int narr, char* tmp,
+                 const uint32_t* perm, int64_t n, hipStream_t st) {
+  ArrayBatch b{};
+  for (int k = 0; k <= narr; ++k) {
+    if (k < narr && arrs[k] != nullptr) {
+      b.arr[b.count]   = arrs[k];
+      b.bytes[b.count] = bytes[k];
+      ++b.count;
+    }
+    if (b.count == kBatch || (k == narr && b.count > 0)) {
+      k_gather_batch<<<blocks_for(n), kB, 0, st>>>(b, tmp, perm, n);
+      k_copy_batch<<<blocks_for(n), kB, 0, st>>>(b, tmp, n);
+      b = ArrayBatch{};
+    }
+  }
 }
 
 }  // namespace
@@ -263,16 +331,16 @@ extern "C" ntt_status ntt_sort_particles(ntt_particles_t* prtls, int32_t dim, co
       hipMemcpyAsync(tile_offsets, cnt, 4 * L.nkeys, hipMemcpyDeviceToDevice, st) != hipSuccess)
     return NTT_ERR_HIP;
 
-  int32_t* i32[6] = { prtls->i1, prtls->i2, prtls->i3, prtls->i1_prev, prtls->i2_prev, prtls->i3_prev };
-  float*   f32[10] = { prtls->dx1, prtls->dx2, prtls->dx3, prtls->dx1_prev, prtls->dx2_prev,
-                       prtls->dx3_prev, prtls->ux1, prtls->ux2, prtls->ux3, prtls->weight };
-  for (int k = 0; k < 6; ++k)
-    if ((k % 3) < dim) permute<int32_t>(i32[k], static_cast<int32_t*>(tmp), perm, np, st);
-  for (int k = 0; k < 10; ++k)
-    if (k >= 6 || (k % 3) < dim) permute<float>(f32[k], static_cast<float*>(tmp), perm, np, st);
-  permute<float>(prtls->phi, static_cast<float*>(tmp), perm, np, st);
-  permute<int16_t>(prtls->tag, static_cast<int16_t*>(tmp), perm, np, st);
-  permute<int64_t>(prtls->pld_i, static_cast<int64_t*>(tmp), perm, np, st);
+  // every SoA array of the species (coordinates beyond dim are not used)
+  void* arrs[19] = {
+    prtls->i1, prtls->i2, dim == 3 ? prtls->i3 : nullptr,
+    prtls->i1_prev, prtls->i2_prev, dim == 3 ? prtls->i3_prev : nullptr,
+    prtls->dx1, prtls->dx2, dim == 3 ? prtls->dx3 : nullptr,
+    prtls->dx1_prev, prtls->dx2_prev, dim == 3 ? prtls->dx3_prev : nullptr,
+    prtls->ux1, prtls->ux2, prtls->ux3, prtls->weight, prtls->phi, prtls->tag, prtls->pld_i
+  };
+  const int32_t bytes[19] = { 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 2, 8 };
+  permute_all(arrs, bytes, 19, static_cast<char*>(tmp), perm, np, st);
   s = check_launch();
   if (s != NTT_OK) return s;
   if (npart_alive != nullptr) {
