@@ -18,6 +18,7 @@ NTT_OK, NTT_ERR_ARG, NTT_ERR_HIP, NTT_ERR_DEVICE, NTT_ERR_UNSUPPORTED, NTT_ERR_C
 DEVERR_INVALID_TAG = 0x1
 DEVERR_OUT_OF_BOUNDS = 0x2
 DEVERR_PUSHER_FLAGS = 0x4
+DEVERR_CURRENT_RANGE = 0x8
 
 TAG_DEAD, TAG_ALIVE = 0, 1
 
@@ -96,7 +97,7 @@ class NttSrStepParams(C.Structure):
                 ("shape_order", C.c_int32), ("n_filters", C.c_int32),
                 ("fieldsolver_enable", C.c_int32), ("deposit_enable", C.c_int32),
                 ("step", C.c_int64), ("fd", NttFdCoeffs), ("prtl_bc", NttPrtlBC),
-                ("particles_per_thread", C.c_int32)]
+                ("particles_per_thread", C.c_int32), ("weight_max", C.c_float)]
 
 
 class NttPlasmaInit(C.Structure):

@@ -150,6 +150,12 @@ extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t
   // currents.h:68: J = 0 -- here the fixed-point accumulator (a fill kernel
   // rather than a memset, so that the step can be captured in a graph)
   if (prm->deposit_enable) launch_zero_u64(acc, J.cstride * 3, st);
+  // fixed-point scale of the accumulator, common to every species
+  float qmax = 0.0f;
+  for (int sp = 0; sp < nspecies; ++sp)
+    if (prep[sp].dep) qmax = fmaxf(qmax, fabsf(species[sp].charge));
+  const float wmax   = prm->weight_max > 0.0f ? prm->weight_max : 1.0f;
+  const float fscale = fixed_point_scale(qmax * wmax, dt);
   for (int sp = 0; sp < nspecies; ++sp) {
     const Prep& p = prep[sp];
     if (!p.run) continue;
@@ -157,14 +163,14 @@ extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t
     std::pair<hipEvent_t, hipEvent_t> ev;
     const bool timed = prof_begin(st, &ev);
     s = launch_push_deposit_periodic(p.pv, E, J, acc, D, prm->shape_order, p.alg, p.ndth, dt,
-                                     metric->dx, species[sp].charge, p.dep,
+                                     metric->dx, species[sp].charge, p.dep, fscale,
                                      species[sp].tile_offsets, species[sp].ntiles, ppt, st);
     if (timed) prof_end(st, ev);
     if (s != NTT_OK) return s;
   }
   if (prm->deposit_enable) {
     // folded current (active cells + periodic ghosts) from the accumulator
-    launch_j_from_fixed(J, acc, st);
+    launch_j_from_fixed(J, acc, 1.0 / static_cast<double>(fscale), st);
     // currents.h:89-119: n filter passes (ping-pong between cur and buff)
     if (prm->n_filters > 0) {
       const FieldView* src = &J;

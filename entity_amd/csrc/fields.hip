@@ -389,15 +389,16 @@ __global__ void k_fill_zero_u64(unsigned long long* __restrict__ d, int64_t n) {
   for (int64_t k = i; k < n; k += stride) d[k] = 0ull;
 }
 
-// float current from the 64-bit fixed-point accumulator (value * 2^32):
-// exact in double below 2^53, then one rounding to float
+// float current from the 64-bit fixed-point accumulator (units of 1/scale,
+// a power of two): exact in double below 2^53, then one rounding to float
 __global__ __launch_bounds__(BX* BY) void k_j_from_fixed(FieldView J,
-                                                         const unsigned long long* __restrict__ acc) {
+                                                         const unsigned long long* __restrict__ acc,
+                                                         double inv_scale) {
   const CellIdx c = active_cell(J);
   if (!c.ok) return;
   for (int comp = 0; comp < 3; ++comp) {
     const long long q = static_cast<long long>(acc[J.idx(c.i, c.j, c.k, comp)]);
-    const float v = static_cast<float>(static_cast<double>(q) * 2.3283064365386963e-10);
+    const float v = static_cast<float>(static_cast<double>(q) * inv_scale);
     store_with_ghosts(J, c.i, c.j, c.k, comp, v);
   }
 }
@@ -554,8 +555,9 @@ void launch_zero_u64(unsigned long long* d, int64_t n, hipStream_t st) {
   k_fill_zero_u64<<<flat_grid(n), 256, 0, st>>>(d, n);
 }
 
-void launch_j_from_fixed(const FieldView& J, const unsigned long long* acc, hipStream_t st) {
-  k_j_from_fixed<<<active_grid(J), dim3(BX, BY), 0, st>>>(J, acc);
+void launch_j_from_fixed(const FieldView& J, const unsigned long long* acc, double inv_scale,
+                         hipStream_t st) {
+  k_j_from_fixed<<<active_grid(J), dim3(BX, BY), 0, st>>>(J, acc, inv_scale);
 }
 
 }  // namespace ntt

@@ -20,8 +20,14 @@ ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView&
                                         const FieldView& cur, unsigned long long* acc,
                                         int D, int order, int alg,
                                         float ndth, float dt, float dxm, float charge,
-                                        bool deposit, const uint32_t* tile_offsets,
+                                        bool deposit, float fscale,
+                                        const uint32_t* tile_offsets,
                                         int64_t ntiles, int ppt, hipStream_t st);
+// fixed-point scale 2^k of the current accumulator: the largest power of two
+// for which a particle with |charge * weight| <= qw_max keeps every deposit
+// contribution below 2^30 units
+float fixed_point_scale(float qw_max, float dt);
+float fixed_point_small_coeff(float dt);
 
 // fields.hip
 void launch_faraday_periodic(const FieldView& F, float c1, float c2, hipStream_t st);
@@ -33,7 +39,8 @@ void launch_fold(const FieldView& J, hipStream_t st);
 void launch_zero(float* d, int64_t n, hipStream_t st);
 void launch_copy(float* dst, const float* src, int64_t n, hipStream_t st);
 void launch_zero_u64(unsigned long long* d, int64_t n, hipStream_t st);
-// J (active cells and their periodic ghost copies) = acc * 2^-32
-void launch_j_from_fixed(const FieldView& J, const unsigned long long* acc, hipStream_t st);
+// J (active cells and their periodic ghost copies) = acc * inv_scale
+void launch_j_from_fixed(const FieldView& J, const unsigned long long* acc, double inv_scale,
+                         hipStream_t st);
 
 }  // namespace ntt

@@ -31,6 +31,9 @@
 //   Metadomain::SynchronizeFields directly (metadomain_comm.cpp:408-562).
 //
 // Reference: src/kernels/pushers/sr.hpp:117-814, src/kernels/currents_deposit.hpp:108-761.
+#include <algorithm>
+#include <cmath>
+
 #include "sr_particle.hpp"
 
 namespace ntt {
@@ -41,7 +44,16 @@ template <> struct Win<2> { static constexpr int W1 = 32, W2 = 16, W3 = 1; };
 
 namespace {
 
-constexpr int kThreads = 256;
+// Workgroup size and occupancy target of the fused kernel.  The LDS windows
+// (~48 KB in 3D) admit three workgroups per CU; with 512 threads and at most
+// 128 VGPRs two workgroups give 4 waves per SIMD.
+#ifndef NTT_PD_THREADS
+#define NTT_PD_THREADS 512
+#endif
+#ifndef NTT_PD_WAVES_PER_EU
+#define NTT_PD_WAVES_PER_EU 4
+#endif
+constexpr int kThreads = NTT_PD_THREADS;
 constexpr int kWaves   = kThreads / 64;
 
 // LDS window accessor for em (6 components)
@@ -59,11 +71,11 @@ struct LdsEM {
   }
 };
 
-// 64-bit fixed point (units of 2^-32) of a value x already multiplied by 2^32
-// (the deposit coefficient carries the factor, an exact power-of-two
-// scaling).  x = h * 2^32 + r with h = rint(x / 2^32) and |r| <= 2^31; r is
+// 64-bit fixed point of a value x already multiplied by the accumulator's
+// scale (the deposit coefficient carries the factor, an exact power of two).
+// General form: x = h * 2^32 + r with h = rint(x / 2^32) and |r| <= 2^31; r is
 // exact in float (a multiple of ulp(x) no larger than |x|), so the result is
-// x truncated toward zero to an integer: error below one unit (2^-32).
+// x truncated toward zero to an integer (error below one unit).
 __device__ __forceinline__ unsigned long long to_fixed(float x) {
   const float   h  = rintf(x * 2.3283064365386963e-10f);
   const float   r  = fminf(fmaf(h, -4294967296.0f, x), 2147483520.0f);  // < 2^31
@@ -72,7 +84,12 @@ __device__ __forceinline__ unsigned long long to_fixed(float x) {
   return (static_cast<unsigned long long>(static_cast<uint32_t>(hi)) << 32) |
          static_cast<uint32_t>(lo);
 }
-constexpr float kFixedScale = 4294967296.0f;  // 2^32
+// Short form for |x| < 2^31: the 32-bit truncation, sign-extended.
+__device__ __forceinline__ unsigned long long to_fixed_small(float x) {
+  const int32_t lo = static_cast<int32_t>(x);
+  return (static_cast<unsigned long long>(static_cast<uint32_t>(lo >> 31)) << 32) |
+         static_cast<uint32_t>(lo);
+}
 
 // LDS current window sink (3 components), 64-bit integer LDS atomics
 template <int D>
@@ -87,11 +104,12 @@ struct LdsJ {
     else return (i - o1) + W1 * ((j - o2) + W2 * (k - o3));
   }
   __device__ __forceinline__ void set_origin(int i, int j, int k) { base = index(i, j, k); }
+  // values are below 2^31 units (checked per particle before the deposit)
   __device__ __forceinline__ void add_rel(int a, int b, int g, int c, float v) {
-    atomicAdd(w + base + (c * WC + a + W1 * (b + W2 * g)), to_fixed(v));
+    atomicAdd(w + base + (c * WC + a + W1 * (b + W2 * g)), to_fixed_small(v));
   }
   __device__ __forceinline__ void add(int i, int j, int k, int c, float v) {
-    atomicAdd(w + c * WC + index(i, j, k), to_fixed(v));
+    atomicAdd(w + c * WC + index(i, j, k), to_fixed_small(v));
   }
 };
 
@@ -145,6 +163,9 @@ struct FusedArgs {
   int32_t         ngh;
   int32_t         n[3];
   float           ndth, dt, dxm, charge, inv_dt;
+  float           fscale;      // fixed-point units per unit of current (2^k)
+  float           small_coeff; // |coeff * fscale| up to this: LDS short path
+  float           big_coeff;   // |coeff * fscale| up to this: general conversion
   int64_t         nchunks;
   int32_t         ppt;   // particles per thread of a fixed chunk
   const uint32_t* toff;  // tile offsets (nchunks + 1 entries) or null
@@ -214,7 +235,8 @@ __device__ __forceinline__ int wrap_any(int x, int n) {
 }
 
 template <int D, int O, int ALG>
-__global__ __launch_bounds__(kThreads) void k_push_deposit(FusedArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NTT_PD_WAVES_PER_EU)))
+void k_push_deposit(FusedArgs a) {
   constexpr int W1 = Win<D>::W1, W2 = Win<D>::W2, W3 = Win<D>::W3;
   constexpr int WC = W1 * W2 * W3;
   constexpr int WM = (W1 > W2 ? W1 : W2) > W3 ? (W1 > W2 ? W1 : W2) : W3;
@@ -341,8 +363,13 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(FusedArgs a) {
       }
       sr_move<D, O, ALG>(a.F, G, a.ndth, a.dt, a.dxm, s);
       if (do_dep) {
-        WrapJ js = a.J;
-        deposit_particle<D, O>(js, G, s, a.P.w[p] * a.charge * kFixedScale, a.inv_dt);
+        const float coeff = a.P.w[p] * a.charge * a.fscale;
+        if (fabsf(coeff) <= a.big_coeff) {
+          WrapJ js = a.J;
+          deposit_particle<D, O>(js, G, s, coeff, a.inv_dt);
+        } else {
+          raise_device_error(a.err, NTT_DEVERR_CURRENT_RANGE);
+        }
       }
       sr_wrap_periodic<D>(a.n, s);
       store_state<D>(a.P, p, s);
@@ -412,13 +439,17 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(FusedArgs a) {
       if (l + G < oo[d] || h + G >= oo[d] + ee[d]) inwin = false;
     }
     if (do_dep) {
-      // coefficient times 2^32 (exact): the sinks take fixed-point-scaled values
-      const float coeff = w * a.charge * kFixedScale;
-      if (inwin) {
+      // coefficient times the fixed-point scale (exact): the sinks take
+      // values in fixed-point units; the LDS sink's short conversion needs
+      // every contribution below 2^31 units
+      const float coeff = w * a.charge * a.fscale;
+      if (inwin && fabsf(coeff) <= a.small_coeff) {
         deposit_particle<D, O>(jw, G, s, coeff, a.inv_dt);
-      } else {
+      } else if (fabsf(coeff) <= a.big_coeff) {
         WrapJ js = a.J;
         deposit_particle<D, O>(js, G, s, coeff, a.inv_dt);
+      } else {
+        raise_device_error(a.err, NTT_DEVERR_CURRENT_RANGE);
       }
     }
     sr_wrap_periodic<D>(a.n, s);
@@ -459,15 +490,34 @@ void launch_fused_o(int order, const FusedArgs& a, hipStream_t st) {
 
 }  // namespace
 
+// Every deposit contribution of a particle is bounded by |charge * weight| *
+// max(1/dt, 1) (Esirkepov: prefix sums of shape differences and the
+// transverse weights are at most 1; zig-zag: displacements below one cell;
+// out-of-plane: |v| < 1).  Contributions of particles with |coeff * scale|
+// up to small_coeff therefore stay below 2^30 units (a factor 2 of margin
+// below the 2^31 of the short conversion); the general conversion holds
+// contributions below 2^62 units, i.e. |coeff * scale| below 2^32 small_coeff.
+float fixed_point_small_coeff(float dt) { return 1073741824.0f / fmaxf(ONE / dt, ONE); }
+
+float fixed_point_scale(float qw_max, float dt) {
+  if (!(qw_max > 0.0f) || !std::isfinite(qw_max)) return 4294967296.0f;
+  int k = 0;
+  std::frexp(fixed_point_small_coeff(dt) / qw_max, &k);  // ratio in [2^(k-1), 2^k)
+  k -= 1;
+  k = std::max(-100, std::min(k, 100));
+  return std::ldexp(1.0f, k);
+}
+
 // Fused push + deposit of one species over a periodic box.  em must have its
 // periodic ghosts filled; the folded current is added, as 64-bit fixed point
-// (value * 2^32), to the active cells of `acc`, an integer array with the
-// layout of the current field `cur` (the caller zeroes it first).
+// in units of 1/fscale, to the active cells of `acc`, an integer array with
+// the layout of the current field `cur` (the caller zeroes it first).
 ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView& em,
                                         const FieldView& cur, unsigned long long* acc,
                                         int D, int order, int alg,
                                         float ndth, float dt, float dxm, float charge,
-                                        bool deposit, const uint32_t* tile_offsets,
+                                        bool deposit, float fscale,
+                                        const uint32_t* tile_offsets,
                                         int64_t ntiles, int ppt, hipStream_t st) {
   if (pv.n == 0) return NTT_OK;
   uint32_t* err = device_error_word();
@@ -485,6 +535,9 @@ ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView&
   a.dxm = dxm;
   a.charge = deposit ? charge : ZERO;
   a.inv_dt = ONE / dt;
+  a.fscale = fscale;
+  a.small_coeff = fixed_point_small_coeff(dt);
+  a.big_coeff = a.small_coeff * 4294967296.0f;
   a.err = err;
   a.toff = tile_offsets;
   a.ppt = ppt < 1 ? 1 : ppt;

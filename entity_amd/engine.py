@@ -101,6 +101,8 @@ class SRPICEngine:
             for s, m in zip(species, maxnpart)]
         self.step = 0
         self.time = 0.0
+        # bound on |weight| of every particle (sets the current's fixed-point scale)
+        self.weight_max = 1.0
         self._sort_ws = None
         self.lib = _abi.load(strict)
         # tile offsets of every species (device uint32, ntiles + 1), or None
@@ -136,6 +138,7 @@ class SRPICEngine:
                                                          C.byref(init), _stream()),
                        "ntt_init_uniform_plasma")
             sp.npart = abi.npart
+            self.weight_max = max(self.weight_max, abs(weight))
             # the generator lays particles out tile by tile: ppc * cells_per_tile each
             per_tile = ppc * math.prod(self.params.tile[: self.params.dim])
             self.tile_offsets[s] = (torch.arange(self.ntiles + 1, device=self.device,
@@ -151,6 +154,7 @@ class SRPICEngine:
         s.shape_order, s.n_filters = p.shape_order, p.n_filters
         s.fieldsolver_enable, s.deposit_enable = int(p.fieldsolver_enable), int(p.deposit_enable)
         s.step = self.step
+        s.weight_max = self.weight_max
         for k, v in p.fd.items():
             setattr(s.fd, k, v)
         for d in range(3):

@@ -63,6 +63,8 @@ typedef enum ntt_status {
 #define NTT_DEVERR_INVALID_TAG   0x1u /* tag neither alive nor dead (sr.hpp:118-122) */
 #define NTT_DEVERR_OUT_OF_BOUNDS 0x2u /* gather/deposit stencil outside the array     */
 #define NTT_DEVERR_PUSHER_FLAGS  0x4u /* invalid pusher algorithm flags               */
+#define NTT_DEVERR_CURRENT_RANGE 0x8u /* a particle's current exceeds the fixed-point
+                                         range (|weight| far above weight_max)     */
 
 /* ParticleTag (src/global/global.h:168-171) */
 #define NTT_TAG_DEAD  0
@@ -267,6 +269,9 @@ typedef struct ntt_sr_step_params {
   ntt_prtl_bc_t   prtl_bc;            /* must be periodic in every direction        */
   int32_t         particles_per_thread; /* chunk = 256 * this when a species has no
                                            tile offsets (0: 4)                        */
+  float           weight_max;         /* bound on |weight| of every particle (0: 1);
+                                           sets the fixed-point scale of the current;
+                                           heavier particles stay exact, only slower  */
 } ntt_sr_step_params_t;
 
 /* One full SRPIC step of a periodic box, in the reference's phase order:
@@ -278,8 +283,11 @@ typedef struct ntt_sr_step_params {
  * (the species arrays should be sorted by spatial tile, see ntt_sort_particles;
  * unsorted input is handled, only slower).  `buff` is scratch (its contents
  * afterwards are unspecified).  The deposit accumulates the current as 64-bit
- * fixed point (value * 2^32) in `workspace` (ntt_sr_step_workspace_size
- * bytes), so J does not depend on the order of the atomic additions. */
+ * fixed point in `workspace` (ntt_sr_step_workspace_size bytes), in units of
+ * 2^-k with 2^k the largest power of two for which every contribution of a
+ * particle with |charge * weight| <= max|charge| * weight_max stays below
+ * 2^30 units (each contribution is truncated to a unit), so J does not depend
+ * on the order of the atomic additions. */
 NTT_API ntt_status ntt_sr_step(ntt_field_t*                em,
                                ntt_field_t*                cur,
                                ntt_field_t*                buff,

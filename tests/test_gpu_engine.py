@@ -307,3 +307,31 @@ def test_chunking_invariance(hip, case):
             np.testing.assert_array_equal(flds[k], ref_flds[k], err_msg=k)
         for got, ref in zip(parts, ref_parts):
             np.testing.assert_array_equal(_rows(got, dim), _rows(ref, dim))
+
+
+def test_heavy_particles_take_exact_path(hip, oracle):
+    """Particles heavier than the declared weight bound leave the LDS short
+    conversion for the general 64-bit one: the step still matches the oracle;
+    a weight far beyond the fixed-point range is reported, not wrapped."""
+    from entity_amd import _abi
+    from entity_amd.kernels import device_errors
+    dim, n, order, alg, nf = 3, (16, 12, 8), 2, VAY, 1
+    eng = make_engine(dim, n, order, alg, nf)
+    rng = np.random.default_rng(11)
+    for sp in eng.species:
+        w = np.where(rng.random(sp.npart) < 0.2, 37.0, 1.0).astype(np.float32)
+        sp.weight[: sp.npart].copy_(torch.from_numpy(w))
+    eng.weight_max = 1.0
+    parts0, flds0 = snapshot(eng)
+    eng.step_forward()
+    assert device_errors() == 0
+    parts1, flds1 = snapshot(eng)
+    _, em_ref, cur_ref = oracle_step(oracle, eng, parts0, flds0)
+    jscale = np.abs(cur_ref).max()
+    np.testing.assert_allclose(flds1["cur"], cur_ref, rtol=0, atol=4e-6 * jscale)
+    escale = np.abs(em_ref).max()
+    np.testing.assert_allclose(flds1["em"][:3], em_ref[:3], rtol=0, atol=4e-6 * escale)
+    # out of range: flagged on the device
+    eng.species[0].weight[:1].fill_(1e30)
+    eng.step_forward()
+    assert device_errors() & _abi.DEVERR_CURRENT_RANGE
