@@ -111,7 +111,17 @@ typedef struct ort_gr_params {
   int ntt_oracle_sr_step_##SUF(ort_field_t* em, ort_field_t* cur,                \
                                ort_field_t* buff, ort_species_t* species,        \
                                int32_t nspecies, const ort_step_params_t* prm,   \
-                               const uint8_t prtl_bc[3][2]);
+                               const uint8_t prtl_bc[3][2]);                     \
+  int ntt_oracle_gr_push_##SUF(const ort_gr_params_t* prm, ort_particles_t* prtls, \
+                               const ort_field_t* em, const ort_field_t* em0,    \
+                               int32_t shape_order);                             \
+  int ntt_oracle_gr_metric_##SUF(const ort_gr_params_t* prm, int64_t n,          \
+                                 const double* x1, const double* x2,             \
+                                 double* out);                                   \
+  int ntt_oracle_gr_transform_##SUF(const ort_gr_params_t* prm, int32_t which,   \
+                                    int64_t n, const double* x1,                 \
+                                    const double* x2, const double* vin,         \
+                                    double* vout);
 
 ORT_DECL(f32, float)
 ORT_DECL(f64, double)

@@ -217,3 +217,80 @@ def sr_step(em, cur, buff, species: list[dict], *, dim, prm: dict, bc, prec="f32
     return _call("sr_step", prec, C.byref(field_struct(em, dim)), C.byref(field_struct(cur, dim)),
                  C.byref(field_struct(buff, dim)), arr, C.c_int32(len(species)), C.byref(sp_),
                  bcs)
+
+
+# ---- GR pusher over the Kerr-Schild metrics ---------------------------------------
+METRIC_KIND = {"kerr_schild": 0, "qkerr_schild": 1, "kerr_schild_0": 2}
+
+# order of the quantities returned by gr_metric, per point
+GR_METRIC_QUANTITIES = (
+    "h_11", "h_22", "h_33", "h_13", "h11", "h22", "h33", "h13", "alpha", "beta1",
+    "dr_alpha", "dt_alpha", "dr_beta1", "dt_beta1", "dr_h11", "dr_h22", "dr_h33", "dr_h13",
+    "dt_h11", "dt_h22", "dt_h33", "dt_h13", "r", "theta")
+
+GR_TRANSFORMS = {"D->U": 0, "U->D": 1, "D->T": 2, "T->D": 3, "U->T": 4, "T->U": 5}
+
+
+class OrtGrParams(C.Structure):
+    _fields_ = [("mass", C.c_float), ("charge", C.c_float), ("dt", C.c_double),
+                ("omegaB0", C.c_double), ("niter", C.c_int32), ("ni1", C.c_int32),
+                ("ni2", C.c_int32), ("massless", C.c_int32), ("absorb_i1min", C.c_int32),
+                ("absorb_i1max", C.c_int32), ("axis_i2min", C.c_int32),
+                ("axis_i2max", C.c_int32), ("metric_kind", C.c_int32), ("a", C.c_double),
+                ("r0", C.c_double), ("h", C.c_double), ("x1_min", C.c_double),
+                ("x1_max", C.c_double), ("x2_min", C.c_double), ("x2_max", C.c_double),
+                ("nx1", C.c_int32), ("nx2", C.c_int32)]
+
+
+def gr_params(metric: str, n, extent, *, a=0.0, r0=0.0, h=0.0, dt=0.0, mass=1.0, charge=1.0,
+              omegaB0=1.0, niter=10, massless=False, absorb=(True, True), axis=(True, True)):
+    p = OrtGrParams()
+    p.mass, p.charge, p.dt, p.omegaB0, p.niter = mass, charge, dt, omegaB0, niter
+    p.ni1, p.ni2 = n
+    p.nx1, p.nx2 = n
+    p.massless = int(massless)
+    p.absorb_i1min, p.absorb_i1max = (int(x) for x in absorb)
+    p.axis_i2min, p.axis_i2max = (int(x) for x in axis)
+    p.metric_kind = METRIC_KIND[metric]
+    p.a, p.r0, p.h = a, r0, h
+    (p.x1_min, p.x1_max), (p.x2_min, p.x2_max) = extent
+    return p
+
+
+def gr_push(prm: OrtGrParams, p: dict, em=None, em0=None, *, order=1, prec="f64") -> int:
+    """kernel::gr::Pusher_kernel over every particle (D from em, B from em0)."""
+    s = particles_struct(p)
+    fe = field_struct(em, 2) if em is not None else None
+    f0 = field_struct(em0, 2) if em0 is not None else None
+    fn = getattr(lib(), f"ntt_oracle_gr_push_{prec}")
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
+    return fn(C.addressof(prm), C.addressof(s), C.addressof(fe) if fe is not None else None,
+              C.addressof(f0) if f0 is not None else None, int(order))
+
+
+def gr_metric(prm: OrtGrParams, x1, x2, *, prec="f64") -> dict:
+    """Metric quantities at code coordinates (arrays); keys GR_METRIC_QUANTITIES."""
+    x1 = np.ascontiguousarray(x1, dtype=np.float64).ravel()
+    x2 = np.ascontiguousarray(x2, dtype=np.float64).ravel()
+    out = np.zeros((x1.size, len(GR_METRIC_QUANTITIES)))
+    fn = getattr(lib(), f"ntt_oracle_gr_metric_{prec}")
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+    assert fn(C.addressof(prm), x1.size, _ptr(x1), _ptr(x2), _ptr(out)) == 0
+    return {k: out[:, q] for q, k in enumerate(GR_METRIC_QUANTITIES)}
+
+
+def gr_transform(prm: OrtGrParams, which: str, x1, x2, v, *, prec="f64") -> np.ndarray:
+    """Vector transformation ("D->U", "U->D", "D->T", "T->D", "U->T", "T->U")."""
+    x1 = np.ascontiguousarray(x1, dtype=np.float64).ravel()
+    x2 = np.ascontiguousarray(x2, dtype=np.float64).ravel()
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros_like(v)
+    fn = getattr(lib(), f"ntt_oracle_gr_transform_{prec}")
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                   C.c_void_p]
+    assert fn(C.addressof(prm), GR_TRANSFORMS[which], x1.size, _ptr(x1), _ptr(x2),
+              _ptr(v), _ptr(out)) == 0
+    return out

@@ -1,0 +1,793 @@
+/*
+ * ntt_oracle_gr.cpp -- CPU restatement of Entity's GR particle pusher over
+ * the Kerr-Schild family of metrics (2D axisymmetric, r-theta).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP GR pusher.  Product code
+ * never links or calls this file.
+ *
+ * Metric functions follow src/metrics/kerr_schild.h, qkerr_schild.h and
+ * kerr_schild_0.h, function by function, in their operation order; the
+ * pusher follows src/kernels/pushers/gr.hpp:177-865.  Every function is a
+ * template on the real type (float: the reference's SINGLE_PRECISION build;
+ * double: its double build).
+ */
+#include <cmath>
+#include <cstdint>
+#include <limits>
+
+#include "ntt_oracle_common.hpp"
+
+namespace {
+
+using namespace ort;
+
+constexpr double PI_D         = 3.14159265358979323846;
+constexpr double INV_PI_D     = 0.31830988618379067154;
+constexpr double PI_SQR_D     = 9.86960440108935861882;
+constexpr double INV_PI_SQR_D = 0.10132118364233777144;
+constexpr double SQRT3_D      = 1.73205080756887729352;
+constexpr double SMALL_ANGLE_GR_D = 1e-5;
+
+template <typename R> inline R sqr(R x) { return x * x; }
+template <typename R> inline R cube(R x) { return x * x * x; }
+
+// cmp::AlmostZero with the machine epsilon of R (utils/comparators.h:38-42)
+template <typename R> inline bool almost_zero(R a) {
+  return std::fabs(a) <= std::numeric_limits<R>::epsilon();
+}
+
+/* ---- metrics --------------------------------------------------------------
+ * kind 0: KerrSchild (kerr_schild.h), 1: QKerrSchild (qkerr_schild.h),
+ * 2: KerrSchild0 (kerr_schild_0.h).  Coordinates x are code units. */
+template <typename R>
+struct GRMetric {
+  int kind;
+  R   a, r0, h0;
+  R   x1_min, x2_min;
+  // KS / KS0
+  R dr, dtheta, dr_inv, dtheta_inv;
+  // QKS
+  R chi_min, eta_min, dchi, deta, dchi_inv, deta_inv;
+
+  explicit GRMetric(const ort_gr_params_t* p)
+    : kind(p->metric_kind), a(static_cast<R>(p->a)), r0(static_cast<R>(p->r0)),
+      h0(static_cast<R>(p->h)), x1_min(static_cast<R>(p->x1_min)),
+      x2_min(static_cast<R>(p->x2_min)) {
+    const R x1_max = static_cast<R>(p->x1_max), x2_max = static_cast<R>(p->x2_max);
+    if (kind == 2) a = K<R>::ZERO;
+    // kerr_schild.h:79-90, kerr_schild_0.h:47-56
+    dr         = (x1_max - x1_min) / static_cast<R>(p->nx1);
+    dtheta     = (x2_max - x2_min) / static_cast<R>(p->nx2);
+    dr_inv     = K<R>::ONE / dr;
+    dtheta_inv = K<R>::ONE / dtheta;
+    // qkerr_schild.h:79-94
+    chi_min = eta_min = dchi = deta = dchi_inv = deta_inv = K<R>::ZERO;
+    if (kind == 1) {
+      chi_min  = std::log(x1_min - r0);
+      eta_min  = theta2eta(x2_min);
+      dchi     = (std::log(x1_max - r0) - chi_min) / static_cast<R>(p->nx1);
+      deta     = (theta2eta(x2_max) - eta_min) / static_cast<R>(p->nx2);
+      dchi_inv = K<R>::ONE / dchi;
+      deta_inv = K<R>::ONE / deta;
+    }
+  }
+
+  // Kerr functions (kerr_schild.h:44-58)
+  R Delta(R r) const { return sqr(r) - K<R>::TWO * r + sqr(a); }
+  R Sigma(R r, R th) const { return sqr(r) + sqr(a) * sqr(std::cos(th)); }
+  R A(R r, R th) const {
+    return sqr(sqr(r) + sqr(a)) - sqr(a) * Delta(r) * sqr(std::sin(th));
+  }
+  R z(R r, R th) const { return K<R>::TWO * r / Sigma(r, th); }
+
+  // quasi-spherical angle stretching (qkerr_schild.h:857-914)
+  R dtheta_deta(R eta) const {
+    if (almost_zero(h0)) return K<R>::ONE;
+    return (K<R>::ONE + K<R>::TWO * h0 +
+            static_cast<R>(12.0) * h0 * (eta * static_cast<R>(INV_PI_D)) *
+              ((eta * static_cast<R>(INV_PI_D)) - K<R>::ONE));
+  }
+  R eta2theta(R eta) const {
+    if (almost_zero(h0)) return eta;
+    return eta + K<R>::TWO * h0 * eta * (static_cast<R>(PI_D) - K<R>::TWO * eta) *
+                   (static_cast<R>(PI_D) - eta) * static_cast<R>(INV_PI_SQR_D);
+  }
+  R dx_dt(R eta) const {
+    if (almost_zero(h0)) return deta;
+    return deta * (K<R>::ONE + K<R>::TWO * h0 * static_cast<R>(INV_PI_SQR_D) *
+                                 (K<R>::TWO * K<R>::THREE * sqr(eta) -
+                                  K<R>::TWO * K<R>::THREE * static_cast<R>(PI_D) * eta +
+                                  static_cast<R>(PI_SQR_D)));
+  }
+  // closed-form inverse of eta2theta, evaluated in double (qkerr_schild.h:889-913)
+  R theta2eta(R theta) const {
+    if (almost_zero(h0)) return theta;
+    const double hd = static_cast<double>(h0), th = static_cast<double>(theta);
+    const double TWO_PI_D = 6.28318530717958647692;
+    const double Rv = std::pow(
+      -9.0 * hd * hd * (PI_D - 2.0 * th) +
+        SQRT3_D * std::sqrt((hd * hd * hd *
+                             ((4.0 - hd) * (PI_D + hd * TWO_PI_D) * (PI_D + hd * TWO_PI_D) -
+                              108.0 * hd * PI_D * th + 108.0 * hd * th * th))),
+      1.0 / 3.0);
+    const double PI_23 = 2.14502939711102560008, PI_13 = 1.46459188756152326302;
+    const double TWO_23 = 1.58740105196819947475, THREE_13 = 1.442249570307408382321;
+    const double TWO_13 = 1.2599210498948731647672, THREE_PI_23 = 4.46184094890142313715794;
+    return static_cast<R>(PI_23 *
+                          (6.0 * PI_13 + 2.0 * TWO_13 * (hd - 1.0) * THREE_PI_23 / Rv +
+                           TWO_23 * THREE_13 * Rv / hd) /
+                          12.0);
+  }
+
+  // physical r, theta of code coordinates
+  R r_of(const R* x) const {
+    if (kind == 1) return r0 + std::exp(x[0] * dchi + chi_min);
+    return x[0] * dr + x1_min;
+  }
+  R th_of(const R* x) const {
+    if (kind == 1) return eta2theta(x[1] * deta + eta_min);
+    return x[1] * dtheta + x2_min;
+  }
+  // convert<2, Cd, Ph> and convert<2, Ph, Cd>
+  R theta_cd2ph(R x) const {
+    if (kind == 1) return eta2theta(x * deta + eta_min);
+    return x * dtheta + x2_min;
+  }
+  R theta_ph2cd(R th) const {
+    if (kind == 1) return (theta2eta(th) - eta_min) * deta_inv;
+    return (th - x2_min) * dtheta_inv;
+  }
+
+  // ---- h_ij (lower) ----
+  R h_11(const R* x) const {
+    if (kind == 0) return sqr(dr) * (K<R>::ONE + z(x[0] * dr + x1_min, x[1] * dtheta + x2_min));
+    if (kind == 2) return sqr(dr);
+    return sqr(dchi) * std::exp(K<R>::TWO * (x[0] * dchi + chi_min)) *
+           (K<R>::ONE + z(r0 + std::exp(x[0] * dchi + chi_min), eta2theta(x[1] * deta + eta_min)));
+  }
+  R h_22(const R* x) const {
+    if (kind == 0) return sqr(dtheta) * Sigma(x[0] * dr + x1_min, x[1] * dtheta + x2_min);
+    if (kind == 2) return sqr(dtheta) * sqr(x[0] * dr + x1_min);
+    return sqr(deta) * sqr(dtheta_deta(x[1] * deta + eta_min)) *
+           Sigma(r0 + std::exp(x[0] * dchi + chi_min), eta2theta(x[1] * deta + eta_min));
+  }
+  R h_33(const R* x) const {
+    if (kind == 0) {
+      const R r = x[0] * dr + x1_min, th = x[1] * dtheta + x2_min;
+      return A(r, th) * sqr(std::sin(th)) / Sigma(r, th);
+    }
+    if (kind == 2) return sqr((x[0] * dr + x1_min) * std::sin(x[1] * dtheta + x2_min));
+    const R th = eta2theta(x[1] * deta + eta_min);
+    return A(r0 + std::exp(x[0] * dchi + chi_min), th) * sqr(std::sin(th)) /
+           Sigma(r0 + std::exp(x[0] * dchi + chi_min), th);
+  }
+  R h_13(const R* x) const {
+    if (kind == 0)
+      return -dr * a * (K<R>::ONE + z(x[0] * dr + x1_min, x[1] * dtheta + x2_min)) *
+             sqr(std::sin(x[1] * dtheta + x2_min));
+    if (kind == 2) return K<R>::ZERO;
+    const R th = eta2theta(x[1] * deta + eta_min);
+    return -dchi * std::exp(x[0] * dchi + chi_min) * a *
+           (K<R>::ONE + z(r0 + std::exp(x[0] * dchi + chi_min), th)) * sqr(std::sin(th));
+  }
+  // ---- h^ij (upper) ----
+  R h11(const R* x) const {
+    if (kind == 0) {
+      const R r = x[0] * dr + x1_min, th = x[1] * dtheta + x2_min;
+      const R S = Sigma(r, th);
+      return sqr(dr_inv) * A(r, th) / (S * (S + K<R>::TWO * r));
+    }
+    if (kind == 2) return sqr(dr_inv);
+    const R th = eta2theta(x[1] * deta + eta_min);
+    const R r  = r0 + std::exp(x[0] * dchi + chi_min);
+    const R S  = Sigma(r, th);
+    return (std::exp(-K<R>::TWO * (x[0] * dchi + chi_min)) / sqr(dchi)) * A(r, th) /
+           (S * (S + K<R>::TWO * r));
+  }
+  R h22(const R* x) const {
+    if (kind == 0) return sqr(dtheta_inv) / Sigma(x[0] * dr + x1_min, x[1] * dtheta + x2_min);
+    if (kind == 2) return sqr(dtheta_inv / (x[0] * dr + x1_min));
+    const R th = eta2theta(x[1] * deta + eta_min);
+    return K<R>::ONE / (Sigma(r0 + std::exp(x[0] * dchi + chi_min), th) *
+                        sqr(dtheta_deta(x[1] * deta + eta_min)) * sqr(deta));
+  }
+  R h33(const R* x) const {
+    if (kind == 0) {
+      const R th = x[1] * dtheta + x2_min;
+      return K<R>::ONE / (Sigma(x[0] * dr + x1_min, th) * sqr(std::sin(th)));
+    }
+    if (kind == 2) return K<R>::ONE / sqr((x[0] * dr + x1_min) * std::sin(x[1] * dtheta + x2_min));
+    const R th = eta2theta(x[1] * deta + eta_min);
+    return K<R>::ONE / (Sigma(r0 + std::exp(x[0] * dchi + chi_min), th) * sqr(std::sin(th)));
+  }
+  R h13(const R* x) const {
+    if (kind == 0) return dr_inv * a / Sigma(x[0] * dr + x1_min, x[1] * dtheta + x2_min);
+    if (kind == 2) return K<R>::ZERO;
+    const R th = eta2theta(x[1] * deta + eta_min);
+    return (std::exp(-(x[0] * dchi + chi_min)) * dchi_inv) * a /
+           Sigma(r0 + std::exp(x[0] * dchi + chi_min), th);
+  }
+  // ---- lapse, shift ----
+  R alpha(const R* x) const {
+    if (kind == 2) return K<R>::ONE;
+    return K<R>::ONE / std::sqrt(K<R>::ONE + z(r_of(x), th_of(x)));
+  }
+  R beta1(const R* x) const {
+    if (kind == 2) return K<R>::ZERO;
+    if (kind == 0) {
+      const R z_ = z(x[0] * dr + x1_min, x[1] * dtheta + x2_min);
+      return dr_inv * z_ / (K<R>::ONE + z_);
+    }
+    const R chi = x[0] * dchi + chi_min;
+    const R z_  = z(r0 + std::exp(chi), eta2theta(x[1] * deta + eta_min));
+    return std::exp(-chi) * dchi_inv * z_ / (K<R>::ONE + z_);
+  }
+  // d/d(theta) of Sigma and A (with the AlmostZero cut of the reference)
+  R dt_Sigma_ks(R th) const {
+    const R v = -K<R>::TWO * sqr(a) * std::sin(th) * std::cos(th) * dtheta;
+    return almost_zero(v) ? K<R>::ZERO : v;
+  }
+  R dt_A_ks(R r, R th) const {
+    const R v = -K<R>::TWO * sqr(a) * std::sin(th) * std::cos(th) * Delta(r) * dtheta;
+    return almost_zero(v) ? K<R>::ZERO : v;
+  }
+  R dt_Sigma_q(R eta) const {
+    const R th = eta2theta(eta);
+    const R v  = -K<R>::TWO * sqr(a) * std::sin(th) * std::cos(th) * dx_dt(eta);
+    return almost_zero(v) ? K<R>::ZERO : v;
+  }
+  R dt_A_q(R r, R eta) const {
+    const R th = eta2theta(eta);
+    const R v  = -K<R>::TWO * sqr(a) * std::sin(th) * std::cos(th) * Delta(r) * dx_dt(eta);
+    return almost_zero(v) ? K<R>::ZERO : v;
+  }
+
+  R dr_alpha(const R* x) const {
+    if (kind == 2) return K<R>::ZERO;
+    const R r = r_of(x), th = th_of(x);
+    const R dx_r = (kind == 0) ? dr : dchi * std::exp(x[0] * dchi + chi_min);
+    const R dr_S = K<R>::TWO * r * dx_r;
+    return -(dx_r * Sigma(r, th) - r * dr_S) * cube(alpha(x)) / sqr(Sigma(r, th));
+  }
+  R dt_alpha(const R* x) const {
+    if (kind == 2) return K<R>::ZERO;
+    if (kind == 0) {
+      const R r = x[0] * dr + x1_min, th = x[1] * dtheta + x2_min;
+      return cube(alpha(x)) * r * dt_Sigma_ks(th) / sqr(Sigma(r, th));
+    }
+    const R r   = r0 + std::exp(x[0] * dchi + chi_min);
+    const R eta = x[1] * deta + eta_min;
+    const R th  = eta2theta(eta);
+    const R dxdt = deta * (K<R>::ONE + K<R>::TWO * h0 * static_cast<R>(INV_PI_SQR_D) *
+                                         (K<R>::TWO * K<R>::THREE * sqr(eta) -
+                                          K<R>::TWO * K<R>::THREE * static_cast<R>(PI_D) * eta +
+                                          static_cast<R>(PI_SQR_D)));
+    const R dt_S = -K<R>::TWO * sqr(a) * std::sin(th) * std::cos(th) * dxdt;
+    return r * dt_S * cube(alpha(x)) / sqr(Sigma(r, th));
+  }
+  R dr_beta1(const R* x) const {
+    if (kind == 2) return K<R>::ZERO;
+    if (kind == 0) {
+      const R r = x[0] * dr + x1_min, th = x[1] * dtheta + x2_min;
+      const R dr_S = K<R>::TWO * r * dr;
+      return dr_inv * K<R>::TWO * (dr * Sigma(r, th) - r * dr_S) /
+             sqr(Sigma(r, th) + K<R>::TWO * r);
+    }
+    const R chi  = x[0] * dchi + chi_min;
+    const R r    = r0 + std::exp(chi);
+    const R th   = eta2theta(x[1] * deta + eta_min);
+    const R z_   = z(r, th);
+    const R dx_r = dchi * std::exp(x[0] * dchi + chi_min);
+    const R dr_S = K<R>::TWO * r * dx_r;
+    return std::exp(-chi) * dchi_inv * K<R>::TWO * (dx_r * Sigma(r, th) - r * dr_S) /
+             sqr(Sigma(r, th) + K<R>::TWO * r) -
+           dchi * std::exp(-chi) * dchi_inv * z_ / (K<R>::ONE + z_);
+  }
+  R dt_beta1(const R* x) const {
+    if (kind == 2) return K<R>::ZERO;
+    if (kind == 0) {
+      const R r = x[0] * dr + x1_min, th = x[1] * dtheta + x2_min;
+      return -dr_inv * K<R>::TWO * r * dt_Sigma_ks(th) /
+             sqr(Sigma(r, th) * (K<R>::ONE + z(r, th)));
+    }
+    const R chi = x[0] * dchi + chi_min;
+    const R r   = r0 + std::exp(chi);
+    const R eta = x[1] * deta + eta_min;
+    const R th  = eta2theta(eta);
+    return -std::exp(-chi) * dchi_inv * K<R>::TWO * r * dt_Sigma_q(eta) /
+           sqr(Sigma(r, th) * (K<R>::ONE + z(r, th)));
+  }
+  R dr_h11(const R* x) const {
+    if (kind == 2) return K<R>::ZERO;
+    const R r = r_of(x), th = th_of(x);
+    const R dx_r  = (kind == 0) ? dr : dchi * std::exp(x[0] * dchi + chi_min);
+    const R dr_S  = K<R>::TWO * r * dx_r;
+    const R dr_D  = K<R>::TWO * dx_r * (r - K<R>::ONE);
+    const R dr_A  = K<R>::FOUR * r * dx_r * (sqr(r) + sqr(a)) - sqr(a) * sqr(std::sin(th)) * dr_D;
+    const R S = Sigma(r, th), Av = A(r, th);
+    if (kind == 0)
+      return (S * (S + K<R>::TWO * r) * dr_A -
+              K<R>::TWO * Av * (r * dr_S + S * (dr_S + dr))) /
+             (sqr(S * (S + K<R>::TWO * r))) * sqr(dr_inv);
+    return (std::exp(-K<R>::TWO * (x[0] * dchi + chi_min)) / sqr(dchi) *
+            (S * (S + K<R>::TWO * r) * dr_A -
+             K<R>::TWO * Av * (r * dr_S + S * (dr_S + dx_r))) /
+            (sqr(S * (S + K<R>::TWO * r)))) -
+           K<R>::TWO * dchi * std::exp(-K<R>::TWO * (x[0] * dchi + chi_min)) / sqr(dchi) * Av /
+             (S * (S + K<R>::TWO * r));
+  }
+  R dr_h22(const R* x) const {
+    if (kind == 2) {
+      const R r = x[0] * dr + x1_min;
+      return -K<R>::TWO / cube(r) * sqr(dtheta_inv) * dr;
+    }
+    const R r = r_of(x), th = th_of(x);
+    const R dx_r = (kind == 0) ? dr : dchi * std::exp(x[0] * dchi + chi_min);
+    const R dr_S = K<R>::TWO * r * dx_r;
+    if (kind == 0) return -dr_S / sqr(Sigma(r, th)) * sqr(dtheta_inv);
+    return -dr_S / sqr(Sigma(r, th)) / sqr(deta);
+  }
+  R dr_h33(const R* x) const {
+    if (kind == 2) {
+      const R r = x[0] * dr + x1_min, th = x[1] * dtheta + x2_min;
+      return -K<R>::TWO / cube(r) / sqr(std::sin(th)) * dr;
+    }
+    const R r = r_of(x), th = th_of(x);
+    const R dx_r = (kind == 0) ? dr : dchi * std::exp(x[0] * dchi + chi_min);
+    const R dr_S = K<R>::TWO * r * dx_r;
+    return -dr_S / sqr(Sigma(r, th)) / sqr(std::sin(th));
+  }
+  R dr_h13(const R* x) const {
+    if (kind == 2) return K<R>::ZERO;
+    const R r = r_of(x), th = th_of(x);
+    if (kind == 0) {
+      const R dr_S = K<R>::TWO * r * dr;
+      return -a * dr_S / sqr(Sigma(r, th)) * dr_inv;
+    }
+    const R dx_r = dchi * std::exp(x[0] * dchi + chi_min);
+    const R dr_S = K<R>::TWO * r * dx_r;
+    return -a * dr_S / sqr(Sigma(r, th)) * (std::exp(-(x[0] * dchi + chi_min)) * dchi_inv) -
+           dchi * (std::exp(-(x[0] * dchi + chi_min)) * dchi_inv) * a / Sigma(r, th);
+  }
+  R dt_h11(const R* x) const {
+    if (kind == 2) return K<R>::ZERO;
+    if (kind == 0) {
+      const R r = x[0] * dr + x1_min, th = x[1] * dtheta + x2_min;
+      const R S = Sigma(r, th);
+      return (S * (S + K<R>::TWO * r) * dt_A_ks(r, th) -
+              K<R>::TWO * A(r, th) * dt_Sigma_ks(th) * (r + S)) /
+             (sqr(S * (S + K<R>::TWO * r))) * sqr(dr_inv);
+    }
+    const R r   = r0 + std::exp(x[0] * dchi + chi_min);
+    const R eta = x[1] * deta + eta_min;
+    const R th  = eta2theta(eta);
+    const R S   = Sigma(r, th);
+    return std::exp(-K<R>::TWO * (x[0] * dchi + chi_min)) / sqr(dchi) *
+           (S * (S + K<R>::TWO * r) * dt_A_q(r, eta) -
+            K<R>::TWO * A(r, th) * dt_Sigma_q(eta) * (r + S)) /
+           (sqr(S * (S + K<R>::TWO * r)));
+  }
+  R dt_h22(const R* x) const {
+    if (kind == 2) return K<R>::ZERO;
+    if (kind == 0) {
+      const R r = x[0] * dr + x1_min, th = x[1] * dtheta + x2_min;
+      return -dt_Sigma_ks(th) / sqr(Sigma(r, th)) * sqr(dtheta_inv);
+    }
+    const R r   = r0 + std::exp(x[0] * dchi + chi_min);
+    const R eta = x[1] * deta + eta_min;
+    return -dt_Sigma_q(eta) / sqr(Sigma(r, eta2theta(eta))) / sqr(deta);
+  }
+  R dt_h33(const R* x) const {
+    if (kind == 2) {
+      const R r = x[0] * dr + x1_min, th = x[1] * dtheta + x2_min;
+      return -K<R>::TWO * std::cos(th) / sqr(r) / cube(std::sin(th)) * dtheta;
+    }
+    if (kind == 0) {
+      const R r = x[0] * dr + x1_min, th = x[1] * dtheta + x2_min;
+      return -K<R>::TWO * dtheta * std::cos(th) * (Sigma(r, th) - sqr(a) * sqr(std::sin(th))) /
+             cube(std::sin(th)) / sqr(Sigma(r, th));
+    }
+    const R r   = r0 + std::exp(x[0] * dchi + chi_min);
+    const R eta = x[1] * deta + eta_min;
+    const R th  = eta2theta(eta);
+    return -(dt_Sigma_q(eta) + K<R>::TWO * std::cos(th) / std::sin(th) * Sigma(r, th) * dx_dt(eta)) /
+           sqr(Sigma(r, th) * std::sin(th));
+  }
+  R dt_h13(const R* x) const {
+    if (kind == 2) return K<R>::ZERO;
+    if (kind == 0) {
+      const R r = x[0] * dr + x1_min, th = x[1] * dtheta + x2_min;
+      return -a * dt_Sigma_ks(th) / sqr(Sigma(r, th)) * dr_inv;
+    }
+    const R r   = r0 + std::exp(x[0] * dchi + chi_min);
+    const R eta = x[1] * deta + eta_min;
+    return -a * dt_Sigma_q(eta) / sqr(Sigma(r, eta2theta(eta))) *
+           (std::exp(-(x[0] * dchi + chi_min)) * dchi_inv);
+  }
+
+  // ---- vector transformations (kerr_schild.h:599-666, kerr_schild_0.h) ----
+  void cov_to_cntrv(const R* x, const R* vi, R* vo) const {
+    vo[0] = vi[0] * h11(x) + vi[2] * h13(x);
+    vo[1] = vi[1] * h22(x);
+    vo[2] = vi[0] * h13(x) + vi[2] * h33(x);
+  }
+  void cntrv_to_cov(const R* x, const R* vi, R* vo) const {
+    vo[0] = vi[0] * h_11(x) + vi[2] * h_13(x);
+    vo[1] = vi[1] * h_22(x);
+    vo[2] = vi[0] * h_13(x) + vi[2] * h_33(x);
+  }
+  void cntrv_to_hat(const R* x, const R* vi, R* vo) const {
+    if (kind == 2) {
+      vo[0] = vi[0] / std::sqrt(h11(x));
+      vo[1] = vi[1] * std::sqrt(h_22(x));
+      vo[2] = vi[2] * std::sqrt(h_33(x));
+      return;
+    }
+    vo[0] = vi[0] / std::sqrt(h11(x));
+    vo[1] = vi[1] * std::sqrt(h_22(x));
+    vo[2] = vi[2] * std::sqrt(h_33(x)) + vi[0] * h_13(x) / std::sqrt(h_33(x));
+  }
+  void cov_to_hat(const R* x, const R* vi, R* vo) const {
+    if (kind == 2) {
+      vo[0] = vi[0] * std::sqrt(h11(x));
+      vo[1] = vi[1] / std::sqrt(h_22(x));
+      vo[2] = vi[2] / std::sqrt(h_33(x));
+      return;
+    }
+    const R A0 = std::sqrt(h11(x));
+    vo[0] = vi[0] * A0 - vi[2] * A0 * h_13(x) / h_33(x);
+    vo[1] = vi[1] / std::sqrt(h_22(x));
+    vo[2] = vi[2] / std::sqrt(h_33(x));
+  }
+  void hat_to_cov(const R* x, const R* vi, R* vo) const {
+    if (kind == 2) {
+      vo[0] = vi[0] / std::sqrt(h11(x));
+      vo[1] = vi[1] * std::sqrt(h_22(x));
+      vo[2] = vi[2] * std::sqrt(h_33(x));
+      return;
+    }
+    vo[0] = vi[0] / std::sqrt(h11(x)) + vi[2] * h_13(x) / std::sqrt(h_33(x));
+    vo[1] = vi[1] * std::sqrt(h_22(x));
+    vo[2] = vi[2] * std::sqrt(h_33(x));
+  }
+  void hat_to_cntrv(const R* x, const R* vi, R* vo) const {
+    if (kind == 2) {
+      vo[0] = vi[0] * std::sqrt(h11(x));
+      vo[1] = vi[1] / std::sqrt(h_22(x));
+      vo[2] = vi[2] / std::sqrt(h_33(x));
+      return;
+    }
+    const R A0 = std::sqrt(h11(x));
+    vo[0] = vi[0] * A0;
+    vo[1] = vi[1] / std::sqrt(h_22(x));
+    vo[2] = vi[2] / std::sqrt(h_33(x)) - vi[0] * A0 * h_13(x) / h_33(x);
+  }
+};
+
+/* ---- GR pusher: src/kernels/pushers/gr.hpp ------------------------------ */
+template <typename R>
+struct GRPusher {
+  const ort_gr_params_t* prm;
+  GRMetric<R>            metric;
+  Fld<R>                 DB, DB0;
+  Prt<R>                 P;
+  R                      dt, ndth;
+  int                    order;
+
+  GRPusher(const ort_gr_params_t* p, ort_particles_t* prtls, const ort_field_t* em,
+           const ort_field_t* em0, int order_)
+    : prm(p), metric(p), DB(em), DB0(em0), P(prtls), dt(static_cast<R>(p->dt)), order(order_) {
+    // gr.hpp:108-109
+    ndth = K<R>::HALF * (p->charge / p->mass) * static_cast<R>(p->omegaB0) * dt;
+  }
+
+  // computeGamma (gr.hpp:236-254)
+  R gamma(bool massless, const R* ucov, const R* ucnt) const {
+    const R s = ucov[0] * ucnt[0] + ucov[1] * ucnt[1] + ucov[2] * ucnt[2];
+    return massless ? std::sqrt(s) : std::sqrt(K<R>::ONE + s);
+  }
+
+  // theta clamp of the midpoint (gr.hpp:355-368 and 742-756)
+  R clamp_theta(R theta_cd) const {
+    const R      small = static_cast<R>(SMALL_ANGLE_GR_D);
+    const double large = PI_D - static_cast<double>(small);
+    const R      th_ph = metric.theta_cd2ph(theta_cd);
+    if (th_ph < small) return metric.theta_ph2cd(small);
+    if (static_cast<double>(th_ph) >= large) return metric.theta_ph2cd(static_cast<R>(large));
+    return theta_cd;
+  }
+
+  // EMHalfPush (gr.hpp:177-217)
+  void em_half_push(const R* xp, const R* vp, const R* Dh, const R* Bh, R* vp_upd) const {
+    R D0[3] = { Dh[0], Dh[1], Dh[2] }, B0[3] = { Bh[0], Bh[1], Bh[2] };
+    R vh[3] = { K<R>::ZERO, K<R>::ZERO, K<R>::ZERO }, vuh[3];
+    metric.cov_to_hat(xp, vp, vuh);
+    R COEFF = ndth * K<R>::HALF * metric.alpha(xp);
+    D0[0] *= COEFF; D0[1] *= COEFF; D0[2] *= COEFF;
+    vuh[0] += D0[0]; vuh[1] += D0[1]; vuh[2] += D0[2];
+    COEFF *= K<R>::ONE / std::sqrt(K<R>::ONE + sqr(vuh[0]) + sqr(vuh[1]) + sqr(vuh[2]));
+    B0[0] *= COEFF; B0[1] *= COEFF; B0[2] *= COEFF;
+    COEFF = K<R>::TWO / (K<R>::ONE + sqr(B0[0]) + sqr(B0[1]) + sqr(B0[2]));
+    vh[0] = (vuh[0] + vuh[1] * B0[2] - vuh[2] * B0[1]) * COEFF;
+    vh[1] = (vuh[1] + vuh[2] * B0[0] - vuh[0] * B0[2]) * COEFF;
+    vh[2] = (vuh[2] + vuh[0] * B0[1] - vuh[1] * B0[0]) * COEFF;
+    vuh[0] += vh[1] * B0[2] - vh[2] * B0[1] + D0[0];
+    vuh[1] += vh[2] * B0[0] - vh[0] * B0[2] + D0[1];
+    vuh[2] += vh[0] * B0[1] - vh[1] * B0[0] + D0[2];
+    metric.hat_to_cov(xp, vuh, vp_upd);
+  }
+
+  // GeodesicMomentumPush, 2D (gr.hpp:260-331)
+  void momentum_push(bool massless, const R* xp, const R* vp, R* vp_upd) const {
+    R vm[3] = { K<R>::ZERO, K<R>::ZERO, K<R>::ZERO }, vmc[3] = { K<R>::ZERO, K<R>::ZERO, K<R>::ZERO };
+    vp_upd[0] = vp[0];
+    vp_upd[1] = vp[1];
+    vp_upd[2] = vp[2];
+    for (int it = 0; it < prm->niter; ++it) {
+      vm[0] = K<R>::HALF * (vp[0] + vp_upd[0]);
+      vm[1] = K<R>::HALF * (vp[1] + vp_upd[1]);
+      vm[2] = vp[2];
+      metric.cov_to_cntrv(xp, vm, vmc);
+      const R u0 = gamma(massless, vm, vmc) / metric.alpha(xp);
+      vp_upd[0] = vp[0] + dt * (-metric.alpha(xp) * u0 * metric.dr_alpha(xp) +
+                                vm[0] * metric.dr_beta1(xp) -
+                                (K<R>::HALF / u0) *
+                                  (metric.dr_h11(xp) * sqr(vm[0]) + metric.dr_h22(xp) * sqr(vm[1]) +
+                                   metric.dr_h33(xp) * sqr(vm[2]) +
+                                   K<R>::TWO * metric.dr_h13(xp) * vm[0] * vm[2]));
+      vp_upd[1] = vp[1] + dt * (-metric.alpha(xp) * u0 * metric.dt_alpha(xp) +
+                                vm[0] * metric.dt_beta1(xp) -
+                                (K<R>::HALF / u0) *
+                                  (metric.dt_h11(xp) * sqr(vm[0]) + metric.dt_h22(xp) * sqr(vm[1]) +
+                                   metric.dt_h33(xp) * sqr(vm[2]) +
+                                   K<R>::TWO * metric.dt_h13(xp) * vm[0] * vm[2]));
+    }
+  }
+
+  // GeodesicCoordinatePush, 2D (gr.hpp:333-383)
+  void coordinate_push(bool massless, const R* xp, const R* vp, R* xp_upd) const {
+    R vc[3] = { K<R>::ZERO, K<R>::ZERO, K<R>::ZERO }, xm[2] = { K<R>::ZERO, K<R>::ZERO };
+    xp_upd[0] = xp[0];
+    xp_upd[1] = xp[1];
+    for (int it = 0; it < prm->niter; ++it) {
+      xm[0] = K<R>::HALF * (xp[0] + xp_upd[0]);
+      xm[1] = K<R>::HALF * (xp[1] + xp_upd[1]);
+      const R xm_[2] = { xm[0], clamp_theta(xm[1]) };
+      metric.cov_to_cntrv(xm_, vp, vc);
+      const R g  = gamma(massless, vp, vc);
+      const R u0 = g / metric.alpha(xm_);
+      xp_upd[0] = xp[0] + dt * (vc[0] / u0 - metric.beta1(xm_));
+      xp_upd[1] = xp[1] + dt * (vc[1] / u0);
+    }
+  }
+
+  // interpolateFields (gr.hpp:482-663): D from em, B from em0
+  void interpolate(int64_t p, R* e0, R* b0) const {
+    const int i = P.i1[p] + DB.ngh, j = P.i2[p] + DB.ngh;
+    const R   dx1_ = static_cast<R>(P.dx1[p]), dx2_ = static_cast<R>(P.dx2[p]);
+    if (order == 0) {
+      const int indx = static_cast<int>(dx1_ + K<R>::HALF);
+      const int indy = static_cast<int>(dx2_ + K<R>::HALF);
+      const R ponpmx = K<R>::ONE - dx1_, ponppx = dx1_;
+      const R ponpmy = K<R>::ONE - dx2_, ponppy = dx2_;
+      const R pondmx = static_cast<R>(indx + 1) - (dx1_ + K<R>::HALF);
+      const R pondpx = K<R>::ONE - pondmx;
+      const R pondmy = static_cast<R>(indy + 1) - (dx2_ + K<R>::HALF);
+      const R pondpy = K<R>::ONE - pondmy;
+      R c00, c10;
+      // (dual, primal)
+      auto dp = [&](const Fld<R>& F, int c) {
+        c00 = F(i - 1 + indx, j, c) * pondmx + F(i + indx, j, c) * pondpx;
+        c10 = F(i - 1 + indx, j + 1, c) * pondmx + F(i + indx, j + 1, c) * pondpx;
+        return c00 * ponpmy + c10 * ponppy;
+      };
+      // (primal, dual)
+      auto pd = [&](const Fld<R>& F, int c) {
+        c00 = F(i, j - 1 + indy, c) * ponpmx + F(i + 1, j - 1 + indy, c) * ponppx;
+        c10 = F(i, j + indy, c) * ponpmx + F(i + 1, j + indy, c) * ponppx;
+        return c00 * pondmy + c10 * pondpy;
+      };
+      // (primal, primal)
+      auto pp = [&](const Fld<R>& F, int c) {
+        c00 = F(i, j, c) * ponpmx + F(i + 1, j, c) * ponppx;
+        c10 = F(i, j + 1, c) * ponpmx + F(i + 1, j + 1, c) * ponppx;
+        return c00 * ponpmy + c10 * ponppy;
+      };
+      // (dual, dual)
+      auto dd = [&](const Fld<R>& F, int c) {
+        c00 = F(i - 1 + indx, j - 1 + indy, c) * pondmx + F(i + indx, j - 1 + indy, c) * pondpx;
+        c10 = F(i - 1 + indx, j + indy, c) * pondmx + F(i + indx, j + indy, c) * pondpx;
+        return c00 * pondmy + c10 * pondpy;
+      };
+      e0[0] = dp(DB, 0);
+      e0[1] = pd(DB, 1);
+      e0[2] = pp(DB, 2);
+      b0[0] = pd(DB0, 3);
+      b0[1] = dp(DB0, 4);
+      b0[2] = dd(DB0, 5);
+      return;
+    }
+    if (order == 1) interp_shape<1>(i, j, dx1_, dx2_, e0, b0);
+    else interp_shape<2>(i, j, dx1_, dx2_, e0, b0);
+  }
+
+  template <int O>
+  static R gather(const Fld<R>& F, int c, int imin, int jmin, const R* S1, const R* S2) {
+    R e = K<R>::ZERO;
+    for (int idx2 = 0; idx2 < O + 1; idx2++) {
+      R c0 = K<R>::ZERO;
+      for (int idx1 = 0; idx1 < O + 1; idx1++) c0 += S1[idx1] * F(imin + idx1, jmin + idx2, c);
+      e += c0 * S2[idx2];
+    }
+    return e;
+  }
+
+  template <int O>
+  void interp_shape(int i, int j, R dx1_, R dx2_, R* e0, R* b0) const {
+    R   S1p[O + 1], S1d[O + 1], S2p[O + 1], S2d[O + 1];
+    int ip, id, jp, jd;
+    shape_order<R, false, O>(i, dx1_, ip, S1p);
+    shape_order<R, false, O>(j, dx2_, jp, S2p);
+    shape_order<R, true, O>(i, dx1_, id, S1d);
+    shape_order<R, true, O>(j, dx2_, jd, S2d);
+    e0[0] = gather<O>(DB, 0, id, jp, S1d, S2p);
+    e0[1] = gather<O>(DB, 1, ip, jd, S1p, S2d);
+    e0[2] = gather<O>(DB, 2, ip, jp, S1p, S2p);
+    b0[0] = gather<O>(DB0, 3, ip, jd, S1p, S2d);
+    b0[1] = gather<O>(DB0, 4, id, jp, S1d, S2p);
+    b0[2] = gather<O>(DB0, 5, id, jd, S1d, S2d);
+  }
+
+  // from_Xi_to_i_di (gr.hpp:36-45)
+  static void to_i_di(R X, int32_t& i, float& di) {
+    i  = static_cast<int>(X + 1) - 1;
+    di = static_cast<float>(X) - static_cast<float>(i);
+  }
+
+  // boundaryConditions (gr.hpp:818-865), single domain
+  void bcs(int64_t p) const {
+    if ((P.i1[p] < 0 && prm->absorb_i1min) || (P.i1[p] >= prm->ni1 && prm->absorb_i1max))
+      P.tag[p] = 0;
+    if (P.i2[p] < 0) {
+      if (prm->axis_i2min) {
+        P.i2[p]  = 0;
+        P.dx2[p] = 1.0f - P.dx2[p];
+        P.ux2[p] = -P.ux2[p];
+      }
+    } else if (P.i2[p] >= prm->ni2) {
+      if (prm->axis_i2max) {
+        P.i2[p]  = prm->ni2 - 1;
+        P.dx2[p] = 1.0f - P.dx2[p];
+        P.ux2[p] = -P.ux2[p];
+      }
+    }
+  }
+
+  // operator()(Massless_t) gr.hpp:667-723 and operator()(Massive_t) gr.hpp:727-814
+  int push(int64_t p) const {
+    if (P.tag[p] != 1) return (P.tag[p] != 0) ? 1 : 0;
+    P.i1p[p]  = P.i1[p];
+    P.i2p[p]  = P.i2[p];
+    P.dx1p[p] = P.dx1[p];
+    P.dx2p[p] = P.dx2[p];
+    const R xp[2] = { static_cast<R>(P.i1[p]) + static_cast<R>(P.dx1[p]),
+                      static_cast<R>(P.i2[p]) + static_cast<R>(P.dx2[p]) };
+    R vp[3] = { P.ux1[p], P.ux2[p], P.ux3[p] };
+    R vp_upd[3], xp_upd[2];
+    if (prm->massless) {
+      momentum_push(true, xp, vp, vp_upd);
+      coordinate_push(true, xp, vp_upd, xp_upd);
+    } else {
+      const R xp_[2] = { xp[0], clamp_theta(xp[1]) };
+      R Dc[3], Bc[3], Dh[3], Bh[3];
+      interpolate(p, Dc, Bc);
+      metric.cntrv_to_hat(xp, Dc, Dh);
+      metric.cntrv_to_hat(xp, Bc, Bh);
+      em_half_push(xp, vp, Dh, Bh, vp_upd);
+      vp[0] = vp_upd[0]; vp[1] = vp_upd[1]; vp[2] = vp_upd[2];
+      momentum_push(false, xp_, vp, vp_upd);
+      vp[0] = vp_upd[0]; vp[1] = vp_upd[1]; vp[2] = vp_upd[2];
+      em_half_push(xp, vp, Dh, Bh, vp_upd);
+      coordinate_push(false, xp, vp_upd, xp_upd);
+    }
+    // UpdatePhi (gr.hpp:459-480) takes phi by value: the stored phi is unchanged
+    int32_t i1_, i2_;
+    float   dx1_, dx2_;
+    to_i_di(xp_upd[0], i1_, dx1_);
+    to_i_di(xp_upd[1], i2_, dx2_);
+    P.i1[p]  = i1_;
+    P.dx1[p] = dx1_;
+    P.i2[p]  = i2_;
+    P.dx2[p] = dx2_;
+    P.ux1[p] = vp_upd[0];
+    P.ux2[p] = vp_upd[1];
+    P.ux3[p] = vp_upd[2];
+    bcs(p);
+    return 0;
+  }
+};
+
+template <typename R>
+int gr_push(const ort_gr_params_t* prm, ort_particles_t* prtls, const ort_field_t* em,
+            const ort_field_t* em0, int32_t order) {
+  if (prm == nullptr || prtls == nullptr || order < 0 || order > 2) return 1;
+  if (!prm->massless && (em == nullptr || em0 == nullptr)) return 1;
+  const ort_field_t* e  = em ? em : em0;
+  const ort_field_t* e0 = em0 ? em0 : em;
+  ort_field_t dummy{};
+  dummy.dim = 2;
+  dummy.ngh = 2;
+  dummy.ncomp = 6;
+  GRPusher<R> g(prm, prtls, e ? e : &dummy, e0 ? e0 : &dummy, order);
+  int err = 0;
+  for (int64_t p = 0; p < g.P.n; ++p) err |= g.push(p);
+  return err;
+}
+
+// Metric quantities at code coordinates, for the tests: per point, NQ values
+constexpr int NQ = 24;
+template <typename R>
+int gr_metric(const ort_gr_params_t* prm, int64_t n, const double* x1, const double* x2,
+              double* out) {
+  GRMetric<R> m(prm);
+  for (int64_t k = 0; k < n; ++k) {
+    const R x[2] = { static_cast<R>(x1[k]), static_cast<R>(x2[k]) };
+    double* o = out + NQ * k;
+    o[0]  = m.h_11(x);     o[1]  = m.h_22(x);     o[2]  = m.h_33(x);     o[3]  = m.h_13(x);
+    o[4]  = m.h11(x);      o[5]  = m.h22(x);      o[6]  = m.h33(x);      o[7]  = m.h13(x);
+    o[8]  = m.alpha(x);    o[9]  = m.beta1(x);
+    o[10] = m.dr_alpha(x); o[11] = m.dt_alpha(x); o[12] = m.dr_beta1(x); o[13] = m.dt_beta1(x);
+    o[14] = m.dr_h11(x);   o[15] = m.dr_h22(x);   o[16] = m.dr_h33(x);   o[17] = m.dr_h13(x);
+    o[18] = m.dt_h11(x);   o[19] = m.dt_h22(x);   o[20] = m.dt_h33(x);   o[21] = m.dt_h13(x);
+    o[22] = m.r_of(x);     o[23] = m.th_of(x);
+  }
+  return 0;
+}
+
+// vector transformations, for the tests: which = 0 D->U, 1 U->D, 2 D->T,
+// 3 T->D, 4 U->T, 5 T->U
+template <typename R>
+int gr_transform(const ort_gr_params_t* prm, int32_t which, int64_t n, const double* x1,
+                 const double* x2, const double* vin, double* vout) {
+  GRMetric<R> m(prm);
+  for (int64_t k = 0; k < n; ++k) {
+    const R x[2] = { static_cast<R>(x1[k]), static_cast<R>(x2[k]) };
+    const R vi[3] = { static_cast<R>(vin[3 * k]), static_cast<R>(vin[3 * k + 1]),
+                      static_cast<R>(vin[3 * k + 2]) };
+    R vo[3];
+    switch (which) {
+      case 0: m.cov_to_cntrv(x, vi, vo); break;
+      case 1: m.cntrv_to_cov(x, vi, vo); break;
+      case 2: m.cov_to_hat(x, vi, vo); break;
+      case 3: m.hat_to_cov(x, vi, vo); break;
+      case 4: m.cntrv_to_hat(x, vi, vo); break;
+      case 5: m.hat_to_cntrv(x, vi, vo); break;
+      default: return 1;
+    }
+    for (int c = 0; c < 3; ++c) vout[3 * k + c] = static_cast<double>(vo[c]);
+  }
+  return 0;
+}
+
+}  // namespace
+
+#define ORT_GR_DEFINE(SUF, R)                                                               \
+  extern "C" int ntt_oracle_gr_push_##SUF(const ort_gr_params_t* prm,                       \
+                                          ort_particles_t* prtls, const ort_field_t* em,    \
+                                          const ort_field_t* em0, int32_t order) {          \
+    return gr_push<R>(prm, prtls, em, em0, order);                                          \
+  }                                                                                         \
+  extern "C" int ntt_oracle_gr_metric_##SUF(const ort_gr_params_t* prm, int64_t n,          \
+                                            const double* x1, const double* x2,             \
+                                            double* out) {                                  \
+    return gr_metric<R>(prm, n, x1, x2, out);                                               \
+  }                                                                                         \
+  extern "C" int ntt_oracle_gr_transform_##SUF(const ort_gr_params_t* prm, int32_t which,   \
+                                               int64_t n, const double* x1,                 \
+                                               const double* x2, const double* vin,         \
+                                               double* vout) {                              \
+    return gr_transform<R>(prm, which, n, x1, x2, vin, vout);                               \
+  }
+
+ORT_GR_DEFINE(f32, float)
+ORT_GR_DEFINE(f64, double)
