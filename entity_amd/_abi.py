@@ -106,6 +106,22 @@ class NttPlasmaInit(C.Structure):
                 ("weight", C.c_float), ("tile", C.c_int32 * 3)]
 
 
+METRIC_KERR_SCHILD, METRIC_QKERR_SCHILD, METRIC_KERR_SCHILD_0 = 0, 1, 2
+
+
+class NttKsMetric(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("nx1", C.c_int32), ("nx2", C.c_int32),
+                ("x1_min", C.c_double), ("x1_max", C.c_double), ("x2_min", C.c_double),
+                ("x2_max", C.c_double), ("a", C.c_double), ("r0", C.c_double), ("h", C.c_double)]
+
+
+class NttGrPusherCtx(C.Structure):
+    _fields_ = [("mass", C.c_float), ("charge", C.c_float), ("dt", C.c_float),
+                ("omegaB0", C.c_float), ("epsilon", C.c_float), ("niter", C.c_int32),
+                ("ni1", C.c_int32), ("ni2", C.c_int32), ("ni3", C.c_int32),
+                ("pusher_flags", C.c_uint8), ("precision", C.c_uint8)]
+
+
 class NttError(RuntimeError):
     def __init__(self, status: int, what: str = ""):
         self.status = status
@@ -125,6 +141,8 @@ SIGNATURES: dict[str, tuple] = {
     "ntt_get_device_errors": (C.c_int, [_P(C.c_uint32)]),
     "ntt_sr_push": (C.c_int, [_P(NttSrPusherCtx), _P(NttPrtlBC), _P(NttParticles), _P(NttField),
                               _P(NttMinkowski), C.c_int32, C.c_void_p]),
+    "ntt_gr_push": (C.c_int, [_P(NttGrPusherCtx), _P(NttPrtlBC), _P(NttParticles), _P(NttField),
+                              _P(NttField), _P(NttKsMetric), C.c_int32, C.c_void_p]),
     "ntt_deposit_currents": (C.c_int, [_P(NttParticles), _P(NttField), _P(NttMinkowski),
                                        C.c_float, C.c_float, C.c_int32, C.c_void_p]),
     "ntt_faraday_mink": (C.c_int, [_P(NttField), C.c_float, C.c_float, _P(NttFdCoeffs),

@@ -14,6 +14,8 @@ Reference functors (paths relative to the Entity source tree):
 * mink.Ampere_kernel         src/kernels/ampere_mink.hpp:31-89
 * mink.CurrentsAmpere_kernel src/kernels/ampere_mink.hpp:96-215
 * DigitalFilter_kernel       src/kernels/digital_filter.hpp:58-389
+* gr.Pusher_kernel           src/kernels/pushers/gr.hpp:71-865 (metrics
+                             src/metrics/kerr_schild*.h, qkerr_schild.h)
 """
 from __future__ import annotations
 
@@ -96,6 +98,65 @@ class sr:  # namespace kernel::sr
                                       C.byref(p), C.byref(f), C.byref(self.metric.abi()),
                                       self.shape_order, _stream(stream))
             _abi.check(st, "ntt_sr_push")
+
+
+class gr:  # namespace kernel::gr
+    @dataclass
+    class PusherContext:
+        """kernel::gr::PusherContext (context.h:182-211).  precision 64: FP64
+        metric and pusher on FP32 particle storage; 32: FP32 arithmetic."""
+        pusher_flags: int
+        mass: float
+        charge: float
+        dt: float
+        omegaB0: float
+        ni1: int
+        ni2: int
+        niter: int = 10
+        epsilon: float = 1e-6
+        precision: int = 64
+
+        def abi(self) -> _abi.NttGrPusherCtx:
+            return _abi.NttGrPusherCtx(self.mass, self.charge, self.dt, self.omegaB0,
+                                       self.epsilon, self.niter, self.ni1, self.ni2, 1,
+                                       self.pusher_flags, self.precision)
+
+    @dataclass
+    class KerrSchild:
+        """metric::KerrSchild / QKerrSchild / KerrSchild0 of an (r, theta) grid."""
+        kind: int
+        n: tuple
+        r_extent: tuple
+        theta_extent: tuple = (0.0, 3.141592653589793)
+        a: float = 0.0
+        r0: float = 0.0
+        h: float = 0.0
+
+        def abi(self) -> _abi.NttKsMetric:
+            return _abi.NttKsMetric(self.kind, self.n[0], self.n[1], self.r_extent[0],
+                                    self.r_extent[1], self.theta_extent[0], self.theta_extent[1],
+                                    self.a, self.r0, self.h)
+
+    class Pusher_kernel(_Launch):
+        """kernel::gr::Pusher_kernel<M>(ctx, bc, particles, DB, DB0, metric):
+        D is gathered from DB (em), B from DB0 (em0)."""
+
+        def __init__(self, ctx, bc: PusherBoundaries, particles: Particles, DB, DB0,
+                     shape: GridShape, metric, shape_order: int = 1):
+            self.ctx, self.bc, self.particles = ctx, bc, particles
+            self.DB, self.DB0, self.shape, self.metric = DB, DB0, shape, metric
+            self.shape_order = shape_order
+
+        def __call__(self, stream=None) -> None:
+            p = self.particles.abi()
+            fe = field_abi(self.DB, self.shape) if self.DB is not None else None
+            f0 = field_abi(self.DB0, self.shape) if self.DB0 is not None else None
+            st = self.lib.ntt_gr_push(C.byref(self.ctx.abi()), C.byref(self.bc.abi()),
+                                      C.byref(p), C.byref(fe) if fe is not None else None,
+                                      C.byref(f0) if f0 is not None else None,
+                                      C.byref(self.metric.abi()), self.shape_order,
+                                      _stream(stream))
+            _abi.check(st, "ntt_gr_push")
 
 
 class DepositCurrents_kernel(_Launch):
@@ -193,6 +254,6 @@ def device_errors(strict: bool = False) -> int:
     return int(v.value)
 
 
-__all__ = ["PusherContext", "PusherBoundaries", "Minkowski", "sr", "mink",
+__all__ = ["PusherContext", "PusherBoundaries", "Minkowski", "sr", "gr", "mink",
            "DepositCurrents_kernel", "DigitalFilter_kernel", "communicate_fields",
            "synchronize_currents", "device_errors", "Fields", "GridShape", "Particles"]

@@ -179,6 +179,54 @@ NTT_API ntt_status ntt_sr_push(const ntt_sr_pusher_ctx_t* ctx,
                        int32_t                    shape_order,
                        void*                      stream);
 
+/* ------------------------------------------------------------ GR pusher */
+
+/* Metrics of the Kerr-Schild family on a 2D axisymmetric (r, theta) grid:
+ * metric::KerrSchild (src/metrics/kerr_schild.h), metric::QKerrSchild
+ * (qkerr_schild.h: r = r0 + exp(chi), theta(eta) stretched by h) and
+ * metric::KerrSchild0 (kerr_schild_0.h: a = 0, flat). */
+#define NTT_METRIC_KERR_SCHILD   0
+#define NTT_METRIC_QKERR_SCHILD  1
+#define NTT_METRIC_KERR_SCHILD_0 2
+
+typedef struct ntt_ks_metric {
+  int32_t kind;           /* NTT_METRIC_*                                   */
+  int32_t nx1, nx2;       /* active cells in r and theta                    */
+  double  x1_min, x1_max; /* r extent [r_g]                                 */
+  double  x2_min, x2_max; /* theta extent [rad]                             */
+  double  a;              /* spin (ignored by KerrSchild0)                  */
+  double  r0, h;          /* quasi-spherical parameters (QKerrSchild)       */
+} ntt_ks_metric_t;
+
+/* kernel::gr::PusherContext (src/kernels/pushers/context.h:182-211). */
+typedef struct ntt_gr_pusher_ctx {
+  float   mass, charge;
+  float   dt, omegaB0;
+  float   epsilon;        /* algorithms.gr.pusher_eps (unused by the pusher) */
+  int32_t niter;          /* algorithms.gr.pusher_niter (default 10)        */
+  int32_t ni1, ni2, ni3;
+  uint8_t pusher_flags;   /* NTT_PUSHER_PHOTON (massless) or NTT_PUSHER_BORIS */
+  uint8_t precision;      /* 64 (or 0): metric and pusher in FP64 on FP32
+                             storage; 32: the reference's FP32 arithmetic   */
+} ntt_gr_pusher_ctx_t;
+
+/* GR particle push of one species (2D): photons along geodesics, massive
+ * particles by EM half push -> geodesic momentum push -> EM half push ->
+ * geodesic coordinate push, with niter fixed-point iterations each.  D is
+ * read from em, B from em0 (contravariant components, ghost-extended; both
+ * may be NULL for photons).  Boundaries: ABSORB or HORIZON at r kills the
+ * particle, AXIS at theta reflects it.
+ * Replaces kernel::gr::Pusher_kernel<M> (src/kernels/pushers/gr.hpp:71-865)
+ * as launched by grpic::ParticlePush (src/engines/grpic/particle_pusher.h:26-89). */
+NTT_API ntt_status ntt_gr_push(const ntt_gr_pusher_ctx_t* ctx,
+                               const ntt_prtl_bc_t*       bc,
+                               ntt_particles_t*           prtls,
+                               const ntt_field_t*         em,
+                               const ntt_field_t*         em0,
+                               const ntt_ks_metric_t*     metric,
+                               int32_t                    shape_order,
+                               void*                      stream);
+
 /* Charge-conserving current deposit, accumulated (atomically) into `cur`
  * including its ghost cells, exactly where the reference writes.
  * Replaces kernel::DepositCurrents_kernel<SRPIC, Minkowski<D>, O>
