@@ -38,8 +38,16 @@
 
 namespace ntt {
 
+// LDS window extents (cells).  3D: a 4^3 tile plus the [-1, +2] reach and
+// one cell of drift on each side needs 10 per dimension.
+#ifndef NTT_PD_W3D
+#define NTT_PD_W3D 10, 10, 10
+#endif
 template <int D> struct Win;
-template <> struct Win<3> { static constexpr int W1 = 10, W2 = 10, W3 = 10; };
+template <> struct Win<3> {
+  static constexpr int W[3] = { NTT_PD_W3D };
+  static constexpr int W1 = W[0], W2 = W[1], W3 = W[2];
+};
 template <> struct Win<2> { static constexpr int W1 = 32, W2 = 16, W3 = 1; };
 
 namespace {

@@ -471,32 +471,48 @@ __device__ __forceinline__ void deposit_esirkepov_2d(Sink& J, int G, const PStat
   i1_min += G;
   i2_min += G;
   J.set_origin(i1_min, i2_min, 0);
-  // jx1: prefix sum over i of -Q*Wx1
+  // jx1: prefix sum over i of -Q*Wx1.  The per-slot coefficients
+  // -Qdt/2 (fS - iS) and the transverse sums are formed once, so every slot
+  // is one FMA (a reassociation of the reference's product: round-off only)
+  float c1[M], c2[M], s1[M], s2[M];
+#pragma unroll
+  for (int a = 0; a < M; ++a) {
+    c1[a] = -Qdt * (HALF * (fS1[a] - iS1[a]));
+    s1[a] = fS1[a] + iS1[a];
+  }
+#pragma unroll
+  for (int b = 0; b < M; ++b) {
+    c2[b] = -Qdt * (HALF * (fS2[b] - iS2[b]));
+    s2[b] = fS2[b] + iS2[b];
+  }
   float jx1[M][M], jx2[M][M];
 #pragma unroll
-  for (int b = 0; b < M; ++b) jx1[0][b] = -Qdt * (HALF * (fS1[0] - iS1[0]) * (fS2[b] + iS2[b]));
+  for (int b = 0; b < M; ++b) jx1[0][b] = c1[0] * s2[b];
 #pragma unroll
   for (int a = 1; a < M; ++a)
 #pragma unroll
-    for (int b = 0; b < M; ++b)
-      jx1[a][b] = jx1[a - 1][b] - Qdt * (HALF * (fS1[a] - iS1[a]) * (fS2[b] + iS2[b]));
+    for (int b = 0; b < M; ++b) jx1[a][b] = fmaf(c1[a], s2[b], jx1[a - 1][b]);
 #pragma unroll
-  for (int a = 0; a < M; ++a) jx2[a][0] = -Qdt * (HALF * (fS1[a] + iS1[a]) * (fS2[0] - iS2[0]));
+  for (int a = 0; a < M; ++a) jx2[a][0] = c2[0] * s1[a];
 #pragma unroll
   for (int b = 1; b < M; ++b)
 #pragma unroll
-    for (int a = 0; a < M; ++a)
-      jx2[a][b] = jx2[a][b - 1] - Qdt * (HALF * (fS1[a] + iS1[a]) * (fS2[b] - iS2[b]));
+    for (int a = 0; a < M; ++a) jx2[a][b] = fmaf(c2[b], s1[a], jx2[a][b - 1]);
+  // jx3 = QVx3 * THIRD * (fS2 (iS1/2 + fS1) + iS2 (fS1/2 + iS1))
+  const float q3 = QVx3 * THIRD;
+  float p1[M], r1[M];
+#pragma unroll
+  for (int a = 0; a < M; ++a) {
+    p1[a] = HALF * iS1[a] + fS1[a];
+    r1[a] = HALF * fS1[a] + iS1[a];
+  }
 #pragma unroll
   for (int a = 0; a < M; ++a)
 #pragma unroll
     for (int b = 0; b < M; ++b) {
       if (a < d1 && b <= d2) J.add_rel(a, b, 0, 0, jx1[a][b]);
       if (a <= d1 && b < d2) J.add_rel(a, b, 0, 1, jx2[a][b]);
-      if (a <= d1 && b <= d2) {
-        const float w3 = THIRD * (fS2[b] * (HALF * iS1[a] + fS1[a]) + iS2[b] * (HALF * fS1[a] + iS1[a]));
-        J.add_rel(a, b, 0, 2, QVx3 * w3);
-      }
+      if (a <= d1 && b <= d2) J.add_rel(a, b, 0, 2, q3 * fmaf(fS2[b], p1[a], iS2[b] * r1[a]));
     }
 }
 
@@ -515,6 +531,18 @@ __device__ __forceinline__ void deposit_esirkepov_3d(Sink& J, int G, const PStat
   i2_min += G;
   i3_min += G;
   J.set_origin(i1_min, i2_min, i3_min);
+  // Per-slot form: J = prefix over the longitudinal index of
+  // -Qdt/3 (fS - iS) times the transverse weight t = iS iS' + fS fS' +
+  // (iS fS' + fS iS')/2; the longitudinal coefficients are formed once, so a
+  // slot costs one FMA (a reassociation of the reference's product:
+  // round-off only)
+  float c1[M], c2[M], c3[M];
+#pragma unroll
+  for (int a = 0; a < M; ++a) {
+    c1[a] = -Qdt * (THIRD * (fS1[a] - iS1[a]));
+    c2[a] = -Qdt * (THIRD * (fS2[a] - iS2[a]));
+    c3[a] = -Qdt * (THIRD * (fS3[a] - iS3[a]));
+  }
   // jx1 (running prefix over a)
 #pragma unroll
   for (int b = 0; b < M; ++b) {
@@ -524,7 +552,7 @@ __device__ __forceinline__ void deposit_esirkepov_3d(Sink& J, int G, const PStat
       float acc = ZERO;
 #pragma unroll
       for (int a = 0; a < M; ++a) {
-        acc = (a == 0 ? ZERO : acc) - Qdt * (THIRD * (fS1[a] - iS1[a]) * t23);
+        acc = (a == 0) ? c1[0] * t23 : fmaf(c1[a], t23, acc);
         if (a < d1 && b <= d2 && g <= d3) J.add_rel(a, b, g, 0, acc);
       }
     }
@@ -538,7 +566,7 @@ __device__ __forceinline__ void deposit_esirkepov_3d(Sink& J, int G, const PStat
       float acc = ZERO;
 #pragma unroll
       for (int b = 0; b < M; ++b) {
-        acc = (b == 0 ? ZERO : acc) - Qdt * (THIRD * (fS2[b] - iS2[b]) * t13);
+        acc = (b == 0) ? c2[0] * t13 : fmaf(c2[b], t13, acc);
         if (a <= d1 && b < d2 && g <= d3) J.add_rel(a, b, g, 1, acc);
       }
     }
@@ -552,7 +580,7 @@ __device__ __forceinline__ void deposit_esirkepov_3d(Sink& J, int G, const PStat
       float acc = ZERO;
 #pragma unroll
       for (int g = 0; g < M; ++g) {
-        acc = (g == 0 ? ZERO : acc) - Qdt * (THIRD * (fS3[g] - iS3[g]) * t12);
+        acc = (g == 0) ? c3[0] * t12 : fmaf(c3[g], t12, acc);
         if (a <= d1 && b <= d2 && g < d3) J.add_rel(a, b, g, 2, acc);
       }
     }
