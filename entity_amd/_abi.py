@@ -100,6 +100,13 @@ class NttSrStepParams(C.Structure):
                 ("particles_per_thread", C.c_int32), ("weight_max", C.c_float)]
 
 
+class NttBox(C.Structure):
+    _fields_ = [("em", NttField), ("cur", NttField), ("buff", NttField),
+                ("species", C.POINTER(NttSpecies)), ("nspecies", C.c_int32),
+                ("prm", NttSrStepParams), ("metric", NttMinkowski), ("workspace", C.c_void_p),
+                ("workspace_bytes", C.c_uint64)]
+
+
 class NttPlasmaInit(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("position_seed", C.c_uint64), ("species_id", C.c_int32),
                 ("ppc", C.c_int32), ("temperature", C.c_float), ("drift_u", C.c_float * 3),
@@ -157,6 +164,9 @@ SIGNATURES: dict[str, tuple] = {
                               _P(NttSrStepParams), _P(NttMinkowski), C.c_void_p, C.c_uint64,
                               C.c_void_p]),
     "ntt_sr_step_workspace_size": (C.c_int, [_P(NttField), _P(C.c_uint64)]),
+    "ntt_ensemble_create": (C.c_int, [_P(NttBox), C.c_int32, _P(C.c_void_p)]),
+    "ntt_ensemble_step": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
+    "ntt_ensemble_destroy": (C.c_int, [C.c_void_p]),
     "ntt_sort_workspace_size": (C.c_int, [C.c_int64, C.c_int32, _P(C.c_int32), _P(C.c_int32),
                                           _P(C.c_uint64)]),
     "ntt_sort_particles": (C.c_int, [_P(NttParticles), C.c_int32, _P(C.c_int32), _P(C.c_int32),

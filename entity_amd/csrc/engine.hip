@@ -67,25 +67,50 @@ extern "C" ntt_status ntt_sr_step_workspace_size(const ntt_field_t* cur, uint64_
   return NTT_OK;
 }
 
-extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t* buff,
-                                  ntt_species_t* species, int32_t nspecies,
-                                  const ntt_sr_step_params_t* prm, const ntt_minkowski_t* metric,
-                                  void* workspace, uint64_t workspace_bytes, void* stream) {
+namespace {
+
+// Everything one SRPIC step of a periodic box needs, validated and derived
+// from the step's arguments before anything is enqueued.
+struct Prep {
+  ParticleView pv;
+  PrtlBCs      bcs;
+  float        ndth;
+  int          alg;
+  bool         run, dep;
+};
+
+struct StepPlan {
+  FieldView           E, J, Bf;
+  unsigned long long* acc;
+  int                 D;
+  float               dt, dxh;
+  bool                ext;               // extended Faraday stencil
+  std::vector<Prep>   prep;
+  float               fscale;
+  float               far_c1, far_c2;    // Faraday(1/2)
+  float               amp_c1, amp_c2, amp_coeff;
+};
+
+ntt_status plan_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t* buff,
+                     ntt_species_t* species, int32_t nspecies, const ntt_sr_step_params_t* prm,
+                     const ntt_minkowski_t* metric, void* workspace, uint64_t workspace_bytes,
+                     StepPlan* P) {
   if (prm == nullptr || metric == nullptr || (nspecies > 0 && species == nullptr) || nspecies < 0)
     return NTT_ERR_ARG;
-  FieldView E, J, Bf;
-  ntt_status s = make_field_view(em, 6, &E);
+  ntt_status s = make_field_view(em, 6, &P->E);
   if (s != NTT_OK) return s;
-  s = make_field_view(cur, 3, &J);
+  s = make_field_view(cur, 3, &P->J);
   if (s != NTT_OK) return s;
-  s = make_field_view(buff, 3, &Bf);
+  s = make_field_view(buff, 3, &P->Bf);
   if (s != NTT_OK) return s;
+  const FieldView &E = P->E, &J = P->J, &Bf = P->Bf;
   if (!same_grid(E, J) || !same_grid(E, Bf) || J.d == Bf.d) return NTT_ERR_ARG;
   const uint64_t ws_need = static_cast<uint64_t>(J.cstride) * 3 * sizeof(unsigned long long);
-  if (prm != nullptr && prm->deposit_enable && (workspace == nullptr || workspace_bytes < ws_need))
+  if (prm->deposit_enable && (workspace == nullptr || workspace_bytes < ws_need))
     return NTT_ERR_CAPACITY;
-  unsigned long long* acc = static_cast<unsigned long long*>(workspace);
+  P->acc = static_cast<unsigned long long*>(workspace);
   const int D = em->dim;
+  P->D = D;
   if (metric->dim != D || !(metric->dx > 0.0f) || !(prm->dt > 0.0f)) return NTT_ERR_ARG;
   if (E.ngh != 2) return NTT_ERR_UNSUPPORTED;
   const int na[3] = { E.n1, E.n2, E.n3 };
@@ -95,24 +120,15 @@ extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t
       return NTT_ERR_UNSUPPORTED;
   }
   if (prm->shape_order < 0 || prm->shape_order > 2 || prm->n_filters < 0) return NTT_ERR_ARG;
-  hipStream_t st = as_stream(stream);
   const float dt = prm->dt;
+  P->dt = dt;
   // dx = sqrt(h_11) as the drivers compute it (fieldsolvers.h:50,110)
   const float dxh = sqrtf(metric->dx * metric->dx);
-  const bool  ext = fd_nonzero(prm->fd);
-
-  // validate every species before enqueueing anything
-  struct Prep {
-    ParticleView pv;
-    PrtlBCs      bcs;
-    float        ndth;
-    int          alg;
-    bool         run, dep;
-  };
-  Prep prep[64];
-  if (nspecies > 64) return NTT_ERR_UNSUPPORTED;
+  P->dxh = dxh;
+  P->ext = fd_nonzero(prm->fd);
+  P->prep.assign(nspecies, Prep{});
   for (int sp = 0; sp < nspecies; ++sp) {
-    Prep& p = prep[sp];
+    Prep& p = P->prep[sp];
     p.run = species[sp].pusher_flags != NTT_PUSHER_NONE && species[sp].prtls.npart > 0;
     p.dep = p.run && prm->deposit_enable && species[sp].charge != 0.0f;
     if (!p.run) continue;
@@ -124,20 +140,46 @@ extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t
                      &p.alg, &p.pv, &fv, &p.bcs, &p.ndth);
     if (s != NTT_OK) return s;
   }
+  // fixed-point scale of the accumulator, common to every species
+  float qmax = 0.0f;
+  for (int sp = 0; sp < nspecies; ++sp)
+    if (P->prep[sp].dep) qmax = fmaxf(qmax, fabsf(species[sp].charge));
+  const float wmax = prm->weight_max > 0.0f ? prm->weight_max : 1.0f;
+  P->fscale = fixed_point_scale(qmax * wmax, dt);
+  // fieldsolvers.h:44-47: Faraday(1/2) coefficients
+  const float dT = HALF * prm->correction * dt;
+  if (D == 2) { P->far_c1 = dT / (dxh * dxh); P->far_c2 = dT; }
+  else { P->far_c1 = dT / dxh; P->far_c2 = ZERO; }
+  // fieldsolvers.h:108-117, 157: Ampere and the current term
+  const float dTa = ONE * prm->correction * dt;
+  if (D == 2) { P->amp_c1 = dTa / (dxh * dxh); P->amp_c2 = dTa; }
+  else { P->amp_c1 = dTa / dxh; P->amp_c2 = ZERO; }
+  P->amp_coeff = -dt * prm->q0 / (prm->B0 * prm->V0);
+  return NTT_OK;
+}
+
+}  // namespace
+
+extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t* buff,
+                                  ntt_species_t* species, int32_t nspecies,
+                                  const ntt_sr_step_params_t* prm, const ntt_minkowski_t* metric,
+                                  void* workspace, uint64_t workspace_bytes, void* stream) {
+  StepPlan P;
+  ntt_status s = plan_step(em, cur, buff, species, nspecies, prm, metric, workspace,
+                           workspace_bytes, &P);
+  if (s != NTT_OK) return s;
+  hipStream_t st = as_stream(stream);
+  const FieldView &E = P.E, &J = P.J, &Bf = P.Bf;
 
   // srpic.hpp:71-82: fill E and B ghosts at step 0
   if (prm->step == 0) launch_ghost_fill(E, 0, 6, st);
 
   auto faraday_half = [&]() -> ntt_status {
-    const float dT = HALF * prm->correction * dt;  // fieldsolvers.h:44-47
-    float c1, c2;
-    if (D == 2) { c1 = dT / (dxh * dxh); c2 = dT; }
-    else { c1 = dT / dxh; c2 = ZERO; }
-    if (!ext) {
-      launch_faraday_periodic(E, c1, c2, st);
+    if (!P.ext) {
+      launch_faraday_periodic(E, P.far_c1, P.far_c2, st);
       return check_launch();
     }
-    ntt_status r = ntt_faraday_mink(em, c1, c2, &prm->fd, stream);
+    ntt_status r = ntt_faraday_mink(em, P.far_c1, P.far_c2, &prm->fd, stream);
     if (r != NTT_OK) return r;
     launch_ghost_fill(E, 3, 6, st);
     return check_launch();
@@ -149,28 +191,22 @@ extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t
   }
   // currents.h:68: J = 0 -- here the fixed-point accumulator (a fill kernel
   // rather than a memset, so that the step can be captured in a graph)
-  if (prm->deposit_enable) launch_zero_u64(acc, J.cstride * 3, st);
-  // fixed-point scale of the accumulator, common to every species
-  float qmax = 0.0f;
-  for (int sp = 0; sp < nspecies; ++sp)
-    if (prep[sp].dep) qmax = fmaxf(qmax, fabsf(species[sp].charge));
-  const float wmax   = prm->weight_max > 0.0f ? prm->weight_max : 1.0f;
-  const float fscale = fixed_point_scale(qmax * wmax, dt);
+  if (prm->deposit_enable) launch_zero_u64(P.acc, J.cstride * 3, st);
   for (int sp = 0; sp < nspecies; ++sp) {
-    const Prep& p = prep[sp];
+    const Prep& p = P.prep[sp];
     if (!p.run) continue;
     const int ppt = prm->particles_per_thread > 0 ? prm->particles_per_thread : 4;
     std::pair<hipEvent_t, hipEvent_t> ev;
     const bool timed = prof_begin(st, &ev);
-    s = launch_push_deposit_periodic(p.pv, E, J, acc, D, prm->shape_order, p.alg, p.ndth, dt,
-                                     metric->dx, species[sp].charge, p.dep, fscale,
+    s = launch_push_deposit_periodic(p.pv, E, J, P.acc, P.D, prm->shape_order, p.alg, p.ndth,
+                                     P.dt, metric->dx, species[sp].charge, p.dep, P.fscale,
                                      species[sp].tile_offsets, species[sp].ntiles, ppt, st);
     if (timed) prof_end(st, ev);
     if (s != NTT_OK) return s;
   }
   if (prm->deposit_enable) {
     // folded current (active cells + periodic ghosts) from the accumulator
-    launch_j_from_fixed(J, acc, 1.0 / static_cast<double>(fscale), st);
+    launch_j_from_fixed(J, P.acc, 1.0 / static_cast<double>(P.fscale), st);
     // currents.h:89-119: n filter passes (ping-pong between cur and buff)
     if (prm->n_filters > 0) {
       const FieldView* src = &J;
@@ -187,15 +223,230 @@ extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t
   if (prm->fieldsolver_enable) {
     s = faraday_half();
     if (s != NTT_OK) return s;
-    const float dT = ONE * prm->correction * dt;
-    float c1, c2;
-    if (D == 2) { c1 = dT / (dxh * dxh); c2 = dT; }
-    else { c1 = dT / dxh; c2 = ZERO; }
-    // fieldsolvers.h:157: coeff = -dt * q0 / (B0 * V0)
-    const float coeff = -dt * prm->q0 / (prm->B0 * prm->V0);
-    launch_ampere_currents_periodic(E, J, c1, c2, coeff, prm->ppc0, prm->deposit_enable != 0, st);
+    launch_ampere_currents_periodic(E, J, P.amp_c1, P.amp_c2, P.amp_coeff, prm->ppc0,
+                                    prm->deposit_enable != 0, st);
   }
   return check_launch();
+}
+
+// ---- ensembles of independent periodic boxes ------------------------------------
+struct ntt_ensemble {
+  struct Group {
+    int      D, order, alg;
+    void*    table;    // device FusedArgs[nent]
+    int64_t* chunk0;   // device, nent + 1
+    int      nent;
+    int64_t  nchunks;
+  };
+  struct Seg {
+    void*   dev;
+    int     n;
+    int64_t blocks;
+  };
+  std::vector<FieldView> ghost_E;   // step 0: E/B ghost fill per box
+  Seg                    far, zero, jfix, amp, copy;
+  std::vector<Seg>       filter_pass;
+  std::vector<Group>     groups;
+  std::vector<void*>     allocations;
+  int                    device = 0;
+};
+
+namespace {
+
+ntt_status upload(ntt_ensemble* e, const void* host, size_t bytes, void** dev) {
+  *dev = nullptr;
+  if (bytes == 0) return NTT_OK;
+  if (hipMalloc(dev, bytes) != hipSuccess) return NTT_ERR_HIP;
+  e->allocations.push_back(*dev);
+  if (hipMemcpy(*dev, host, bytes, hipMemcpyHostToDevice) != hipSuccess) return NTT_ERR_HIP;
+  return NTT_OK;
+}
+
+template <class T>
+ntt_status upload_segs(ntt_ensemble* e, std::vector<T>& v, int64_t blocks, ntt_ensemble::Seg* s) {
+  s->n = static_cast<int>(v.size());
+  s->blocks = blocks;
+  return upload(e, v.data(), v.size() * sizeof(T), &s->dev);
+}
+
+void destroy(ntt_ensemble* e) {
+  for (void* p : e->allocations) (void)hipFree(p);
+  delete e;
+}
+
+}  // namespace
+
+extern "C" ntt_status ntt_ensemble_create(const ntt_box_t* boxes, int32_t nboxes,
+                                          ntt_ensemble_t* out) {
+  if (out == nullptr || nboxes < 0 || (nboxes > 0 && boxes == nullptr)) return NTT_ERR_ARG;
+  *out = nullptr;
+  ntt_ensemble* e = new ntt_ensemble();
+  (void)hipGetDevice(&e->device);
+  std::vector<FieldSeg> far, jfix, amp;
+  std::vector<FlatSeg>  zero, copy;
+  std::vector<std::vector<FieldSeg>> passes;
+  int64_t nb_far = 0, nb_jfix = 0, nb_amp = 0, nb_zero = 0, nb_copy = 0;
+  std::vector<int64_t> nb_pass;
+  struct GroupHost {
+    int D, order, alg;
+    std::vector<char>    args;
+    std::vector<int64_t> chunk0;
+  };
+  std::vector<GroupHost> groups;
+  const size_t asz = fused_args_size();
+  ntt_status s = NTT_OK;
+  for (int b = 0; b < nboxes && s == NTT_OK; ++b) {
+    const ntt_box_t& B = boxes[b];
+    ntt_field_t em = B.em, cur = B.cur, buff = B.buff;
+    StepPlan P;
+    s = plan_step(&em, &cur, &buff, B.species, B.nspecies, &B.prm, &B.metric, B.workspace,
+                  B.workspace_bytes, &P);
+    if (s != NTT_OK) break;
+    if (P.ext) { s = NTT_ERR_UNSUPPORTED; break; }  // extended Faraday: use ntt_sr_step
+    const ntt_sr_step_params_t& prm = B.prm;
+    e->ghost_E.push_back(P.E);
+    FieldSeg g{};
+    int32_t gx, gy;
+    const int64_t nblk = field_seg_blocks(P.E, &gx, &gy);
+    g.gx = gx;
+    g.gy = gy;
+    if (prm.fieldsolver_enable) {
+      g.F = P.E;
+      g.c1 = P.far_c1;
+      g.c2 = P.far_c2;
+      g.blk0 = nb_far;
+      far.push_back(g);
+      nb_far += nblk;
+      FieldSeg a = g;
+      a.J = P.J;
+      a.c1 = P.amp_c1;
+      a.c2 = P.amp_c2;
+      a.coeff = P.amp_coeff;
+      a.ppc0 = prm.ppc0;
+      a.with_j = prm.deposit_enable != 0;
+      a.blk0 = nb_amp;
+      amp.push_back(a);
+      nb_amp += nblk;
+    }
+    if (prm.deposit_enable) {
+      const int64_t n = P.J.cstride * 3;
+      zero.push_back(FlatSeg{ P.acc, nullptr, n, nb_zero });
+      nb_zero += flat_seg_blocks(n);
+      FieldSeg j{};
+      j.gx = gx;
+      j.gy = gy;
+      j.F = P.J;
+      j.acc = P.acc;
+      j.inv_scale = 1.0 / static_cast<double>(P.fscale);
+      j.blk0 = nb_jfix;
+      jfix.push_back(j);
+      nb_jfix += nblk;
+      const FieldView* src = &P.J;
+      const FieldView* dst = &P.Bf;
+      for (int f = 0; f < prm.n_filters; ++f) {
+        if (static_cast<int>(passes.size()) <= f) {
+          passes.emplace_back();
+          nb_pass.push_back(0);
+        }
+        FieldSeg q{};
+        q.gx = gx;
+        q.gy = gy;
+        q.F = *dst;
+        q.J = *src;
+        q.blk0 = nb_pass[f];
+        passes[f].push_back(q);
+        nb_pass[f] += nblk;
+        std::swap(src, dst);
+      }
+      if (src != &P.J) {
+        copy.push_back(FlatSeg{ P.J.d, P.Bf.d, n, nb_copy });
+        nb_copy += flat_seg_blocks(n);
+      }
+    }
+    const int ppt = prm.particles_per_thread > 0 ? prm.particles_per_thread : 4;
+    for (int sp = 0; sp < B.nspecies && s == NTT_OK; ++sp) {
+      const Prep& p = P.prep[sp];
+      if (!p.run) continue;
+      const int order = (p.alg == ALG_PHOTON) ? 0 : prm.shape_order;
+      GroupHost* grp = nullptr;
+      for (auto& gh : groups)
+        if (gh.D == P.D && gh.order == order && gh.alg == p.alg) grp = &gh;
+      if (grp == nullptr) {
+        groups.push_back(GroupHost{ P.D, order, p.alg, {}, { 0 } });
+        grp = &groups.back();
+      }
+      std::vector<char> a(asz);
+      int64_t           nch = 0;
+      s = build_fused_args(a.data(), &nch, p.pv, P.E, P.J, P.acc, P.D, p.ndth, P.dt, B.metric.dx,
+                           B.species[sp].charge, p.dep, P.fscale, B.species[sp].tile_offsets,
+                           B.species[sp].ntiles, ppt);
+      if (s != NTT_OK) break;
+      if (nch == 0) continue;
+      grp->args.insert(grp->args.end(), a.begin(), a.end());
+      grp->chunk0.push_back(grp->chunk0.back() + nch);
+    }
+  }
+  if (s == NTT_OK) s = upload_segs(e, far, nb_far, &e->far);
+  if (s == NTT_OK) s = upload_segs(e, amp, nb_amp, &e->amp);
+  if (s == NTT_OK) s = upload_segs(e, jfix, nb_jfix, &e->jfix);
+  if (s == NTT_OK) s = upload_segs(e, zero, nb_zero, &e->zero);
+  if (s == NTT_OK) s = upload_segs(e, copy, nb_copy, &e->copy);
+  for (size_t f = 0; f < passes.size() && s == NTT_OK; ++f) {
+    ntt_ensemble::Seg seg;
+    s = upload_segs(e, passes[f], nb_pass[f], &seg);
+    e->filter_pass.push_back(seg);
+  }
+  for (auto& gh : groups) {
+    if (s != NTT_OK) break;
+    ntt_ensemble::Group g{ gh.D, gh.order, gh.alg, nullptr, nullptr,
+                           static_cast<int>(gh.chunk0.size()) - 1, gh.chunk0.back() };
+    if (g.nent == 0) continue;
+    s = upload(e, gh.args.data(), gh.args.size(), &g.table);
+    if (s == NTT_OK) {
+      void* c0 = nullptr;
+      s = upload(e, gh.chunk0.data(), gh.chunk0.size() * sizeof(int64_t), &c0);
+      g.chunk0 = static_cast<int64_t*>(c0);
+    }
+    e->groups.push_back(g);
+  }
+  if (s != NTT_OK) {
+    destroy(e);
+    return s;
+  }
+  *out = e;
+  return NTT_OK;
+}
+
+extern "C" ntt_status ntt_ensemble_step(ntt_ensemble_t e, int64_t step, void* stream) {
+  if (e == nullptr) return NTT_ERR_ARG;
+  hipStream_t st = as_stream(stream);
+  if (step == 0)
+    for (const FieldView& E : e->ghost_E) launch_ghost_fill(E, 0, 6, st);
+  auto fields = [&](int op, const ntt_ensemble::Seg& sg) {
+    launch_fields_batched(op, static_cast<const FieldSeg*>(sg.dev), sg.n, sg.blocks, st);
+  };
+  auto flat = [&](int op, const ntt_ensemble::Seg& sg) {
+    launch_flat_batched(op, static_cast<const FlatSeg*>(sg.dev), sg.n, sg.blocks, st);
+  };
+  fields(SEG_FARADAY, e->far);
+  flat(FLAT_ZERO_U64, e->zero);
+  for (const auto& g : e->groups) {
+    ntt_status s = launch_push_deposit_batched(g.D, g.order, g.alg, g.table, g.chunk0, g.nent,
+                                               g.nchunks, st);
+    if (s != NTT_OK) return s;
+  }
+  fields(SEG_JFIX, e->jfix);
+  for (const auto& p : e->filter_pass) fields(SEG_FILTER, p);
+  flat(FLAT_COPY_F32, e->copy);
+  fields(SEG_FARADAY, e->far);
+  fields(SEG_AMPERE, e->amp);
+  return check_launch();
+}
+
+extern "C" ntt_status ntt_ensemble_destroy(ntt_ensemble_t e) {
+  if (e == nullptr) return NTT_OK;
+  destroy(e);
+  return NTT_OK;
 }
 
 extern "C" ntt_status ntt_profile_enable(int32_t on) {

@@ -4,7 +4,7 @@
 // One thread per active cell, x fastest (64-wide rows), so that every
 // wavefront reads and writes contiguous 256-byte segments of each component
 // plane.  These kernels are HBM-bound.
-#include "ntt_common.hpp"
+#include "internal.hpp"
 
 namespace ntt {
 namespace {
@@ -293,11 +293,9 @@ __device__ __forceinline__ void store_with_ghosts(const FieldView& F, int i, int
   }
 }
 
-__global__ __launch_bounds__(BX* BY) void k_faraday_periodic(FieldView F, float c1, float c2) {
-  const CellIdx c = active_cell(F);
-  if (!c.ok) return;
+__device__ __forceinline__ void faraday_periodic_cell(const FieldView& F, int i, int j, int k,
+                                                      float c1, float c2) {
   const PIdx X{ F.ngh, F.n1 }, Y{ F.ngh, F.n2 }, Z{ F.ngh, F.n3 };
-  const int i = c.i, j = c.j, k = c.k;
   const int ip = X.at(i, 1), jp = Y.at(j, 1);
   if (F.N3 == 1) {
     const float b1 = F(i, j, 0, 3) + c1 * (-(F(i, jp, 0, 2) - F(i, j, 0, 2)));
@@ -317,16 +315,19 @@ __global__ __launch_bounds__(BX* BY) void k_faraday_periodic(FieldView F, float 
   }
 }
 
-// Ampere (E += coeff * curl B) followed by the current term (E += J * coeff,
-// J /= ppc0) in one pass; with_j = false skips the current term.
-__global__ __launch_bounds__(BX* BY) void k_ampere_currents_periodic(FieldView F, FieldView J,
-                                                                     float c1, float c2,
-                                                                     float coeff, float ppc0,
-                                                                     int with_j) {
+__global__ __launch_bounds__(BX* BY) void k_faraday_periodic(FieldView F, float c1, float c2) {
   const CellIdx c = active_cell(F);
   if (!c.ok) return;
+  faraday_periodic_cell(F, c.i, c.j, c.k, c1, c2);
+}
+
+// Ampere (E += coeff * curl B) followed by the current term (E += J * coeff,
+// J /= ppc0) in one pass; with_j = false skips the current term.
+__device__ __forceinline__ void ampere_currents_periodic_cell(const FieldView& F, const FieldView& J,
+                                                              int i, int j, int k, float c1,
+                                                              float c2, float coeff, float ppc0,
+                                                              int with_j) {
   const PIdx X{ F.ngh, F.n1 }, Y{ F.ngh, F.n2 }, Z{ F.ngh, F.n3 };
-  const int i = c.i, j = c.j, k = c.k;
   const int im = X.at(i, -1), jm = Y.at(j, -1);
   float e[3];
   if (F.N3 == 1) {
@@ -349,12 +350,19 @@ __global__ __launch_bounds__(BX* BY) void k_ampere_currents_periodic(FieldView F
   }
 }
 
-// one filter pass dst = K * src (periodic reads), 2D and 3D
-__global__ __launch_bounds__(BX* BY) void k_filter_periodic(FieldView Dst, FieldView Src) {
-  const CellIdx c = active_cell(Dst);
+__global__ __launch_bounds__(BX* BY) void k_ampere_currents_periodic(FieldView F, FieldView J,
+                                                                     float c1, float c2,
+                                                                     float coeff, float ppc0,
+                                                                     int with_j) {
+  const CellIdx c = active_cell(F);
   if (!c.ok) return;
+  ampere_currents_periodic_cell(F, J, c.i, c.j, c.k, c1, c2, coeff, ppc0, with_j);
+}
+
+// one filter pass dst = K * src (periodic reads), 2D and 3D
+__device__ __forceinline__ void filter_periodic_cell(const FieldView& Dst, const FieldView& Src,
+                                                     int i, int j, int k) {
   const PIdx X{ Src.ngh, Src.n1 }, Y{ Src.ngh, Src.n2 }, Z{ Src.ngh, Src.n3 };
-  const int i = c.i, j = c.j, k = c.k;
   const int im = X.at(i, -1), ip = X.at(i, 1), jm = Y.at(j, -1), jp = Y.at(j, 1);
   for (int comp = 0; comp < 3; ++comp) {
     float v;
@@ -377,6 +385,12 @@ __global__ __launch_bounds__(BX* BY) void k_filter_periodic(FieldView Dst, Field
   }
 }
 
+__global__ __launch_bounds__(BX* BY) void k_filter_periodic(FieldView Dst, FieldView Src) {
+  const CellIdx c = active_cell(Dst);
+  if (!c.ok) return;
+  filter_periodic_cell(Dst, Src, c.i, c.j, c.k);
+}
+
 __global__ void k_fill_zero(float* __restrict__ d, int64_t n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
@@ -391,22 +405,83 @@ __global__ void k_fill_zero_u64(unsigned long long* __restrict__ d, int64_t n) {
 
 // float current from the 64-bit fixed-point accumulator (units of 1/scale,
 // a power of two): exact in double below 2^53, then one rounding to float
+__device__ __forceinline__ void j_from_fixed_cell(const FieldView& J,
+                                                  const unsigned long long* __restrict__ acc,
+                                                  double inv_scale, int i, int j, int k) {
+  for (int comp = 0; comp < 3; ++comp) {
+    const long long q = static_cast<long long>(acc[J.idx(i, j, k, comp)]);
+    const float v = static_cast<float>(static_cast<double>(q) * inv_scale);
+    store_with_ghosts(J, i, j, k, comp, v);
+  }
+}
+
 __global__ __launch_bounds__(BX* BY) void k_j_from_fixed(FieldView J,
                                                          const unsigned long long* __restrict__ acc,
                                                          double inv_scale) {
   const CellIdx c = active_cell(J);
   if (!c.ok) return;
-  for (int comp = 0; comp < 3; ++comp) {
-    const long long q = static_cast<long long>(acc[J.idx(c.i, c.j, c.k, comp)]);
-    const float v = static_cast<float>(static_cast<double>(q) * inv_scale);
-    store_with_ghosts(J, c.i, c.j, c.k, comp, v);
-  }
+  j_from_fixed_cell(J, acc, inv_scale, c.i, c.j, c.k);
 }
 
 __global__ void k_copy(float* __restrict__ dst, const float* __restrict__ src, int64_t n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t k = i; k < n; k += stride) dst[k] = src[k];
+}
+
+// ---- batched launches over the boxes of an ensemble ---------------------------
+// Segment s owns blocks [segs[s].blk0, segs[s + 1].blk0) of a 1D grid; a block
+// finds its segment by binary search and runs the same cell function as the
+// single-box kernel, so every box gets the arithmetic of its own launch.
+template <class Seg>
+__device__ __forceinline__ int find_seg(const Seg* __restrict__ segs, int nseg, int64_t b) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].blk0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int OP>
+__global__ __launch_bounds__(BX* BY) void k_fields_batched(const FieldSeg* __restrict__ segs,
+                                                           int nseg) {
+  const int64_t b = blockIdx.x;
+  const int     s = find_seg(segs, nseg, b);
+  const FieldSeg g = segs[s];
+  const int64_t lb = b - g.blk0;
+  const int64_t per_z = static_cast<int64_t>(g.gx) * g.gy;
+  const int bz = static_cast<int>(lb / per_z);
+  const int rr = static_cast<int>(lb - bz * per_z);
+  const int by = rr / g.gx, bx = rr - by * g.gx;
+  const int a = bx * BX + threadIdx.x, bb = by * BY + threadIdx.y;
+  const FieldView& F = g.F;
+  if (a >= F.n1 || bb >= F.n2) return;
+  const int i = a + F.ngh, j = bb + F.ngh, k = (F.N3 > 1) ? bz + F.ngh : 0;
+  if constexpr (OP == SEG_FARADAY) faraday_periodic_cell(F, i, j, k, g.c1, g.c2);
+  else if constexpr (OP == SEG_AMPERE)
+    ampere_currents_periodic_cell(F, g.J, i, j, k, g.c1, g.c2, g.coeff, g.ppc0, g.with_j);
+  else if constexpr (OP == SEG_FILTER) filter_periodic_cell(F, g.J, i, j, k);
+  else j_from_fixed_cell(F, g.acc, g.inv_scale, i, j, k);
+}
+
+constexpr int kFlatBlock = 256, kFlatPerThread = 8;
+
+template <int OP>
+__global__ __launch_bounds__(kFlatBlock) void k_flat_batched(const FlatSeg* __restrict__ segs,
+                                                             int nseg) {
+  const int64_t b = blockIdx.x;
+  const int     s = find_seg(segs, nseg, b);
+  const FlatSeg g = segs[s];
+  const int64_t base = (b - g.blk0) * (kFlatBlock * kFlatPerThread) + threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < kFlatPerThread; ++r) {
+    const int64_t e = base + static_cast<int64_t>(r) * kFlatBlock;
+    if (e >= g.n) break;
+    if constexpr (OP == FLAT_ZERO_U64) static_cast<unsigned long long*>(g.dst)[e] = 0ull;
+    else static_cast<float*>(g.dst)[e] = static_cast<const float*>(g.src)[e];
+  }
 }
 
 ntt_status launch_ok() { return check_launch(); }
@@ -558,6 +633,36 @@ void launch_zero_u64(unsigned long long* d, int64_t n, hipStream_t st) {
 void launch_j_from_fixed(const FieldView& J, const unsigned long long* acc, double inv_scale,
                          hipStream_t st) {
   k_j_from_fixed<<<active_grid(J), dim3(BX, BY), 0, st>>>(J, acc, inv_scale);
+}
+
+int64_t field_seg_blocks(const FieldView& F, int32_t* gx, int32_t* gy) {
+  const dim3 g = active_grid(F);
+  *gx = static_cast<int32_t>(g.x);
+  *gy = static_cast<int32_t>(g.y);
+  return static_cast<int64_t>(g.x) * g.y * g.z;
+}
+
+void launch_fields_batched(int op, const FieldSeg* segs, int nseg, int64_t nblocks,
+                           hipStream_t st) {
+  if (nseg == 0 || nblocks == 0) return;
+  const dim3 grid(static_cast<unsigned>(nblocks)), block(BX, BY);
+  if (op == SEG_FARADAY) k_fields_batched<SEG_FARADAY><<<grid, block, 0, st>>>(segs, nseg);
+  else if (op == SEG_AMPERE) k_fields_batched<SEG_AMPERE><<<grid, block, 0, st>>>(segs, nseg);
+  else if (op == SEG_FILTER) k_fields_batched<SEG_FILTER><<<grid, block, 0, st>>>(segs, nseg);
+  else k_fields_batched<SEG_JFIX><<<grid, block, 0, st>>>(segs, nseg);
+}
+
+int64_t flat_seg_blocks(int64_t n) {
+  const int64_t per = static_cast<int64_t>(kFlatBlock) * kFlatPerThread;
+  return (n + per - 1) / per;
+}
+
+void launch_flat_batched(int op, const FlatSeg* segs, int nseg, int64_t nblocks, hipStream_t st) {
+  if (nseg == 0 || nblocks == 0) return;
+  if (op == FLAT_ZERO_U64)
+    k_flat_batched<FLAT_ZERO_U64><<<static_cast<unsigned>(nblocks), kFlatBlock, 0, st>>>(segs, nseg);
+  else
+    k_flat_batched<FLAT_COPY_F32><<<static_cast<unsigned>(nblocks), kFlatBlock, 0, st>>>(segs, nseg);
 }
 
 }  // namespace ntt

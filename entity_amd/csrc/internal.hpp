@@ -23,6 +23,16 @@ ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView&
                                         bool deposit, float fscale,
                                         const uint32_t* tile_offsets,
                                         int64_t ntiles, int ppt, hipStream_t st);
+// the same, as a kernel argument block written to host memory (for the
+// batched launch over the boxes of an ensemble)
+size_t     fused_args_size();
+ntt_status build_fused_args(void* out, int64_t* nchunks, const ParticleView& pv,
+                            const FieldView& em, const FieldView& cur, unsigned long long* acc,
+                            int D, float ndth, float dt, float dxm, float charge, bool deposit,
+                            float fscale, const uint32_t* tile_offsets, int64_t ntiles, int ppt);
+ntt_status launch_push_deposit_batched(int D, int order, int alg, const void* table,
+                                       const int64_t* chunk0, int nent, int64_t nchunks,
+                                       hipStream_t st);
 // fixed-point scale 2^k of the current accumulator: the largest power of two
 // for which a particle with |charge * weight| <= qw_max keeps every deposit
 // contribution below 2^30 units
@@ -42,5 +52,32 @@ void launch_zero_u64(unsigned long long* d, int64_t n, hipStream_t st);
 // J (active cells and their periodic ghost copies) = acc * inv_scale
 void launch_j_from_fixed(const FieldView& J, const unsigned long long* acc, double inv_scale,
                          hipStream_t st);
+
+// ---- batched field launches over the boxes of an ensemble (fields.hip) ----
+struct FieldSeg {
+  FieldView                 F;          // em (Faraday, Ampere), dst (filter), cur (J from acc)
+  FieldView                 J;          // cur (Ampere), src (filter)
+  const unsigned long long* acc;        // J from the fixed-point accumulator
+  double                    inv_scale;
+  float                     c1, c2, coeff, ppc0;
+  int32_t                   with_j;
+  int32_t                   gx, gy;     // blocks of this box's active grid in x, y
+  int64_t                   blk0;       // first block of this box in the launch
+};
+enum { SEG_FARADAY = 0, SEG_AMPERE = 1, SEG_FILTER = 2, SEG_JFIX = 3 };
+// blocks a box needs in a batched field launch
+int64_t field_seg_blocks(const FieldView& F, int32_t* gx, int32_t* gy);
+void    launch_fields_batched(int op, const FieldSeg* segs, int nseg, int64_t nblocks,
+                              hipStream_t st);
+struct FlatSeg {
+  void*       dst;
+  const void* src;
+  int64_t     n;      // elements
+  int64_t     blk0;
+};
+enum { FLAT_ZERO_U64 = 0, FLAT_COPY_F32 = 1 };
+int64_t flat_seg_blocks(int64_t n);
+void    launch_flat_batched(int op, const FlatSeg* segs, int nseg, int64_t nblocks,
+                            hipStream_t st);
 
 }  // namespace ntt

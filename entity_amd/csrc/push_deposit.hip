@@ -33,6 +33,7 @@
 // Reference: src/kernels/pushers/sr.hpp:117-814, src/kernels/currents_deposit.hpp:108-761.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "sr_particle.hpp"
 
@@ -159,7 +160,7 @@ struct WrapJ {
   __device__ __forceinline__ void add_fixed(int i, int j, int k, int c, unsigned long long v) {
     atomicAdd(d + index(i, j, k, c), v);
   }
-  __device__ __forceinline__ void add_fixed_at(int64_t idx, unsigned long long v) {
+  __device__ __forceinline__ void add_fixed_at(int64_t idx, unsigned long long v) const {
     atomicAdd(d + idx, v);
   }
 };
@@ -242,9 +243,22 @@ __device__ __forceinline__ int wrap_any(int x, int n) {
   return x;
 }
 
+// XCD-aware chunk order: blockIdx.x % 8 is the XCD a workgroup runs on; each
+// XCD gets a contiguous run of chunks, whose windows overlap in its L2.
+// Returns -1 for the padding blocks of the last run.
+__device__ __forceinline__ int64_t xcd_chunk(int64_t nb) {
+  const int64_t bid  = blockIdx.x;
+  const int64_t per  = (nb + 7) / 8;
+  const int64_t xcd  = bid % 8;
+  const int64_t slot = bid / 8;
+  const int64_t full = nb - 8 * (per - 1);  // XCDs owning `per` chunks
+  if (slot >= ((xcd < full) ? per : per - 1)) return -1;
+  return (xcd < full) ? xcd * per + slot : full * per + (xcd - full) * (per - 1) + slot;
+}
+
+// Push and deposit of one chunk (one workgroup).
 template <int D, int O, int ALG>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NTT_PD_WAVES_PER_EU)))
-void k_push_deposit(FusedArgs a) {
+__device__ __forceinline__ void push_deposit_chunk(const FusedArgs& a, int64_t chunk) {
   constexpr int W1 = Win<D>::W1, W2 = Win<D>::W2, W3 = Win<D>::W3;
   constexpr int WC = W1 * W2 * W3;
   constexpr int WM = (W1 > W2 ? W1 : W2) > W3 ? (W1 > W2 ? W1 : W2) : W3;
@@ -254,18 +268,6 @@ void k_push_deposit(FusedArgs a) {
   // ghost-extended index (em) and active index (current) of every window
   // row/column, per dimension, after the periodic wrap
   __shared__ int gtab[3][WM];
-
-  // XCD-aware chunk mapping: blockIdx.x % 8 is the XCD; give each XCD a
-  // contiguous run of chunks
-  const int64_t nb   = a.nchunks;
-  const int64_t bid  = blockIdx.x;
-  const int64_t per  = (nb + 7) / 8;
-  const int64_t xcd  = bid % 8;
-  const int64_t slot = bid / 8;
-  const int64_t full = nb - 8 * (per - 1);  // XCDs owning `per` chunks
-  const int64_t chunk =
-    (xcd < full) ? xcd * per + slot : full * per + (xcd - full) * (per - 1) + slot;
-  if (slot >= ((xcd < full) ? per : per - 1)) return;
 
   const int tid    = threadIdx.x;
   const int G      = a.ngh;
@@ -484,6 +486,32 @@ void k_push_deposit(FusedArgs a) {
 }
 
 template <int D, int O, int ALG>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NTT_PD_WAVES_PER_EU)))
+void k_push_deposit(FusedArgs a) {
+  const int64_t chunk = xcd_chunk(a.nchunks);
+  if (chunk < 0) return;
+  push_deposit_chunk<D, O, ALG>(a, chunk);
+}
+
+// The same over the boxes of an ensemble: entry e owns the global chunks
+// [chunk0[e], chunk0[e + 1]).
+template <int D, int O, int ALG>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NTT_PD_WAVES_PER_EU)))
+void k_push_deposit_batched(const FusedArgs* __restrict__ table,
+                            const int64_t* __restrict__ chunk0, int nent, int64_t nchunks) {
+  const int64_t g = xcd_chunk(nchunks);
+  if (g < 0) return;
+  int lo = 0, hi = nent - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (chunk0[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  const FusedArgs a = table[lo];
+  push_deposit_chunk<D, O, ALG>(a, g - chunk0[lo]);
+}
+
+template <int D, int O, int ALG>
 void launch_fused(const FusedArgs& a, hipStream_t st) {
   if (a.nchunks == 0) return;
   k_push_deposit<D, O, ALG><<<static_cast<unsigned>(a.nchunks), kThreads, 0, st>>>(a);
@@ -516,18 +544,15 @@ float fixed_point_scale(float qw_max, float dt) {
   return std::ldexp(1.0f, k);
 }
 
-// Fused push + deposit of one species over a periodic box.  em must have its
-// periodic ghosts filled; the folded current is added, as 64-bit fixed point
-// in units of 1/fscale, to the active cells of `acc`, an integer array with
-// the layout of the current field `cur` (the caller zeroes it first).
-ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView& em,
-                                        const FieldView& cur, unsigned long long* acc,
-                                        int D, int order, int alg,
-                                        float ndth, float dt, float dxm, float charge,
-                                        bool deposit, float fscale,
-                                        const uint32_t* tile_offsets,
-                                        int64_t ntiles, int ppt, hipStream_t st) {
-  if (pv.n == 0) return NTT_OK;
+size_t fused_args_size() { return sizeof(FusedArgs); }
+
+// Arguments of the fused push + deposit of one species over a periodic box
+// (the kernel argument block, written to `out`).
+ntt_status build_fused_args(void* out, int64_t* nchunks, const ParticleView& pv,
+                            const FieldView& em, const FieldView& cur, unsigned long long* acc,
+                            int D, float ndth, float dt, float dxm, float charge, bool deposit,
+                            float fscale, const uint32_t* tile_offsets, int64_t ntiles,
+                            int ppt) {
   uint32_t* err = device_error_word();
   if (err == nullptr) return NTT_ERR_HIP;
   FusedArgs a;
@@ -549,13 +574,37 @@ ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView&
   a.err = err;
   a.toff = tile_offsets;
   a.ppt = ppt < 1 ? 1 : ppt;
-  if (tile_offsets != nullptr) {
+  if (pv.n == 0) {
+    a.nchunks = 0;
+  } else if (tile_offsets != nullptr) {
     a.nchunks = ntiles;
   } else {
     const int64_t per = static_cast<int64_t>(kThreads) * a.ppt;
     a.nchunks = (pv.n + per - 1) / per;
   }
   if (a.nchunks > 0x7FFFFFFF) return NTT_ERR_UNSUPPORTED;
+  *nchunks = a.nchunks;
+  std::memcpy(out, &a, sizeof(a));
+  return NTT_OK;
+}
+
+// Fused push + deposit of one species over a periodic box.  em must have its
+// periodic ghosts filled; the folded current is added, as 64-bit fixed point
+// in units of 1/fscale, to the active cells of `acc`, an integer array with
+// the layout of the current field `cur` (the caller zeroes it first).
+ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView& em,
+                                        const FieldView& cur, unsigned long long* acc,
+                                        int D, int order, int alg,
+                                        float ndth, float dt, float dxm, float charge,
+                                        bool deposit, float fscale,
+                                        const uint32_t* tile_offsets,
+                                        int64_t ntiles, int ppt, hipStream_t st) {
+  if (pv.n == 0) return NTT_OK;
+  FusedArgs a;
+  int64_t   nch = 0;
+  ntt_status s = build_fused_args(&a, &nch, pv, em, cur, acc, D, ndth, dt, dxm, charge, deposit,
+                                  fscale, tile_offsets, ntiles, ppt);
+  if (s != NTT_OK) return s;
   if (D == 2) {
     if (alg == ALG_BORIS) launch_fused_o<2, ALG_BORIS>(order, a, st);
     else if (alg == ALG_VAY) launch_fused_o<2, ALG_VAY>(order, a, st);
@@ -564,6 +613,41 @@ ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView&
     if (alg == ALG_BORIS) launch_fused_o<3, ALG_BORIS>(order, a, st);
     else if (alg == ALG_VAY) launch_fused_o<3, ALG_VAY>(order, a, st);
     else launch_fused<3, 0, ALG_PHOTON>(a, st);
+  }
+  return check_launch();
+}
+
+namespace {
+template <int D, int O, int ALG>
+void launch_batched_t(const void* table, const int64_t* chunk0, int nent, int64_t nchunks,
+                      hipStream_t st) {
+  k_push_deposit_batched<D, O, ALG><<<static_cast<unsigned>(nchunks), kThreads, 0, st>>>(
+    static_cast<const FusedArgs*>(table), chunk0, nent, nchunks);
+}
+template <int D, int ALG>
+void launch_batched_o(int order, const void* table, const int64_t* chunk0, int nent,
+                      int64_t nchunks, hipStream_t st) {
+  if (order == 0) launch_batched_t<D, 0, ALG>(table, chunk0, nent, nchunks, st);
+  else if (order == 1) launch_batched_t<D, 1, ALG>(table, chunk0, nent, nchunks, st);
+  else launch_batched_t<D, 2, ALG>(table, chunk0, nent, nchunks, st);
+}
+}  // namespace
+
+// One launch over the table entries (all of one dimension, shape order and
+// pusher algorithm); entry e owns global chunks [chunk0[e], chunk0[e + 1]).
+ntt_status launch_push_deposit_batched(int D, int order, int alg, const void* table,
+                                       const int64_t* chunk0, int nent, int64_t nchunks,
+                                       hipStream_t st) {
+  if (nent == 0 || nchunks == 0) return NTT_OK;
+  if (nchunks > 0x7FFFFFFF) return NTT_ERR_UNSUPPORTED;
+  if (D == 2) {
+    if (alg == ALG_BORIS) launch_batched_o<2, ALG_BORIS>(order, table, chunk0, nent, nchunks, st);
+    else if (alg == ALG_VAY) launch_batched_o<2, ALG_VAY>(order, table, chunk0, nent, nchunks, st);
+    else launch_batched_t<2, 0, ALG_PHOTON>(table, chunk0, nent, nchunks, st);
+  } else {
+    if (alg == ALG_BORIS) launch_batched_o<3, ALG_BORIS>(order, table, chunk0, nent, nchunks, st);
+    else if (alg == ALG_VAY) launch_batched_o<3, ALG_VAY>(order, table, chunk0, nent, nchunks, st);
+    else launch_batched_t<3, 0, ALG_PHOTON>(table, chunk0, nent, nchunks, st);
   }
   return check_launch();
 }

@@ -1,0 +1,104 @@
+"""Batched ensembles of independent periodic SR boxes on one GPU.
+
+``SREnsemble`` advances many ``SRPICEngine`` boxes (any sizes, shape orders,
+pushers, filter counts and particle loads) together: each phase of the step
+is one launch over all boxes (``ntt_ensemble_step``), and a step can be
+captured once in a HIP graph and replayed.  Every box's fields and particles
+equal those of ``SRPICEngine.step_forward`` on that box alone, bit for bit.
+The reference runs such parameter scans as separate simulations.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _abi
+from .engine import SRPICEngine, _stream
+
+
+class SREnsemble:
+    def __init__(self, engines: list[SRPICEngine]):
+        if not engines:
+            raise ValueError("empty ensemble")
+        steps = {e.step for e in engines}
+        if len(steps) != 1:
+            raise ValueError("all boxes must be at the same step")
+        self.engines = list(engines)
+        self.lib = engines[0].lib
+        self.step = engines[0].step
+        self._graph = None
+        self._plan = C.c_void_p(None)
+        self._build()
+
+    def _build(self) -> None:
+        boxes = (_abi.NttBox * len(self.engines))()
+        keep = []
+        for b, eng in zip(boxes, self.engines):
+            if eng.params.sort_interval and any(t is None for t in eng.tile_offsets):
+                eng.sort()      # tile offsets the plan can point at
+            sp = eng.species_abi()
+            keep.append(sp)
+            b.em, b.cur, b.buff = (eng.fields.abi(x) for x in ("em", "cur", "buff"))
+            b.species = C.cast(sp, C.POINTER(_abi.NttSpecies))
+            b.nspecies = len(eng.species)
+            b.prm = eng.step_params()
+            b.metric = _abi.NttMinkowski(eng.params.dim, eng.params.dx)
+            b.workspace = eng._step_ws.data_ptr()
+            b.workspace_bytes = eng._step_ws.numel()
+        if self._plan.value:
+            self.lib.ntt_ensemble_destroy(self._plan)
+        plan = C.c_void_p(None)
+        _abi.check(self.lib.ntt_ensemble_create(boxes, len(self.engines), C.byref(plan)),
+                   "ntt_ensemble_create")
+        self._plan = plan
+        self._graph = None
+
+    def __del__(self):
+        try:
+            if self._plan.value:
+                self.lib.ntt_ensemble_destroy(self._plan)
+        except Exception:
+            pass
+
+    def _sort_due(self) -> None:
+        for eng in self.engines:
+            si = eng.params.sort_interval
+            if si and self.step > 0 and self.step % si == 0:
+                eng.sort()
+
+    def _advance_counters(self) -> None:
+        self.step += 1
+        for eng in self.engines:
+            eng.step += 1
+            eng.time += eng.params.dt
+
+    def step_forward(self, stream=None) -> None:
+        """One step of every box (re-sorts first where a box's interval is due)."""
+        self._sort_due()
+        _abi.check(self.lib.ntt_ensemble_step(self._plan, self.step, _stream(stream)),
+                   "ntt_ensemble_step")
+        self._advance_counters()
+
+    def capture(self) -> None:
+        """Records one ensemble step (step > 0) in a HIP graph; step_graph()
+        then replays it.  Re-sorts stay outside the graph."""
+        if self.step == 0:
+            raise ValueError("capture after the first step (step 0 also fills ghosts)")
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                _abi.check(self.lib.ntt_ensemble_step(self._plan, self.step, s.cuda_stream),
+                           "ntt_ensemble_step")
+        torch.cuda.current_stream().wait_stream(s)
+        self._graph = g
+
+    def step_graph(self) -> None:
+        """One step by replaying the captured graph."""
+        if self._graph is None:
+            raise RuntimeError("capture() first")
+        self._sort_due()
+        self._graph.replay()
+        self._advance_counters()

@@ -350,6 +350,40 @@ NTT_API ntt_status ntt_sr_step(ntt_field_t*                em,
 /* bytes of the workspace ntt_sr_step needs for a current field of this shape */
 NTT_API ntt_status ntt_sr_step_workspace_size(const ntt_field_t* cur, uint64_t* bytes);
 
+/* ------------------------------------------------------- batched ensembles */
+
+/* One box of an ensemble: the arguments ntt_sr_step takes for that box. */
+typedef struct ntt_box {
+  ntt_field_t          em, cur, buff;
+  ntt_species_t*       species;          /* host array of nspecies            */
+  int32_t              nspecies;
+  ntt_sr_step_params_t prm;              /* prm.step is ignored (see step)    */
+  ntt_minkowski_t      metric;
+  void*                workspace;        /* ntt_sr_step_workspace_size bytes  */
+  uint64_t             workspace_bytes;
+} ntt_box_t;
+
+typedef struct ntt_ensemble* ntt_ensemble_t;
+
+/* Plans the batched step of independent periodic boxes of any sizes, shape
+ * orders, pushers and filter counts (parameter scans).  The plan records the
+ * device pointers of every box (fields, particle arrays, tile offsets,
+ * workspaces), which must stay valid and in place while it is used; particle
+ * counts are those at creation.  Allocates device memory for its tables.
+ * No reference counterpart: the reference runs one simulation per process. */
+NTT_API ntt_status ntt_ensemble_create(const ntt_box_t* boxes, int32_t nboxes,
+                                       ntt_ensemble_t* ens);
+
+/* One SRPIC step of every box of the plan: each phase of ntt_sr_step is one
+ * launch over all boxes (the push + deposit one launch per dimension, shape
+ * order and pusher), so every box's fields and particles are bit-identical
+ * to ntt_sr_step on that box alone.  step == 0 fills the E/B ghosts first.
+ * Enqueues kernels only (no allocation, copy or synchronisation), so the
+ * call can be captured in a HIP graph. */
+NTT_API ntt_status ntt_ensemble_step(ntt_ensemble_t ens, int64_t step, void* stream);
+
+NTT_API ntt_status ntt_ensemble_destroy(ntt_ensemble_t ens);
+
 /* Launch timing of the fused push+deposit kernels that ntt_sr_step enqueues:
  * while enabled, every such launch is bracketed by HIP events recorded on its
  * stream; ntt_profile_read waits for them and returns the summed elapsed
