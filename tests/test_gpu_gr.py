@@ -7,7 +7,7 @@ reflection) and dead particles; every metric of the family and every shape
 order, photons and massive particles.
 
 * precision 64 (FP64 metric and pusher on FP32 storage) against the FP64
-  oracle: momenta within 2e-6 relative, positions within 2e-6 cells, cell
+  oracle: momenta within 2e-6 relative, positions within two float ulps, cell
   indices and tags identical except for particles that end within 4 ulp of a
   cell face (counted).  The device evaluates sin/cos/exp with its own math
   library and shares subexpressions the reference recomputes, so agreement is
@@ -52,13 +52,19 @@ def _fields(seed, scale):
     return em, em0
 
 
-def _particles(seed, npart=4096, umax=1.0):
+def _particles(seed, npart=4096, umax=1.0, near_axis=True):
+    """near_axis: a block of particles in the first and last theta cells
+    (with no angular momentum).  Only for photons or zero fields: an EM kick
+    gives u_phi, and the 1/sin^3 centrifugal terms there make the step
+    ill-conditioned (round-off amplified by ~1e9)."""
     rng = np.random.default_rng(seed)
     i1 = rng.integers(0, N[0], npart).astype(np.int32)
-    i2 = rng.integers(0, N[1], npart).astype(np.int32)
+    i2 = rng.integers(0 if near_axis else 2, N[1] if near_axis else N[1] - 2,
+                      npart).astype(np.int32)
     k = npart // 8
     i1[:k] = rng.choice([0, N[0] - 1], k)          # absorbing r edges
-    i2[k:2 * k] = rng.choice([0, N[1] - 1], k)     # next to the axis
+    if near_axis:
+        i2[k:2 * k] = rng.choice([0, N[1] - 1], k)  # next to the axis
     p = {"i1": i1, "i2": i2,
          "dx1": rng.random(npart, dtype=np.float32), "dx2": rng.random(npart, dtype=np.float32),
          "ux1": (rng.standard_normal(npart) * umax).astype(np.float32),
@@ -149,14 +155,16 @@ def _compare(got, ref, rtol, xtol, max_flag=16):
 def test_gr_push_fp64(hip, oracle, metric, order, massless):
     if massless and order > 0:
         pytest.skip("photons gather no fields: one shape order covers them")
-    p = _particles(10 + order)
+    p = _particles(10 + order, near_axis=massless)
     em, em0 = _fields(20 + order, 0.5)
     dt = 0.05
     got = _run_gpu(hip, metric, p, None if massless else em, None if massless else em0,
                    massless=massless, order=order, precision=64, dt=dt)
     ref = _run_oracle(oracle, metric, p, None if massless else em, None if massless else em0,
                       massless=massless, order=order, prec="f64", dt=dt)
-    _compare(got, ref, rtol=2e-6, xtol=2e-6)
+    # positions: dx = float(X) - float(i) carries one float ulp of X (3.8e-6
+    # for X in [32, 64)) whenever the double X rounds differently
+    _compare(got, ref, rtol=2e-6, xtol=8e-6)
     # the boundaries were exercised
     assert (np.asarray(ref["tag"]) == 0).sum() > (np.asarray(p["tag"]) == 0).sum()
 
@@ -167,7 +175,7 @@ def test_gr_push_fp32(hip, oracle, metric, massless):
     """FP32 arithmetic is not reproducible bit for bit across math libraries;
     the device's FP32 result must be as close to the FP64 oracle as the FP32
     oracle is (within 4x, plus 1e-5 of the value)."""
-    p = _particles(30)
+    p = _particles(30, near_axis=massless)
     em, em0 = _fields(31, 0.5)
     dt = 0.05
     got = _run_gpu(hip, metric, p, em, em0, massless=massless, order=1, precision=32, dt=dt)
@@ -183,6 +191,17 @@ def test_gr_push_fp32(hip, oracle, metric, massless):
         e_dev = np.abs(np.asarray(got[k], np.float64)[alive] - ref)
         e_orc = np.abs(np.asarray(r32[k], np.float64)[alive] - ref)
         assert np.all(e_dev <= 4 * e_orc + 1e-5 * np.maximum(np.abs(ref), 1.0)), k
+
+
+@pytest.mark.parametrize("metric", list(METRICS))
+def test_gr_axis_massive_zero_fields(hip, oracle, metric):
+    """Massive particles next to the axis (theta clamp of the momentum push
+    and of the coordinate midpoints, axis reflection) with the fields off."""
+    p = _particles(60, near_axis=True)
+    em = np.zeros((6, N[1] + 2 * G, N[0] + 2 * G), np.float32)
+    got = _run_gpu(hip, metric, p, em, em, massless=False, order=1, precision=64, dt=0.05)
+    ref = _run_oracle(oracle, metric, p, em, em, massless=False, order=1, prec="f64", dt=0.05)
+    _compare(got, ref, rtol=2e-6, xtol=8e-6)
 
 
 @pytest.mark.parametrize("metric", ["ks", "qks"])
