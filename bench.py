@@ -49,6 +49,10 @@ def parse():
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-cells", type=str, default="64,64,64")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--batched", type=int, default=1, help="also time the 64-box ensemble")
+    ap.add_argument("--batched-steps", type=int, default=12)
+    ap.add_argument("--gr", type=int, default=1, help="also time the GR pusher runs")
+    ap.add_argument("--gr-steps", type=int, default=5)
     return ap.parse_args()
 
 
@@ -147,6 +151,130 @@ def cpu_baseline(args, cells, order, pusher_flag, nfilt):
                       f"oracle C++ restatement in FP32, one core, {el:.1f} s"}
 
 
+def batched_leg(args):
+    """Ensemble of 64 independent 2D periodic boxes (parameter scan): sizes
+    from {64^2, 128^2, 256^2, 512^2}, 8-32 ppc per species, theta log-uniform
+    in [0.01, 1], counter-drift beta in [0, 0.9], shape order 1 or 2, Boris,
+    1 filter pass, re-sort every 4 steps.  Times the plain per-box loop
+    (SRPICEngine.step_forward box after box) and the batched step replayed
+    from a HIP graph, on the same boxes."""
+    import numpy as np
+    import torch
+    from entity_amd import _abi
+    from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
+    from entity_amd.ensemble import SREnsemble
+    rng = np.random.default_rng(777)
+    engines = []
+    total = 0
+    for b in range(64):
+        n = int(rng.choice([64, 128, 256, 512]))
+        ppc = int(rng.integers(8, 33))
+        theta = float(10 ** rng.uniform(-2, 0))
+        beta = float(rng.uniform(0, 0.9))
+        ud = beta / np.sqrt(1 - beta * beta)
+        order = int(rng.choice([1, 2]))
+        prm = SRParams(dim=2, n=(n, n), shape_order=order, n_filters=1, ppc0=2.0 * ppc,
+                       tile=(8, 8), larmor0=4.0, skindepth0=4.0, sort_interval=4)
+        eng = SRPICEngine(prm, [SpeciesSpec("e-", 1.0, -1.0, _abi.PUSHER_BORIS),
+                                SpeciesSpec("e+", 1.0, 1.0, _abi.PUSHER_BORIS)], n * n * ppc)
+        eng.inject_uniform(ppc, temperature=theta, drifts=[(ud, 0, 0), (-ud, 0, 0)],
+                           seed=1000 + b)
+        engines.append(eng)
+        total += 2 * n * n * ppc
+    for eng in engines:
+        eng.step_forward()
+    torch.cuda.synchronize()
+    K = args.batched_steps
+    t0 = time.perf_counter()
+    for _ in range(K):
+        for eng in engines:
+            eng.step_forward()
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / K
+    ens = SREnsemble(engines)
+    ens.step_forward()
+    ens.capture()
+    ens.step_graph()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        ens.step_graph()
+    torch.cuda.synchronize()
+    t_batched = (time.perf_counter() - t0) / K
+    return {"workload": "64 independent 2D periodic boxes (64^2..512^2, 8-32 ppc/species, "
+                        "theta 0.01-1, drift beta 0-0.9, order 1/2, Boris, 1 filter pass, "
+                        "re-sort every 4 steps)",
+            "particles": total, "steps": K,
+            "value": total / t_batched, "unit": "particle-steps/s",
+            "plain_loop_value": total / t_plain,
+            "ms_per_step": 1e3 * t_batched, "plain_loop_ms_per_step": 1e3 * t_plain,
+            "speedup_vs_plain_loop": t_plain / t_batched}
+
+
+def gr_leg(args):
+    """GR pusher: 16 runs on 768 x 768 quasi-spherical Kerr-Schild grids
+    (r in [0.9 r_h, 50], theta in [0, pi], h = 0), spins linspace(0, 0.99, 16),
+    e-/e+ at 8 ppc per species, theta = 0.1 covariant thermal momenta,
+    synthetic static D and B, FP32 storage with FP64 metric and pusher, first
+    order shape, 10 iterations, absorbing r boundaries and axis reflection."""
+    import numpy as np
+    import torch
+    from entity_amd import _abi
+    from entity_amd.containers import GridShape, Particles
+    from entity_amd.kernels import PusherBoundaries, gr
+    N = (768, 768)
+    ppc = 8
+    dev = torch.device("cuda")
+    shape = GridShape(2, N)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(4242)
+    runs = []
+    npart = N[0] * N[1] * ppc
+    ext = [(e + 2 * 2) for e in N]
+    for r, a in enumerate(np.linspace(0.0, 0.99, 16)):
+        rh = 1 + np.sqrt(1 - a * a)
+        met = gr.KerrSchild(_abi.METRIC_QKERR_SCHILD, N, (0.9 * rh, 50.0), (0.0, np.pi), a=float(a))
+        em = (torch.randn((6, ext[1], ext[0]), device=dev, generator=gen) * 0.01)
+        em0 = (torch.randn((6, ext[1], ext[0]), device=dev, generator=gen) * 0.01)
+        sps = []
+        for q in (-1.0, 1.0):
+            P = Particles(2, npart, device=dev, with_phi=True)
+            cell = torch.arange(npart, device=dev) // ppc
+            P.load(i1=(cell % N[0]).to(torch.int32), i2=(cell // N[0]).to(torch.int32),
+                   dx1=torch.rand(npart, device=dev, generator=gen),
+                   dx2=torch.rand(npart, device=dev, generator=gen),
+                   ux1=torch.randn(npart, device=dev, generator=gen) * 0.3,
+                   ux2=torch.randn(npart, device=dev, generator=gen) * 0.3,
+                   ux3=torch.randn(npart, device=dev, generator=gen) * 0.3,
+                   weight=torch.ones(npart, device=dev), tag=torch.ones(npart, dtype=torch.int16,
+                                                                        device=dev))
+            ctx = gr.PusherContext(_abi.PUSHER_BORIS, 1.0, q, 0.02, 1.0, N[0], N[1])
+            bc = PusherBoundaries([(_abi.PRTL_BC_ABSORB, _abi.PRTL_BC_ABSORB),
+                                   (_abi.PRTL_BC_AXIS, _abi.PRTL_BC_AXIS), (0, 0)])
+            sps.append(gr.Pusher_kernel(ctx, bc, P, em, em0, shape, met, 1))
+        runs.append(sps)
+    for sps in runs:
+        for k in sps:
+            k()
+    torch.cuda.synchronize()
+    K = args.gr_steps
+    s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s0.record()
+    for _ in range(K):
+        for sps in runs:
+            for k in sps:
+                k()
+    s1.record()
+    torch.cuda.synchronize()
+    ms = s0.elapsed_time(s1) / K
+    total = 16 * 2 * npart
+    return {"workload": "16 QKS runs 768x768, spins 0..0.99, e-/e+ 8 ppc/species "
+                        "(151M particles), FP32 storage / FP64 metric, 10 iterations",
+            "particles": total, "steps": K, "value": total / (ms * 1e-3),
+            "unit": "particle-steps/s", "ms_per_step": ms,
+            "hbm_gbs": total * 78 / (ms * 1e-3) / 1e9}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -242,6 +370,10 @@ def main():
         },
         "device_errors": errs,
     }
+    if args.batched:
+        out["batched"] = batched_leg(args)
+    if args.gr:
+        out["gr"] = gr_leg(args)
     if args.cpu_baseline:
         ccells = tuple(int(x) for x in args.cpu_cells.split(","))
         out["cpu_baseline"] = cpu_baseline(args, ccells, args.order, pusher, args.filters)

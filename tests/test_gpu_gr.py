@@ -108,9 +108,11 @@ def _run_gpu(hip, metric, p, em, em0, *, massless, order, precision, dt, nsteps=
 
 def _run_oracle(oracle, metric, p, em, em0, *, massless, order, prec, dt, nsteps=1):
     m = METRICS[metric]
+    # dt and omegaB0 reach the device as float (ntt_gr_pusher_ctx_t)
     prm = oracle.gr_params({0: "kerr_schild", 1: "qkerr_schild", 2: "kerr_schild_0"}[m["kind"]],
-                           N, _extent(m), a=m["a"], r0=m["r0"], h=m["h"], dt=dt, mass=1.0,
-                           charge=-1.0, omegaB0=0.7, niter=10, massless=massless,
+                           N, _extent(m), a=m["a"], r0=m["r0"], h=m["h"],
+                           dt=float(np.float32(dt)), mass=1.0, charge=-1.0,
+                           omegaB0=float(np.float32(0.7)), niter=10, massless=massless,
                            absorb=(True, True), axis=(True, True))
     q = oracle.normalize_particles(p, prec)
     real = np.float64 if prec == "f64" else np.float32
@@ -126,7 +128,7 @@ def near_face(dx, ulps=4):
     return (dx < eps) | (dx > np.float32(1.0) - eps)
 
 
-def _compare(got, ref, rtol, xtol, max_flag=16):
+def _compare(got, ref, rtol, xtol, max_flag=16, inp=None):
     flagged = np.zeros(len(ref["tag"]), bool)
     for d in (1, 2):
         flagged |= near_face(ref[f"dx{d}"]) | near_face(got[f"dx{d}"])
@@ -146,7 +148,13 @@ def _compare(got, ref, rtol, xtol, max_flag=16):
         g = np.asarray(got[k], np.float64)[alive]
         r = np.asarray(ref[k], np.float64)[alive]
         scale = np.maximum(np.abs(r), 1e-3)
-        assert np.all(np.abs(g - r) <= rtol * scale), (k, np.abs(g - r).max())
+        bad = np.abs(g - r) > rtol * scale
+        if bad.any():
+            j = np.nonzero(alive)[0][np.argmax(np.abs(g - r) / scale)]
+            info = {q: (np.asarray(inp[q])[j] if inp is not None else None)
+                    for q in ("i1", "dx1", "i2", "dx2", "ux1", "ux2", "ux3")}
+            raise AssertionError(f"{k}: particle {j} got {np.asarray(got[k])[j]!r} "
+                                 f"ref {np.asarray(ref[k])[j]!r} input {info}")
 
 
 @pytest.mark.parametrize("metric", list(METRICS))
@@ -164,7 +172,7 @@ def test_gr_push_fp64(hip, oracle, metric, order, massless):
                       massless=massless, order=order, prec="f64", dt=dt)
     # positions: dx = float(X) - float(i) carries one float ulp of X (3.8e-6
     # for X in [32, 64)) whenever the double X rounds differently
-    _compare(got, ref, rtol=2e-6, xtol=8e-6)
+    _compare(got, ref, rtol=2e-6, xtol=8e-6, inp=p)
     # the boundaries were exercised
     assert (np.asarray(ref["tag"]) == 0).sum() > (np.asarray(p["tag"]) == 0).sum()
 
