@@ -104,6 +104,23 @@ def init_fields(eng, seed=12345, nmodes=16, kmax=4, dB=0.5, B0=1.0):
     torch.cuda.synchronize()
 
 
+def measured_traffic(cells, ppc):
+    """HBM bytes per fused launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (None for another workload)."""
+    path = ROOT / "profiles" / "k_push_deposit_3d_o2_vay_512x256x256_traffic.md"
+    want = f"workload: 3D periodic {'x'.join(map(str, cells))} cells, e-/e+ {ppc} ppc per species"
+    try:
+        text = path.read_text()
+    except OSError:
+        return None
+    if want not in text:
+        return None
+    for line in text.splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(args, cells, order, pusher_flag, nfilt):
     """The oracle (sequential C++, FP32) on a sub-box of the workload."""
     import numpy as np
@@ -363,7 +380,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(cells, args.ppc) if args.order == 2 and
+                       args.pusher == "vay" else None,
             "avg_launch_ms": avg_ms,
             "launches": int(nl.value),
             "algorithmic_bytes_per_launch": bytes_launch,

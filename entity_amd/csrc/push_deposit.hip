@@ -507,8 +507,10 @@ void k_push_deposit_batched(const FusedArgs* __restrict__ table,
     if (chunk0[mid] <= g) lo = mid;
     else hi = mid - 1;
   }
-  const FusedArgs a = table[lo];
-  push_deposit_chunk<D, O, ALG>(a, g - chunk0[lo]);
+  // the entry is uniform across the workgroup: read its arguments in place
+  // (scalar loads) rather than copying the block into registers
+  const int e = __builtin_amdgcn_readfirstlane(lo);
+  push_deposit_chunk<D, O, ALG>(table[e], g - chunk0[e]);
 }
 
 template <int D, int O, int ALG>
