@@ -129,6 +129,14 @@ class NttGrPusherCtx(C.Structure):
                 ("pusher_flags", C.c_uint8), ("precision", C.c_uint8)]
 
 
+class NttParticleAdjoint(C.Structure):
+    _fields_ = [("lx1", C.c_void_p), ("lx2", C.c_void_p), ("lu1", C.c_void_p),
+                ("lu2", C.c_void_p), ("lu3", C.c_void_p)]
+
+
+VJP_FARADAY, VJP_AMPERE, VJP_FILTER = 0, 1, 2
+
+
 class NttError(RuntimeError):
     def __init__(self, status: int, what: str = ""):
         self.status = status
@@ -150,6 +158,15 @@ SIGNATURES: dict[str, tuple] = {
                               _P(NttMinkowski), C.c_int32, C.c_void_p]),
     "ntt_gr_push": (C.c_int, [_P(NttGrPusherCtx), _P(NttPrtlBC), _P(NttParticles), _P(NttField),
                               _P(NttField), _P(NttKsMetric), C.c_int32, C.c_void_p]),
+    "ntt_sr_push_vjp": (C.c_int, [_P(NttSrPusherCtx), _P(NttParticles), _P(NttField),
+                                  _P(NttMinkowski), C.c_int32, C.c_int32,
+                                  _P(NttParticleAdjoint), _P(NttParticleAdjoint), C.c_void_p,
+                                  C.c_void_p]),
+    "ntt_deposit_vjp": (C.c_int, [_P(NttParticles), _P(C.c_int32), C.c_float, C.c_float,
+                                  C.c_int32, C.c_int32, C.c_void_p, _P(NttParticleAdjoint),
+                                  C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ntt_yee_vjp": (C.c_int, [C.c_int32, _P(C.c_int32), C.c_float, C.c_float, C.c_int32,
+                              C.c_void_p, C.c_void_p, C.c_void_p]),
     "ntt_deposit_currents": (C.c_int, [_P(NttParticles), _P(NttField), _P(NttMinkowski),
                                        C.c_float, C.c_float, C.c_int32, C.c_void_p]),
     "ntt_faraday_mink": (C.c_int, [_P(NttField), C.c_float, C.c_float, _P(NttFdCoeffs),

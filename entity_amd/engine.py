@@ -104,6 +104,7 @@ class SRPICEngine:
         # bound on |weight| of every particle (sets the current's fixed-point scale)
         self.weight_max = 1.0
         self._sort_ws = None
+        self.strict = strict
         self.lib = _abi.load(strict)
         # tile offsets of every species (device uint32, ntiles + 1), or None
         self.tile_offsets: list = [None] * len(self.species)

@@ -350,6 +350,62 @@ NTT_API ntt_status ntt_sr_step(ntt_field_t*                em,
 /* bytes of the workspace ntt_sr_step needs for a current field of this shape */
 NTT_API ntt_status ntt_sr_step_workspace_size(const ntt_field_t* cur, uint64_t* bytes);
 
+/* ---------------------------------------------------------------- adjoint */
+
+/* Per-particle adjoints (d loss / d state) of a 2D species: offsets dx1,
+ * dx2 (the cell index is held fixed) and momenta u1..3; float or double
+ * arrays of npart entries, as the precision argument says. */
+typedef struct ntt_particle_adjoint {
+  void *lx1, *lx2;
+  void *lu1, *lu2, *lu3;
+} ntt_particle_adjoint_t;
+
+#define NTT_VJP_FARADAY 0
+#define NTT_VJP_AMPERE  1
+#define NTT_VJP_FILTER  2
+
+/* Vector-Jacobian product of the 2D Boris push of one species (the forward
+ * of ntt_sr_push / ntt_sr_step: gather of E and B from em at the pre-push
+ * position, Boris, position update), shape order 1 or 2.  `pre` holds the
+ * particles before the push, `post_adj` the adjoint of their state after it.
+ * Adds the position adjoint to pre_adj->lx*, writes pre_adj->lu*, and adds
+ * the adjoint of the gathered fields to lam_em (6 components, active cells,
+ * layout [comp][n2][n1]).  precision 32 or 64 selects float or double
+ * adjoint arrays and arithmetic.  No reference counterpart. */
+NTT_API ntt_status ntt_sr_push_vjp(const ntt_sr_pusher_ctx_t*   ctx,
+                                   const ntt_particles_t*       pre,
+                                   const ntt_field_t*           em,
+                                   const ntt_minkowski_t*       metric,
+                                   int32_t                      shape_order,
+                                   int32_t                      precision,
+                                   const ntt_particle_adjoint_t* post_adj,
+                                   ntt_particle_adjoint_t*       pre_adj,
+                                   void*                        lam_em,
+                                   void*                        stream);
+
+/* Vector-Jacobian product of the 2D Esirkepov deposit (order 1 or 2) of one
+ * species: `post` holds the particles after the push (i_prev, dx_prev, i, dx,
+ * u, weight), lam_j the adjoint of the folded current (3 components, active
+ * cells, [comp][n2][n1]).  Adds to post_adj (positions and momenta after
+ * the push) and writes the adjoint of the pre-push offsets to lx_prev1/2.
+ * No reference counterpart. */
+NTT_API ntt_status ntt_deposit_vjp(const ntt_particles_t* post, const int32_t n[2],
+                                   float charge, float dt, int32_t shape_order,
+                                   int32_t precision, const void* lam_j,
+                                   ntt_particle_adjoint_t* post_adj, void* lx_prev1,
+                                   void* lx_prev2, void* stream);
+
+/* Transposed 2D periodic field operators on active-cell adjoints
+ * ([comp][n2][n1], 3 components each):
+ *   NTT_VJP_FARADAY: lam_out(E) += M^T lam_in(B), M the Faraday update with
+ *                    coefficients c1, c2 (ntt_faraday_mink);
+ *   NTT_VJP_AMPERE:  lam_out(B) += N^T lam_in(E), N the Ampere update;
+ *   NTT_VJP_FILTER:  lam_out = F lam_in, the (symmetric) binomial filter.
+ * No reference counterpart. */
+NTT_API ntt_status ntt_yee_vjp(int32_t op, const int32_t n[2], float c1, float c2,
+                               int32_t precision, const void* lam_in, void* lam_out,
+                               void* stream);
+
 /* ------------------------------------------------------- batched ensembles */
 
 /* One box of an ensemble: the arguments ntt_sr_step takes for that box. */
