@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--batched-steps", type=int, default=12)
     ap.add_argument("--gr", type=int, default=1, help="also time the GR pusher runs")
     ap.add_argument("--gr-steps", type=int, default=5)
+    ap.add_argument("--adjoint", type=int, default=1, help="also time the adjoint (gradient) run")
     return ap.parse_args()
 
 
@@ -294,6 +295,74 @@ def gr_leg(args):
             "hbm_gbs": total * 78 / (ms * 1e-3) / 1e9}
 
 
+def adjoint_leg(args):
+    """Adjoint of a 2D counter-streaming run: periodic 128 x 128 cells, e-/e+
+    beams at u_x +- u_d (u_d = 0.57735, beta = 0.5) over theta = 0.1 thermal
+    momenta, 16 ppc per species x 4 species (64 ppc), uniform B_z = b = 0.1,
+    shape order 2, Boris, 1 filter pass, 40 steps.  Loss L = sum |B|^2 / 2
+    at the last step; gradients dL/du_d and dL/db from the checkpointed
+    reverse pass (checkpoint every 8 steps).  Reports the forward and the
+    gradient wall time, float vs double adjoint agreement, and central
+    finite differences (h = 1e-3) of the forward."""
+    import torch
+    from entity_amd.adjoint import SRAdjoint
+    from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
+    n, ppc, nsteps, K = (128, 128), 16, 40, 8
+    sign = (1.0, 1.0, -1.0, -1.0)
+    ud0, b0, h = 0.57735, 0.1, 1e-3
+
+    def setup(ud, b):
+        prm = SRParams(dim=2, n=n, shape_order=2, n_filters=1, ppc0=4.0 * ppc, larmor0=1.0,
+                       skindepth0=4.0, sort_interval=0, tile=(8, 8))
+        sp = [SpeciesSpec(f"s{k}", 1.0, q) for k, q in enumerate((-1.0, 1.0, -1.0, 1.0))]
+        eng = SRPICEngine(prm, sp, n[0] * n[1] * ppc)
+        eng.inject_uniform(ppc, temperature=0.1, seed=31)
+        for sg, s in zip(sign, eng.species):
+            s.ux1[: s.npart] += sg * ud
+        G = eng.shape.ngh
+        eng.fields.em[5, G:G + n[1], G:G + n[0]] = b
+        return eng
+
+    def grads(prec):
+        eng = setup(ud0, b0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss, lam, lam_u = SRAdjoint(eng, nsteps, K, prec).gradient()
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        g_ud = sum(sg * float(lu[0].double().sum()) for sg, lu in zip(sign, lam_u))
+        return loss, g_ud, float(lam[5].double().sum()), t
+
+    def loss_of(ud, b):
+        eng = setup(ud, b)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L = SRAdjoint(eng, nsteps, K).forward()
+        torch.cuda.synchronize()
+        return L, time.perf_counter() - t0
+
+    loss_of(ud0, b0)            # warm-up
+    L0, t_fwd = loss_of(ud0, b0)
+    grads(32)                   # warm-up
+    L32, gu32, gb32, t32 = grads(32)
+    L64, gu64, gb64, t64 = grads(64)
+    fd_u = (loss_of(ud0 + h, b0)[0] - loss_of(ud0 - h, b0)[0]) / (2 * h)
+    fd_b = (loss_of(ud0, b0 + h)[0] - loss_of(ud0, b0 - h)[0]) / (2 * h)
+    npart = 4 * n[0] * n[1] * ppc
+    return {"workload": "2D periodic 128x128, e-/e+ beams u_x +- 0.57735 over theta 0.1, "
+                        "4 species x 16 ppc (1,048,576 particles), uniform B_z = 0.1, order 2, "
+                        "Boris, 1 filter pass, 40 steps; L = sum B^2/2; checkpoint every 8",
+            "particles": npart, "steps": nsteps,
+            "forward_ms": 1e3 * t_fwd, "forward_value": npart * nsteps / t_fwd,
+            "gradient_ms_f32": 1e3 * t32, "gradient_ms_f64": 1e3 * t64,
+            "gradient_over_forward_f32": t32 / t_fwd,
+            "value": npart * nsteps / t32, "unit": "particle-steps/s (forward + reverse)",
+            "loss": L32, "dL_dud": {"f32": gu32, "f64": gu64, "fd": fd_u},
+            "dL_db": {"f32": gb32, "f64": gb64, "fd": fd_b},
+            "rel_err_vs_fd_f64": {"ud": abs(gu64 - fd_u) / abs(fd_u),
+                                  "b": abs(gb64 - fd_b) / abs(fd_b)}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -397,6 +466,8 @@ def main():
         out["batched_small_boxes"] = batched_leg(args, sizes=(64,))
     if args.gr:
         out["gr"] = gr_leg(args)
+    if args.adjoint:
+        out["adjoint"] = adjoint_leg(args)
     if args.cpu_baseline:
         ccells = tuple(int(x) for x in args.cpu_cells.split(","))
         out["cpu_baseline"] = cpu_baseline(args, ccells, args.order, pusher, args.filters)
