@@ -299,10 +299,11 @@ def adjoint_leg(args):
     """Adjoint of a 2D counter-streaming run: periodic 128 x 128 cells, e-/e+
     beams at u_x +- u_d (u_d = 0.57735, beta = 0.5) over theta = 0.1 thermal
     momenta, 16 ppc per species x 4 species (64 ppc), uniform B_z = b = 0.1,
-    shape order 2, Boris, 1 filter pass, 40 steps.  Loss L = sum |B|^2 / 2
-    at the last step; gradients dL/du_d and dL/db from the checkpointed
-    reverse pass (checkpoint every 8 steps).  Reports the forward and the
-    gradient wall time, float vs double adjoint agreement, and central
+    shape order 2, Boris, 1 filter pass, tile re-sort every 4 steps (particle
+    IDs carry the adjoint through the re-sorts), 40 steps.  Loss
+    L = sum |B|^2 / 2 at the last step; gradients dL/du_d and dL/db from the
+    checkpointed reverse pass (checkpoint every 8 steps).  Reports the forward
+    and the gradient wall time, float vs double adjoint agreement, and central
     finite differences (h = 1e-3) of the forward."""
     import torch
     from entity_amd.adjoint import SRAdjoint
@@ -313,9 +314,9 @@ def adjoint_leg(args):
 
     def setup(ud, b):
         prm = SRParams(dim=2, n=n, shape_order=2, n_filters=1, ppc0=4.0 * ppc, larmor0=1.0,
-                       skindepth0=4.0, sort_interval=0, tile=(8, 8))
+                       skindepth0=4.0, sort_interval=4, tile=(8, 8))
         sp = [SpeciesSpec(f"s{k}", 1.0, q) for k, q in enumerate((-1.0, 1.0, -1.0, 1.0))]
-        eng = SRPICEngine(prm, sp, n[0] * n[1] * ppc)
+        eng = SRPICEngine(prm, sp, n[0] * n[1] * ppc, with_payload=True)
         eng.inject_uniform(ppc, temperature=0.1, seed=31)
         for sg, s in zip(sign, eng.species):
             s.ux1[: s.npart] += sg * ud
@@ -351,7 +352,8 @@ def adjoint_leg(args):
     npart = 4 * n[0] * n[1] * ppc
     return {"workload": "2D periodic 128x128, e-/e+ beams u_x +- 0.57735 over theta 0.1, "
                         "4 species x 16 ppc (1,048,576 particles), uniform B_z = 0.1, order 2, "
-                        "Boris, 1 filter pass, 40 steps; L = sum B^2/2; checkpoint every 8",
+                        "Boris, 1 filter pass, re-sort every 4 steps, 40 steps; L = sum B^2/2; "
+                        "checkpoint every 8",
             "particles": npart, "steps": nsteps,
             "forward_ms": 1e3 * t_fwd, "forward_value": npart * nsteps / t_fwd,
             "gradient_ms_f32": 1e3 * t32, "gradient_ms_f64": 1e3 * t64,

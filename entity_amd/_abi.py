@@ -131,7 +131,7 @@ class NttGrPusherCtx(C.Structure):
 
 class NttParticleAdjoint(C.Structure):
     _fields_ = [("lx1", C.c_void_p), ("lx2", C.c_void_p), ("lu1", C.c_void_p),
-                ("lu2", C.c_void_p), ("lu3", C.c_void_p)]
+                ("lu2", C.c_void_p), ("lu3", C.c_void_p), ("size", C.c_int64)]
 
 
 VJP_FARADAY, VJP_AMPERE, VJP_FILTER = 0, 1, 2
@@ -164,7 +164,11 @@ SIGNATURES: dict[str, tuple] = {
                                   C.c_void_p]),
     "ntt_deposit_vjp": (C.c_int, [_P(NttParticles), _P(C.c_int32), C.c_float, C.c_float,
                                   C.c_int32, C.c_int32, C.c_void_p, _P(NttParticleAdjoint),
-                                  C.c_void_p, C.c_void_p, C.c_void_p]),
+                                  _P(NttParticleAdjoint), C.c_void_p]),
+    "ntt_sr_particle_vjp": (C.c_int, [_P(NttSrPusherCtx), _P(NttParticles), _P(NttParticles),
+                                      _P(NttField), _P(NttMinkowski), C.c_int32, C.c_int32,
+                                      C.c_void_p, _P(NttParticleAdjoint),
+                                      _P(NttParticleAdjoint), C.c_void_p, C.c_void_p]),
     "ntt_yee_vjp": (C.c_int, [C.c_int32, _P(C.c_int32), C.c_float, C.c_float, C.c_int32,
                               C.c_void_p, C.c_void_p, C.c_void_p]),
     "ntt_deposit_currents": (C.c_int, [_P(NttParticles), _P(NttField), _P(NttMinkowski),

@@ -3,13 +3,19 @@
 ``SRAdjoint`` runs an ``SRPICEngine`` forward for ``nsteps`` steps and
 propagates the adjoint of the loss L = sum over active cells of |B|^2 / 2
 at the final step back to the initial state, with the hand-written VJP
-kernels of the library (``ntt_sr_push_vjp``, ``ntt_deposit_vjp``,
-``ntt_yee_vjp``).  Memory is bounded by checkpointing the state every
+kernels of the library (``ntt_sr_particle_vjp``, the fused form of
+``ntt_deposit_vjp`` and ``ntt_sr_push_vjp``, and ``ntt_yee_vjp``).  Memory is bounded by checkpointing the state every
 ``checkpoint`` steps and recomputing the steps in between.
+
+Particle adjoints are indexed by particle ID when the engine carries an ID
+payload (``SRPICEngine(..., with_payload=True)``): the driver numbers the
+particles 0..npart-1 in ``pld_i`` at the start, and the tile re-sorts of the
+forward then permute particles without disturbing the adjoint.  Without a
+payload the adjoint is indexed by position and re-sorting must be off.
 
 ``FieldEnergyLoss`` wraps it as a ``torch.autograd.Function`` of the two
 parameters of a counter-streaming (Weibel) setup: the drift 4-velocity u_d
-(species s gets u_x += sign_s u_d) and a uniform background B_z amplitude b.
+(species s gets u_x += sign_s u_d) and the amplitude b of B_z = b * profile.
 
 Reverse of one step (forward: B_h = B - cf curl E; push with E, B_h;
 deposit; filter; B' = B_h - cf curl E; E' = E + ca curl B' + k J_f):
@@ -29,45 +35,76 @@ import torch
 from . import _abi
 from .containers import field_abi
 from .engine import SRPICEngine, _stream
-from .kernels import communicate_fields, mink
 
-_PRT_STATE = ("i1", "i2", "i1_prev", "i2_prev", "dx1", "dx2", "dx1_prev", "dx2_prev",
-              "ux1", "ux2", "ux3", "weight", "tag")
-
-
-def _snap(eng: SRPICEngine) -> dict:
-    return {"em": eng.fields.em.clone(), "step": eng.step, "time": eng.time,
-            "prt": [{k: getattr(sp, k)[: sp.npart].clone() for k in _PRT_STATE}
-                    for sp in eng.species]}
+# arrays the push changes, in snapshot order (int32 or float32, npart each)
+_MOVED = ("i1", "i2", "i1_prev", "i2_prev", "dx1", "dx2", "dx1_prev", "dx2_prev",
+          "ux1", "ux2", "ux3")
+# float arrays only a re-sort changes (permutes)
+_SORTED = ("weight",)
+_INT32 = ("i1", "i2", "i1_prev", "i2_prev")
 
 
-def _restore(eng: SRPICEngine, s: dict) -> None:
-    eng.fields.em.copy_(s["em"])
-    eng.step, eng.time = s["step"], s["time"]
-    for sp, st in zip(eng.species, s["prt"]):
-        for k, v in st.items():
-            getattr(sp, k)[: v.shape[0]].copy_(v)
+class _Snap:
+    """Particle arrays of one species at one instant, packed in one int32
+    buffer (one launch to take), plus tag and IDs when re-sorts can move them."""
 
+    def __init__(self, sp, with_sorted: bool, views: dict):
+        n = sp.npart
+        self.n = n
+        self.names = _MOVED + (_SORTED if with_sorted else ())
+        self.has_pld = with_sorted and sp.pld_i is not None
+        # int32 views of the species' arrays, made once per (species, kind)
+        key = (id(sp), n, with_sorted)
+        parts = views.get(key)
+        if parts is None:
+            parts = [getattr(sp, k)[:n].view(torch.int32) for k in self.names]
+            if self.has_pld:
+                parts.append(sp.pld_i[:n].view(torch.int32))
+            views[key] = parts
+        self.buf = torch.cat(parts)
+        self.tag = sp.tag[:n].clone() if with_sorted else None
+        self.shared = sp   # weight / tag / IDs when they are not in the snapshot
+        self._abi = None
 
-class _PrtView:
-    """Particle arrays of a snapshot, in the layout of the C ABI."""
+    def array(self, name: str):
+        n = self.n
+        if name in self.names:
+            k = self.names.index(name)
+            v = self.buf[k * n:(k + 1) * n]
+            return v if name in _INT32 else v.view(torch.float32)
+        if name == "pld_i":
+            if self.has_pld:
+                k = len(self.names)
+                return self.buf[k * n:(k + 2) * n].view(torch.int64)
+            return self.shared.pld_i[:n] if self.shared.pld_i is not None else None
+        if name == "tag":
+            return self.tag if self.tag is not None else self.shared.tag[:n]
+        return getattr(self.shared, name)[:n]
 
-    def __init__(self, st: dict, maxnpart: int):
-        self.st = st
-        self.n = st["tag"].shape[0]
-        self.maxnpart = maxnpart
+    def abi(self, maxnpart: int) -> _abi.NttParticles:
+        """The C ABI view (built once; pointers into the packed buffer)."""
+        if self._abi is None:
+            s = _abi.NttParticles()
+            base, n = self.buf.data_ptr(), self.n
+            for k, name in enumerate(self.names):
+                setattr(s, name, base + 4 * k * n)
+            for name in ("weight", "tag"):
+                if name not in self.names:
+                    setattr(s, name, self.array(name).data_ptr())
+            ids = self.array("pld_i")
+            s.pld_i = ids.data_ptr() if ids is not None else None
+            s.npart, s.maxnpart = n, maxnpart
+            self._abi = s
+        return self._abi
 
-    def abi(self) -> _abi.NttParticles:
-        s = _abi.NttParticles()
-        g = self.st
-        s.i1, s.i2 = g["i1"].data_ptr(), g["i2"].data_ptr()
-        s.i1_prev, s.i2_prev = g["i1_prev"].data_ptr(), g["i2_prev"].data_ptr()
-        s.dx1, s.dx2 = g["dx1"].data_ptr(), g["dx2"].data_ptr()
-        s.dx1_prev, s.dx2_prev = g["dx1_prev"].data_ptr(), g["dx2_prev"].data_ptr()
-        s.ux1, s.ux2, s.ux3 = (g[k].data_ptr() for k in ("ux1", "ux2", "ux3"))
-        s.weight, s.tag = g["weight"].data_ptr(), g["tag"].data_ptr()
-        s.npart, s.maxnpart = self.n, self.maxnpart
-        return s
+    def restore(self, sp) -> None:
+        for k in _MOVED + ("weight", "tag", "pld_i"):
+            src = self.array(k)
+            if src is None:
+                continue
+            dst = getattr(sp, k)[: self.n]
+            if src.data_ptr() != dst.data_ptr():
+                dst.copy_(src)
 
 
 class SRAdjoint:
@@ -79,12 +116,17 @@ class SRAdjoint:
             raise ValueError("the adjoint covers shape orders 1 and 2")
         if any(s.pusher != _abi.PUSHER_BORIS for s in engine.species):
             raise ValueError("the adjoint covers the Boris pusher")
-        if p.sort_interval:
-            raise ValueError("re-sorting reorders particles: use sort_interval = 0")
+        self.ids = all(s.pld_i is not None for s in engine.species)
+        if p.sort_interval and not self.ids:
+            raise ValueError("re-sorting permutes particles: create the engine with "
+                             "with_payload=True (particle IDs) or use sort_interval = 0")
         if p.fd:
             raise ValueError("the adjoint covers the Yee stencil")
+        if precision not in (32, 64):
+            raise ValueError("precision is 32 or 64")
         self.eng, self.nsteps, self.K, self.prec = engine, nsteps, max(1, checkpoint), precision
         self.rdt = torch.float64 if precision == 64 else torch.float32
+        self.sorting = bool(p.sort_interval)
         n1, n2 = p.n
         self.nn = (C.c_int32 * 2)(n1, n2)
         # the step's field coefficients, in single precision as the step forms them
@@ -96,6 +138,8 @@ class SRAdjoint:
         self.amp = (float(dTa / (dx * dx)), float(dTa))      # fieldsolvers.h:108-117
         self.kJ = float(-dt * f(p.q0) / (f(p.B0) * f(p.V0)))  # fieldsolvers.h:157
         self.lib = engine.lib
+        self._views = {}
+        self._fd0 = _abi.NttFdCoeffs()   # the plain Yee stencil
 
     # -- helpers -------------------------------------------------------------------
     def _active_B(self) -> torch.Tensor:
@@ -110,20 +154,57 @@ class SRAdjoint:
         """E and B after the step's first Faraday half (what the push gathers)."""
         e = self.eng
         em = e.fields.em.clone()
-        communicate_fields(em, e.shape, 0, 6, strict=e.strict)
-        k = mink.Faraday_kernel(em, e.shape, *self.far)
-        k.strict = e.strict
-        k()
-        communicate_fields(em, e.shape, 3, 6, strict=e.strict)
+        f = field_abi(em, e.shape)
+        st = _stream()
+        _abi.check(self.lib.ntt_communicate_fields(C.byref(f), 0, 6, st), "ntt_communicate_fields")
+        _abi.check(self.lib.ntt_faraday_mink(C.byref(f), self.far[0], self.far[1],
+                                             C.byref(self._fd0), st), "ntt_faraday_mink")
+        _abi.check(self.lib.ntt_communicate_fields(C.byref(f), 3, 6, st), "ntt_communicate_fields")
         return em
+
+    def _checkpoint(self) -> dict:
+        e = self.eng
+        return {"em": e.fields.em.clone(), "step": e.step, "time": e.time,
+                "prt": [_Snap(sp, True, self._views) for sp in e.species],
+                "tiles": [t.clone() if t is not None else None for t in e.tile_offsets]}
+
+    def _restore(self, ck: dict) -> None:
+        e = self.eng
+        e.fields.em.copy_(ck["em"])
+        e.step, e.time = ck["step"], ck["time"]
+        for sp, snap in zip(e.species, ck["prt"]):
+            snap.restore(sp)
+        e.tile_offsets = [t.clone() if t is not None else None for t in ck["tiles"]]
 
     def _vjp_yee(self, op, c1, c2, lin, lout):
         _abi.check(self.lib.ntt_yee_vjp(op, self.nn, c1, c2, self.prec, lin.data_ptr(),
                                         lout.data_ptr(), _stream()), "ntt_yee_vjp")
 
-    def _padj(self, d) -> _abi.NttParticleAdjoint:
+    @staticmethod
+    def _padj(d) -> _abi.NttParticleAdjoint:
         return _abi.NttParticleAdjoint(*(d[k].data_ptr() for k in ("lx1", "lx2", "lu1", "lu2",
-                                                                   "lu3")))
+                                                                   "lu3")), d["lx1"].numel())
+
+    def _prepare_reverse(self, post, pre):
+        """ABI structs of the two particle-adjoint sets (built once)."""
+        return ([self._padj(d) for d in post], [self._padj(d) for d in pre])
+
+    def _record(self, n0: int, n1: int) -> list:
+        """Runs steps n0..n1-1 from the engine's state, recording per step the
+        particles before the push, E/B after the first Faraday half and the
+        particles after the push."""
+        e = self.eng
+        recs = []
+        before = [_Snap(sp, True, self._views) for sp in e.species]
+        for _ in range(n0, n1):
+            if e.maybe_sort():
+                before = [_Snap(sp, True, self._views) for sp in e.species]
+            emh = self._em_half()
+            e.advance()
+            after = [_Snap(sp, self.sorting, self._views) for sp in e.species]
+            recs.append((before, emh, after))
+            before = after
+        return recs
 
     # -- forward / backward ----------------------------------------------------------
     def forward(self) -> float:
@@ -133,19 +214,27 @@ class SRAdjoint:
 
     def gradient(self):
         """Runs forward and backward from the engine's current (initial) state.
-        Returns (loss, lam_em, lam_u) with lam_em = dL/d(E, B) on active cells
-        ([6][n2][n1]) and lam_u[s] = (dL/du1, dL/du2, dL/du3) per species at the
-        initial state.  The engine ends at the final state."""
+        Returns (loss, lam_em, lam_u): lam_em = dL/d(E, B) on the active cells
+        ([6][n2][n1]); lam_u[s] = (dL/du1, dL/du2, dL/du3) of species s at the
+        initial state, indexed by particle ID (the initial position) when the
+        engine carries IDs.  The engine ends at the final state; with IDs, its
+        pld_i are overwritten with the initial positions 0..npart-1."""
         e = self.eng
         N, K = self.nsteps, self.K
+        if self.ids:
+            for sp in e.species:
+                sp.pld_i[: sp.npart] = torch.arange(sp.npart, device=e.device)
+        # checkpoints every K steps; the last segment is recorded on the way
+        # (it is the first one the reverse pass needs)
+        last = ((N - 1) // K) * K if N > 0 else 0
         ckpts = {}
-        for n in range(N):
+        for n in range(last):
             if n % K == 0:
-                ckpts[n] = _snap(e)
+                ckpts[n] = self._checkpoint()
             e.step_forward()
-        torch.cuda.synchronize()
+        recs_last = self._record(last, N)
         loss = self.loss()
-        final = _snap(e)
+        final = self._checkpoint()
         n1, n2 = e.params.n
         dev = e.device
         lam = torch.zeros((6, n2, n1), dtype=self.rdt, device=dev)
@@ -156,47 +245,40 @@ class SRAdjoint:
                  for k in ("lx1", "lx2", "lu1", "lu2", "lu3")} for sp in e.species]
         pre = [{k: torch.zeros_like(v) for k, v in d.items()} for d in post]
         metric = _abi.NttMinkowski(2, e.params.dx)
+        ctxs = [_abi.NttSrPusherCtx(sp.mass, sp.charge, e.params.dt, e.params.omegaB0, n1, n2, 1,
+                                    sp.pusher) for sp in e.species]
+        pa_post_all, pa_pre_all = self._prepare_reverse(post, pre)
+        lam_ptr, lamE, lamB = lam.data_ptr(), lam[0:3], lam[3:6]
+        order, prec, stream = e.params.shape_order, self.prec, _stream()
         for s0 in reversed(range(0, N, K)):
-            _restore(e, ckpts[s0])
-            recs = []
-            for n in range(s0, min(s0 + K, N)):
-                before = {"prt": [{k: getattr(sp, k)[: sp.npart].clone() for k in _PRT_STATE}
-                                  for sp in e.species]}
-                emh = self._em_half()
-                e.step_forward()
-                after = {"prt": [{k: getattr(sp, k)[: sp.npart].clone() for k in _PRT_STATE}
-                                 for sp in e.species]}
-                recs.append((before, emh, after))
+            if s0 == last:
+                recs, recs_last = recs_last, None
+            else:
+                self._restore(ckpts.pop(s0))
+                recs = self._record(s0, min(s0 + K, N))
             for before, emh, after in reversed(recs):
                 # Ampere + current source, second Faraday half
-                lamJ.copy_(lam[0:3]).mul_(self.kJ)
-                self._vjp_yee(_abi.VJP_AMPERE, *self.amp, lam[0:3], lam[3:6])
-                self._vjp_yee(_abi.VJP_FARADAY, *self.far, lam[3:6], lam[0:3])
+                torch.mul(lamE, self.kJ, out=lamJ)
+                self._vjp_yee(_abi.VJP_AMPERE, *self.amp, lamE, lamB)
+                self._vjp_yee(_abi.VJP_FARADAY, *self.far, lamB, lamE)
                 # filter passes (symmetric stencil)
                 src, dst = lamJ, lamJ2
                 for _ in range(e.params.n_filters):
                     self._vjp_yee(_abi.VJP_FILTER, 0.0, 0.0, src, dst)
                     src, dst = dst, src
-                # particles: deposit, then push (both per species)
+                fe = field_abi(emh, e.shape)
+                # particles: deposit and push VJPs of each species in one pass
                 for s, sp in enumerate(e.species):
-                    pv_post = _PrtView(after["prt"][s], sp.maxnpart).abi()
-                    pa_post, pa_pre = self._padj(post[s]), self._padj(pre[s])
-                    _abi.check(self.lib.ntt_deposit_vjp(
-                        C.byref(pv_post), self.nn, sp.charge, e.params.dt, e.params.shape_order,
-                        self.prec, src.data_ptr(), C.byref(pa_post), pre[s]["lx1"].data_ptr(),
-                        pre[s]["lx2"].data_ptr(), _stream()), "ntt_deposit_vjp")
-                    pv_pre = _PrtView(before["prt"][s], sp.maxnpart).abi()
-                    ctx = _abi.NttSrPusherCtx(sp.mass, sp.charge, e.params.dt, e.params.omegaB0,
-                                              n1, n2, 1, sp.pusher)
-                    fe = field_abi(emh, e.shape)
-                    _abi.check(self.lib.ntt_sr_push_vjp(
-                        C.byref(ctx), C.byref(pv_pre), C.byref(fe), C.byref(metric),
-                        e.params.shape_order, self.prec, C.byref(pa_post), C.byref(pa_pre),
-                        lam.data_ptr(), _stream()), "ntt_sr_push_vjp")
+                    _abi.check(self.lib.ntt_sr_particle_vjp(
+                        C.byref(ctxs[s]), C.byref(before[s].abi(sp.maxnpart)),
+                        C.byref(after[s].abi(sp.maxnpart)), C.byref(fe), C.byref(metric), order,
+                        prec, src.data_ptr(), C.byref(pa_post_all[s]), C.byref(pa_pre_all[s]),
+                        lam_ptr, stream), "ntt_sr_particle_vjp")
                 # first Faraday half
-                self._vjp_yee(_abi.VJP_FARADAY, *self.far, lam[3:6], lam[0:3])
+                self._vjp_yee(_abi.VJP_FARADAY, *self.far, lamB, lamE)
                 post, pre = pre, post
-        _restore(e, final)
+                pa_post_all, pa_pre_all = pa_pre_all, pa_post_all
+        self._restore(final)
         torch.cuda.synchronize()
         lam_u = [(d["lu1"], d["lu2"], d["lu3"]) for d in post]
         return loss, lam, lam_u

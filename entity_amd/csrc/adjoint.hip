@@ -252,118 +252,33 @@ struct AdjField {
 };
 
 template <typename R>
-struct PAdj {  // per-particle adjoints
+struct PAdj {  // per-particle adjoints, indexed by particle ID when ids != nullptr
   R *lx1, *lx2, *lu1, *lu2, *lu3;
+  int64_t size;
 };
 
-// ---- push VJP -------------------------------------------------------------------
-// Forward of one particle (sr_move, 2D, Boris): e, b gathered at (i, dx),
-// in-plane components times dxm, Boris, dx' = dx + (u'/dxm) dt/gamma'.
-// Inputs of the Jacobian: dx1, dx2, u1..3, e1..3, b1..3 (11); outputs dx1',
-// dx2', u1'..3' (5).
-template <int O, typename R>
-__global__ __launch_bounds__(256) void k_push_vjp(ParticleView P, EmView F, int ngh, R ndth,
-                                                  R dt, R dxm, PAdj<R> post, PAdj<R> pre,
-                                                  AdjField<R> lem) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= P.n) return;
-  // dead particles, and particles outside the active cells (which the
-  // forward push rejects), carry no adjoint
-  const int n1 = F.N1 - 2 * ngh, n2 = F.N2 - 2 * ngh;
-  if (P.tag[p] != NTT_TAG_ALIVE || P.i1[p] < 0 || P.i1[p] >= n1 || P.i2[p] < 0 ||
-      P.i2[p] >= n2 || !(P.dx1[p] >= 0.0f && P.dx1[p] <= 1.0f) ||
-      !(P.dx2[p] >= 0.0f && P.dx2[p] <= 1.0f)) {
-    pre.lu1[p] = pre.lu2[p] = pre.lu3[p] = R(0);
-    return;
+// entry of particle p in the adjoint arrays: its ID, or p; -1 (and a device
+// error) for an ID outside [0, size)
+__device__ __forceinline__ int64_t adj_index(const int64_t* ids, int64_t p, int64_t size,
+                                             uint32_t* err) {
+  if (!ids) return p;
+  const int64_t q = ids[p];
+  if (q < 0 || q >= size) {
+    raise_device_error(err, NTT_DEVERR_OUT_OF_BOUNDS);
+    return -1;
   }
-  using D11 = Dual<11, R>;
-  using D2  = Dual<2, R>;
-  const int  i = P.i1[p] + ngh, j = P.i2[p] + ngh;
-  const D2   x1 = D2::var(static_cast<R>(P.dx1[p]), 0), x2 = D2::var(static_cast<R>(P.dx2[p]), 1);
-  // gather with position derivatives: e_c, b_c as functions of (dx1, dx2)
-  D2  S1p[O + 1], S1d[O + 1], S2p[O + 1], S2d[O + 1];
-  int ip, id, jp, jd;
-  shape_t<false, O, D2, R>(i, x1, ip, S1p);
-  shape_t<false, O, D2, R>(j, x2, jp, S2p);
-  shape_t<true, O, D2, R>(i, x1, id, S1d);
-  shape_t<true, O, D2, R>(j, x2, jd, S2d);
-  auto gather = [&](int c, int imin, int jmin, const D2* A, const D2* B) -> D2 {
-    D2 e(R(0));
-    for (int b = 0; b < O + 1; b++) {
-      D2 c0(R(0));
-      for (int a = 0; a < O + 1; a++) c0 = c0 + A[a] * static_cast<R>(F(c, imin + a, jmin + b));
-      e = e + c0 * B[b];
-    }
-    return e;
-  };
-  D2 g[6];
-  g[0] = gather(0, id, jp, S1d, S2p);
-  g[1] = gather(1, ip, jd, S1p, S2d);
-  g[2] = gather(2, ip, jp, S1p, S2p);
-  g[3] = gather(3, ip, jd, S1p, S2d);
-  g[4] = gather(4, id, jp, S1d, S2p);
-  g[5] = gather(5, id, jd, S1d, S2d);
-  // Boris + position update over the 11 inputs
-  D11 u[3] = { D11::var(static_cast<R>(P.ux1[p]), 2), D11::var(static_cast<R>(P.ux2[p]), 3),
-               D11::var(static_cast<R>(P.ux3[p]), 4) };
-  D11 e[3], b[3];
-  for (int c = 0; c < 3; ++c) {
-    e[c] = D11::var(g[c].v, 5 + c);
-    b[c] = D11::var(g[3 + c].v, 8 + c);
-  }
-  for (int c = 0; c < 2; ++c) {
-    e[c] = e[c] * dxm;
-    b[c] = b[c] * dxm;
-  }
-  boris_t<D11, R>(ndth, u, e, b);
-  const D11 dtie = dt / tsqrt(R(1) + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
-  const D11 X1 = D11::var(static_cast<R>(P.dx1[p]), 0) + (u[0] / D11(dxm)) * dtie;
-  const D11 X2 = D11::var(static_cast<R>(P.dx2[p]), 1) + (u[1] / D11(dxm)) * dtie;
-  const R lo[5] = { post.lx1[p], post.lx2[p], post.lu1[p], post.lu2[p], post.lu3[p] };
-  R lin[11];
-  for (int k = 0; k < 11; ++k)
-    lin[k] = lo[0] * X1.d[k] + lo[1] * X2.d[k] + lo[2] * u[0].d[k] + lo[3] * u[1].d[k] +
-             lo[4] * u[2].d[k];
-  // adjoint of the gather: scatter into the field adjoint, and the
-  // position derivative of the gathered values
-  R lx1 = lin[0], lx2 = lin[1];
-  for (int c = 0; c < 6; ++c) {
-    const R l = lin[5 + c];
-    lx1 += l * g[c].d[0];
-    lx2 += l * g[c].d[1];
-  }
-  auto scatter = [&](int c, int imin, int jmin, const D2* A, const D2* B, R l) {
-    if (l == R(0)) return;
-    for (int bb = 0; bb < O + 1; bb++)
-      for (int a = 0; a < O + 1; a++)
-        lem.add(c, imin + a - ngh, jmin + bb - ngh, l * A[a].v * B[bb].v);
-  };
-  scatter(0, id, jp, S1d, S2p, lin[5]);
-  scatter(1, ip, jd, S1p, S2d, lin[6]);
-  scatter(2, ip, jp, S1p, S2p, lin[7]);
-  scatter(3, ip, jd, S1p, S2d, lin[8]);
-  scatter(4, id, jp, S1d, S2p, lin[9]);
-  scatter(5, id, jd, S1d, S2d, lin[10]);
-  pre.lx1[p] += lx1;
-  pre.lx2[p] += lx2;
-  pre.lu1[p] = lin[2];
-  pre.lu2[p] = lin[3];
-  pre.lu3[p] = lin[4];
+  return q;
 }
 
 // ---- deposit VJP ----------------------------------------------------------------
 // Esirkepov 2D (currents_deposit.hpp:476-555) over the 7 inputs dx_prev1,2,
 // dx1,2, u1..3 (u enters through the out-of-plane velocity); the adjoint of
 // each current slot is read from the current adjoint at its periodic image.
+// acc[0..6] = d<lam_j, J_p>/d(dx_prev1, dx_prev2, dx1, dx2, u1, u2, u3).
 template <int O, typename R>
-__global__ __launch_bounds__(256) void k_deposit_vjp(ParticleView P, AdjField<R> lj, R charge,
-                                                     R inv_dt, PAdj<R> post, R* lxp1, R* lxp2) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= P.n) return;
-  if (P.tag[p] != NTT_TAG_ALIVE) {
-    lxp1[p] = lxp2[p] = R(0);
-    return;
-  }
+__device__ __forceinline__ void deposit_vjp_acc(const ParticleView& P, int64_t p,
+                                                const AdjField<R>& lj, R charge, R inv_dt,
+                                                R* acc) {
   using D7 = Dual<7, R>;
   constexpr int M = O + 2;
   const D7 xp1 = D7::var(static_cast<R>(P.dx1p[p]), 0), xp2 = D7::var(static_cast<R>(P.dx2p[p]), 1);
@@ -378,9 +293,15 @@ __global__ __launch_bounds__(256) void k_deposit_vjp(ParticleView P, AdjField<R>
   for_deposit_t<O, D7, R>(P.i1p[p], xp1, P.i1[p], x1, i1_min, d1, iS1, fS1);
   for_deposit_t<O, D7, R>(P.i2p[p], xp2, P.i2[p], x2, i2_min, d2, iS2, fS2);
   const R Qdt = coeff * inv_dt;
-  R acc[7] = { R(0), R(0), R(0), R(0), R(0), R(0), R(0) };
+  for (int k = 0; k < 7; ++k) acc[k] = R(0);
+  // slots inside the active range are read directly, others through the
+  // periodic wrap
+  const bool    inner = i1_min >= 0 && i1_min + M <= lj.n1 && i2_min >= 0 && i2_min + M <= lj.n2;
+  const int64_t base  = static_cast<int64_t>(i1_min) + static_cast<int64_t>(lj.n1) * i2_min;
+  const int64_t cs    = static_cast<int64_t>(lj.n1) * lj.n2;
   auto take = [&](int c, int a, int b, const D7& v) {
-    const R l = lj.get(c, i1_min + a, i2_min + b);
+    const R l = inner ? lj.d[base + a + static_cast<int64_t>(lj.n1) * b + cs * c]
+                      : lj.get(c, i1_min + a, i2_min + b);
     for (int k = 0; k < 7; ++k) acc[k] += l * v.d[k];
   };
   // jx1: prefix over a of -Qdt/2 (fS1 - iS1)(fS2 + iS2)
@@ -407,13 +328,266 @@ __global__ __launch_bounds__(256) void k_deposit_vjp(ParticleView P, AdjField<R>
                                iS2[b] * (R(0.5) * fS1[a] + iS1[a]));
         take(2, a, b, QVx3 * w3);
       }
-  lxp1[p] = acc[0];
-  lxp2[p] = acc[1];
-  post.lx1[p] += acc[2];
-  post.lx2[p] += acc[3];
-  post.lu1[p] += acc[4];
-  post.lu2[p] += acc[5];
-  post.lu3[p] += acc[6];
+}
+
+template <int O, typename R>
+__global__ __launch_bounds__(256) void k_deposit_vjp(ParticleView P, const int64_t* ids,
+                                                     AdjField<R> lj, R charge, R inv_dt,
+                                                     PAdj<R> post, PAdj<R> pre, uint32_t* err) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  const int64_t q = adj_index(ids, p, min(pre.size, post.size), err);
+  if (q < 0) return;
+  if (P.tag[p] != NTT_TAG_ALIVE) {
+    pre.lx1[q] = pre.lx2[q] = R(0);
+    return;
+  }
+  R acc[7];
+  deposit_vjp_acc<O, R>(P, p, lj, charge, inv_dt, acc);
+  pre.lx1[q] = acc[0];
+  pre.lx2[q] = acc[1];
+  post.lx1[q] += acc[2];
+  post.lx2[q] += acc[3];
+  post.lu1[q] += acc[4];
+  post.lu2[q] += acc[5];
+  post.lu3[q] += acc[6];
+}
+
+// ---- push VJP -------------------------------------------------------------------
+// Forward of one particle (sr_move, 2D, Boris): e, b gathered at (i, dx),
+// in-plane components times dxm, Boris, dx' = dx + (u'/dxm) dt/gamma'.
+// Inputs of the Jacobian: dx1, dx2, u1..3, e1..3, b1..3 (11); outputs dx1',
+// dx2', u1'..3' (5).
+//
+// The adjoint of the gather is a scatter of 6 x (O+1)^2 values per particle.
+// A workgroup takes a chunk of kVjpChunk consecutive particles; when their
+// cells (with the stencil reach [-1, +2]) fit a kVW x kVW window, in the
+// plain frame or in the frame rotated by n/2 (a chunk straddling the periodic
+// boundary), the scatter goes to an LDS window of doubles (ds_add_f64, about
+// 20x the throughput of the 32-bit float LDS atomic on this chip), which is
+// then added to the field adjoint with one global atomic per non-zero entry.
+// Chunks that do not fit scatter with global atomics.
+constexpr int kVjpThreads = 256, kVjpWaves = kVjpThreads / 64, kVjpPerThread = 2;
+constexpr int kVjpChunk = kVjpThreads * kVjpPerThread;
+constexpr int kVW = 32;
+
+__device__ __forceinline__ int vjp_wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int vjp_wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int vjp_rotated(int c, int n) {
+  const int h = n >> 1;
+  const int r = c + h;
+  return r >= n ? r - n : r;
+}
+
+// DEP = true fuses the deposit VJP of the same step in front: Q holds the
+// particles after the push (same order as P), lj the current adjoint; the
+// deposit's adjoint then feeds the push's in registers, post is only read and
+// pre.lx is written (not accumulated).
+template <int O, typename R, bool DEP>
+__global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const int64_t* ids,
+                                                          EmView F, int ngh, R ndth, R dt, R dxm,
+                                                          PAdj<R> post, PAdj<R> pre,
+                                                          AdjField<R> lem, uint32_t* err,
+                                                          ParticleView Q, AdjField<R> lj,
+                                                          R charge, R inv_dt) {
+  __shared__ double win[6 * kVW * kVW];
+  __shared__ int    red[8][kVjpWaves];
+  const int     tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int64_t beg = static_cast<int64_t>(blockIdx.x) * kVjpChunk;
+  const int64_t end = min(beg + kVjpChunk, P.n);
+  const int     n1 = F.N1 - 2 * ngh, n2 = F.N2 - 2 * ngh;
+  // dead particles, and particles outside the active cells (which the
+  // forward push rejects), carry no adjoint
+  auto valid = [&](int64_t p) {
+    return P.tag[p] == NTT_TAG_ALIVE && P.i1[p] >= 0 && P.i1[p] < n1 && P.i2[p] >= 0 &&
+           P.i2[p] < n2 && P.dx1[p] >= 0.0f && P.dx1[p] <= 1.0f && P.dx2[p] >= 0.0f &&
+           P.dx2[p] <= 1.0f;
+  };
+
+  // ---- pass 1: the chunk's cell box in the plain and the rotated frame
+  int lo[2] = { INT_MAX, INT_MAX }, hi[2] = { INT_MIN, INT_MIN };
+  int rlo[2] = { INT_MAX, INT_MAX }, rhi[2] = { INT_MIN, INT_MIN };
+  for (int64_t p = beg + tid; p < end; p += kVjpThreads) {
+    if (!valid(p)) continue;
+    const int c[2] = { P.i1[p], P.i2[p] }, n[2] = { n1, n2 };
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      const int r = vjp_rotated(c[d], n[d]);
+      lo[d]  = min(lo[d], c[d]);
+      hi[d]  = max(hi[d], c[d]);
+      rlo[d] = min(rlo[d], r);
+      rhi[d] = max(rhi[d], r);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    const int l = vjp_wave_min(lo[d]), h = vjp_wave_max(hi[d]);
+    const int rl = vjp_wave_min(rlo[d]), rh = vjp_wave_max(rhi[d]);
+    if (lane == 0) {
+      red[4 * d][wv]     = l;
+      red[4 * d + 1][wv] = h;
+      red[4 * d + 2][wv] = rl;
+      red[4 * d + 3][wv] = rh;
+    }
+  }
+  __syncthreads();
+  bool fits = true, rot[2];
+  int  wlo[2], ext[2];
+  {
+    const int n[2] = { n1, n2 };
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      int l = red[4 * d][0], h = red[4 * d + 1][0], rl = red[4 * d + 2][0], rh = red[4 * d + 3][0];
+#pragma unroll
+      for (int w = 1; w < kVjpWaves; ++w) {
+        l  = min(l, red[4 * d][w]);
+        h  = max(h, red[4 * d + 1][w]);
+        rl = min(rl, red[4 * d + 2][w]);
+        rh = max(rh, red[4 * d + 3][w]);
+      }
+      if (l > h) fits = false;  // no valid particle
+      rot[d] = (rh - rl) < (h - l);
+      // unwrapped cell coordinates: in the rotated frame, cells c >= n - n/2
+      // are taken as c - n
+      const int ul = rot[d] ? rl - (n[d] >> 1) : l, uh = rot[d] ? rh - (n[d] >> 1) : h;
+      wlo[d] = ul - 1;
+      ext[d] = uh - ul + 4;
+      if (ext[d] > kVW) fits = false;
+    }
+  }
+  if (fits) {
+    for (int e = tid; e < 6 * ext[0] * ext[1]; e += kVjpThreads) {
+      const int x = e % ext[0], r = e / ext[0];
+      win[r * kVW + x] = 0.0;  // r = c * ext[1] + y -> row (c, y) of the window
+    }
+  }
+  __syncthreads();
+
+  // ---- pass 2: per-particle VJP
+  using D11 = Dual<11, R>;
+  using D2  = Dual<2, R>;
+  for (int64_t p = beg + tid; p < end; p += kVjpThreads) {
+    const int64_t q = adj_index(ids, p, min(pre.size, post.size), err);
+    if (q < 0) continue;
+    R a7[7] = { R(0), R(0), R(0), R(0), R(0), R(0), R(0) };
+    if constexpr (DEP) {
+      if (Q.tag[p] == NTT_TAG_ALIVE) deposit_vjp_acc<O, R>(Q, p, lj, charge, inv_dt, a7);
+    }
+    if (!valid(p)) {
+      pre.lu1[q] = pre.lu2[q] = pre.lu3[q] = R(0);
+      if constexpr (DEP) {
+        pre.lx1[q] = a7[0];
+        pre.lx2[q] = a7[1];
+      }
+      continue;
+    }
+    const int i = P.i1[p] + ngh, j = P.i2[p] + ngh;
+    const D2  x1 = D2::var(static_cast<R>(P.dx1[p]), 0), x2 = D2::var(static_cast<R>(P.dx2[p]), 1);
+    // gather with position derivatives: e_c, b_c as functions of (dx1, dx2)
+    D2  S1p[O + 1], S1d[O + 1], S2p[O + 1], S2d[O + 1];
+    int ip, id, jp, jd;
+    shape_t<false, O, D2, R>(i, x1, ip, S1p);
+    shape_t<false, O, D2, R>(j, x2, jp, S2p);
+    shape_t<true, O, D2, R>(i, x1, id, S1d);
+    shape_t<true, O, D2, R>(j, x2, jd, S2d);
+    auto gather = [&](int c, int imin, int jmin, const D2* A, const D2* B) -> D2 {
+      D2 e(R(0));
+      for (int b = 0; b < O + 1; b++) {
+        D2 c0(R(0));
+        for (int a = 0; a < O + 1; a++) c0 = c0 + A[a] * static_cast<R>(F(c, imin + a, jmin + b));
+        e = e + c0 * B[b];
+      }
+      return e;
+    };
+    D2 g[6];
+    g[0] = gather(0, id, jp, S1d, S2p);
+    g[1] = gather(1, ip, jd, S1p, S2d);
+    g[2] = gather(2, ip, jp, S1p, S2p);
+    g[3] = gather(3, ip, jd, S1p, S2d);
+    g[4] = gather(4, id, jp, S1d, S2p);
+    g[5] = gather(5, id, jd, S1d, S2d);
+    // Boris + position update over the 11 inputs
+    D11 u[3] = { D11::var(static_cast<R>(P.ux1[p]), 2), D11::var(static_cast<R>(P.ux2[p]), 3),
+                 D11::var(static_cast<R>(P.ux3[p]), 4) };
+    D11 e[3], b[3];
+    for (int c = 0; c < 3; ++c) {
+      e[c] = D11::var(g[c].v, 5 + c);
+      b[c] = D11::var(g[3 + c].v, 8 + c);
+    }
+    for (int c = 0; c < 2; ++c) {
+      e[c] = e[c] * dxm;
+      b[c] = b[c] * dxm;
+    }
+    boris_t<D11, R>(ndth, u, e, b);
+    const D11 dtie = dt / tsqrt(R(1) + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+    const D11 X1 = D11::var(static_cast<R>(P.dx1[p]), 0) + (u[0] / D11(dxm)) * dtie;
+    const D11 X2 = D11::var(static_cast<R>(P.dx2[p]), 1) + (u[1] / D11(dxm)) * dtie;
+    const R lo5[5] = { post.lx1[q] + a7[2], post.lx2[q] + a7[3], post.lu1[q] + a7[4],
+                       post.lu2[q] + a7[5], post.lu3[q] + a7[6] };
+    R lin[11];
+    for (int k = 0; k < 11; ++k)
+      lin[k] = lo5[0] * X1.d[k] + lo5[1] * X2.d[k] + lo5[2] * u[0].d[k] + lo5[3] * u[1].d[k] +
+               lo5[4] * u[2].d[k];
+    // adjoint of the gather: the position derivative of the gathered values,
+    // and a scatter into the field adjoint
+    R lx1 = lin[0], lx2 = lin[1];
+    for (int c = 0; c < 6; ++c) {
+      const R l = lin[5 + c];
+      lx1 += l * g[c].d[0];
+      lx2 += l * g[c].d[1];
+    }
+    // cell offset of this particle's unwrapped frame (0 or -n)
+    const int sh1 = (rot[0] && P.i1[p] >= n1 - (n1 >> 1)) ? -n1 : 0;
+    const int sh2 = (rot[1] && P.i2[p] >= n2 - (n2 >> 1)) ? -n2 : 0;
+    auto scatter = [&](int c, int imin, int jmin, const D2* A, const D2* B, R l) {
+      if (l == R(0)) return;
+      for (int bb = 0; bb < O + 1; bb++)
+        for (int a = 0; a < O + 1; a++) {
+          const R   v  = l * A[a].v * B[bb].v;
+          const int ia = imin + a - ngh, ja = jmin + bb - ngh;  // active cell (unwrapped)
+          if (fits) {
+            const int x = ia + sh1 - wlo[0], y = ja + sh2 - wlo[1];
+            atomicAdd(&win[(c * ext[1] + y) * kVW + x], static_cast<double>(v));
+          } else {
+            lem.add(c, ia, ja, v);
+          }
+        }
+    };
+    scatter(0, id, jp, S1d, S2p, lin[5]);
+    scatter(1, ip, jd, S1p, S2d, lin[6]);
+    scatter(2, ip, jp, S1p, S2p, lin[7]);
+    scatter(3, ip, jd, S1p, S2d, lin[8]);
+    scatter(4, id, jp, S1d, S2p, lin[9]);
+    scatter(5, id, jd, S1d, S2d, lin[10]);
+    if constexpr (DEP) {
+      pre.lx1[q] = a7[0] + lx1;
+      pre.lx2[q] = a7[1] + lx2;
+    } else {
+      pre.lx1[q] += lx1;
+      pre.lx2[q] += lx2;
+    }
+    pre.lu1[q] = lin[2];
+    pre.lu2[q] = lin[3];
+    pre.lu3[q] = lin[4];
+  }
+  if (!fits) return;
+  __syncthreads();
+  // ---- flush: one global atomic per non-zero window entry
+  for (int e = tid; e < 6 * ext[0] * ext[1]; e += kVjpThreads) {
+    const int    x = e % ext[0], r = e / ext[0];
+    const double v = win[r * kVW + x];
+    if (v == 0.0) continue;
+    const int c = r / ext[1], y = r - c * ext[1];
+    lem.add(c, wlo[0] + x, wlo[1] + y, static_cast<R>(v));
+  }
 }
 
 // ---- field VJPs (2D, periodic) -------------------------------------------------
@@ -462,11 +636,15 @@ __global__ void k_filter_vjp(AdjField<R> src, AdjField<R> dst) {
 template <typename R>
 PAdj<R> padj(const ntt_particle_adjoint_t* a) {
   return PAdj<R>{ static_cast<R*>(a->lx1), static_cast<R*>(a->lx2), static_cast<R*>(a->lu1),
-                  static_cast<R*>(a->lu2), static_cast<R*>(a->lu3) };
+                  static_cast<R*>(a->lu2), static_cast<R*>(a->lu3), a->size };
 }
 
-bool padj_ok(const ntt_particle_adjoint_t* a) {
-  return a && a->lx1 && a->lx2 && a->lu1 && a->lu2 && a->lu3;
+// every array present (positions only when lx_only), and room for every
+// particle when they are indexed by position
+bool padj_ok(const ntt_particle_adjoint_t* a, const ntt_particles_t* p, bool lx_only = false) {
+  if (!a || !a->lx1 || !a->lx2 || a->size < 0) return false;
+  if (!lx_only && (!a->lu1 || !a->lu2 || !a->lu3)) return false;
+  return p->pld_i != nullptr || a->size >= p->npart;
 }
 
 }  // namespace
@@ -480,7 +658,8 @@ extern "C" ntt_status ntt_sr_push_vjp(const ntt_sr_pusher_ctx_t* ctx, const ntt_
                                       const ntt_particle_adjoint_t* post_adj,
                                       ntt_particle_adjoint_t* pre_adj, void* lam_em,
                                       void* stream) {
-  if (!ctx || !pre || !em || !metric || !padj_ok(post_adj) || !padj_ok(pre_adj) || !lam_em)
+  if (!ctx || !pre || !em || !metric || !lam_em || !padj_ok(post_adj, pre) ||
+      !padj_ok(pre_adj, pre))
     return NTT_ERR_ARG;
   if (em->dim != 2 || metric->dim != 2) return NTT_ERR_UNSUPPORTED;
   if (shape_order != 1 && shape_order != 2) return NTT_ERR_UNSUPPORTED;
@@ -493,37 +672,39 @@ extern "C" ntt_status ntt_sr_push_vjp(const ntt_sr_pusher_ctx_t* ctx, const ntt_
   s = make_field_view(em, 6, &F);
   if (s != NTT_OK) return s;
   if (P.n == 0) return NTT_OK;
-  const EmView ev{ F.d, F.N1, F.N2, F.cstride };
-  hipStream_t st = as_stream(stream);
-  const unsigned grid = grid_for(P.n, 256);
+  const EmView     ev{ F.d, F.N1, F.N2, F.cstride };
+  const int64_t*   ids = static_cast<const int64_t*>(pre->pld_i);
+  uint32_t*        err = device_error_word();
+  hipStream_t      st  = as_stream(stream);
+  const unsigned   grid = grid_for(P.n, kVjpChunk);
   // sr.hpp:111: normalized_dt_half = 1/2 (q/m) omegaB0 dt
   const float ndth = HALF * (ctx->charge / ctx->mass) * ctx->omegaB0 * ctx->dt;
-  if (precision == 64) {
-    AdjField<double> L{ static_cast<double*>(lam_em), F.n1, F.n2 };
+  auto run = [&](auto tag) {
+    using R = decltype(tag);
+    AdjField<R> L{ static_cast<R*>(lam_em), F.n1, F.n2 };
+    const auto  po = padj<R>(post_adj), pr = padj<R>(pre_adj);
+    const AdjField<R> none{ nullptr, 1, 1 };
     if (shape_order == 1)
-      k_push_vjp<1, double><<<grid, 256, 0, st>>>(P, ev, F.ngh, ndth, ctx->dt, metric->dx,
-                                                  padj<double>(post_adj), padj<double>(pre_adj), L);
+      k_push_vjp<1, R, false><<<grid, kVjpThreads, 0, st>>>(P, ids, ev, F.ngh, ndth, ctx->dt,
+                                                            metric->dx, po, pr, L, err, P, none,
+                                                            R(0), R(0));
     else
-      k_push_vjp<2, double><<<grid, 256, 0, st>>>(P, ev, F.ngh, ndth, ctx->dt, metric->dx,
-                                                  padj<double>(post_adj), padj<double>(pre_adj), L);
-  } else {
-    AdjField<float> L{ static_cast<float*>(lam_em), F.n1, F.n2 };
-    if (shape_order == 1)
-      k_push_vjp<1, float><<<grid, 256, 0, st>>>(P, ev, F.ngh, ndth, ctx->dt, metric->dx,
-                                                 padj<float>(post_adj), padj<float>(pre_adj), L);
-    else
-      k_push_vjp<2, float><<<grid, 256, 0, st>>>(P, ev, F.ngh, ndth, ctx->dt, metric->dx,
-                                                 padj<float>(post_adj), padj<float>(pre_adj), L);
-  }
+      k_push_vjp<2, R, false><<<grid, kVjpThreads, 0, st>>>(P, ids, ev, F.ngh, ndth, ctx->dt,
+                                                            metric->dx, po, pr, L, err, P, none,
+                                                            R(0), R(0));
+  };
+  if (precision == 64) run(double{});
+  else run(float{});
   return check_launch();
 }
 
 extern "C" ntt_status ntt_deposit_vjp(const ntt_particles_t* post, const int32_t n[2],
                                       float charge, float dt, int32_t shape_order,
                                       int32_t precision, const void* lam_j,
-                                      ntt_particle_adjoint_t* post_adj, void* lx_prev1,
-                                      void* lx_prev2, void* stream) {
-  if (!post || !n || !lam_j || !padj_ok(post_adj) || !lx_prev1 || !lx_prev2) return NTT_ERR_ARG;
+                                      ntt_particle_adjoint_t* post_adj,
+                                      ntt_particle_adjoint_t* pre_adj, void* stream) {
+  if (!post || !n || !lam_j || !padj_ok(post_adj, post) || !padj_ok(pre_adj, post, true))
+    return NTT_ERR_ARG;
   if (shape_order != 1 && shape_order != 2) return NTT_ERR_UNSUPPORTED;
   if (precision != 32 && precision != 64) return NTT_ERR_ARG;
   if (n[0] < 1 || n[1] < 1 || !(dt > 0.0f)) return NTT_ERR_ARG;
@@ -531,31 +712,78 @@ extern "C" ntt_status ntt_deposit_vjp(const ntt_particles_t* post, const int32_t
   ntt_status s = make_particle_view(post, 2, true, &P);
   if (s != NTT_OK) return s;
   if (P.n == 0) return NTT_OK;
-  hipStream_t st = as_stream(stream);
+  const int64_t* ids  = static_cast<const int64_t*>(post->pld_i);
+  uint32_t*      err  = device_error_word();
+  hipStream_t    st   = as_stream(stream);
   const unsigned grid = grid_for(P.n, 256);
-  if (precision == 64) {
-    AdjField<double> L{ const_cast<double*>(static_cast<const double*>(lam_j)), n[0], n[1] };
-    auto pa = padj<double>(post_adj);
+  auto run = [&](auto tag) {
+    using R = decltype(tag);
+    AdjField<R> L{ const_cast<R*>(static_cast<const R*>(lam_j)), n[0], n[1] };
+    const R     inv_dt = R(1) / static_cast<R>(dt);
+    // pre_adj's momentum arrays are not touched (they may be null)
+    PAdj<R> pr{ static_cast<R*>(pre_adj->lx1), static_cast<R*>(pre_adj->lx2), nullptr, nullptr,
+                nullptr, pre_adj->size };
     if (shape_order == 1)
-      k_deposit_vjp<1, double><<<grid, 256, 0, st>>>(P, L, charge, 1.0 / dt, pa,
-                                                     static_cast<double*>(lx_prev1),
-                                                     static_cast<double*>(lx_prev2));
+      k_deposit_vjp<1, R><<<grid, 256, 0, st>>>(P, ids, L, charge, inv_dt, padj<R>(post_adj), pr,
+                                                err);
     else
-      k_deposit_vjp<2, double><<<grid, 256, 0, st>>>(P, L, charge, 1.0 / dt, pa,
-                                                     static_cast<double*>(lx_prev1),
-                                                     static_cast<double*>(lx_prev2));
-  } else {
-    AdjField<float> L{ const_cast<float*>(static_cast<const float*>(lam_j)), n[0], n[1] };
-    auto pa = padj<float>(post_adj);
+      k_deposit_vjp<2, R><<<grid, 256, 0, st>>>(P, ids, L, charge, inv_dt, padj<R>(post_adj), pr,
+                                                err);
+  };
+  if (precision == 64) run(double{});
+  else run(float{});
+  return check_launch();
+}
+
+extern "C" ntt_status ntt_sr_particle_vjp(const ntt_sr_pusher_ctx_t* ctx,
+                                          const ntt_particles_t* pre, const ntt_particles_t* post,
+                                          const ntt_field_t* em, const ntt_minkowski_t* metric,
+                                          int32_t shape_order, int32_t precision,
+                                          const void* lam_j,
+                                          const ntt_particle_adjoint_t* post_adj,
+                                          ntt_particle_adjoint_t* pre_adj, void* lam_em,
+                                          void* stream) {
+  if (!ctx || !pre || !post || !em || !metric || !lam_j || !lam_em || !padj_ok(post_adj, pre) ||
+      !padj_ok(pre_adj, pre))
+    return NTT_ERR_ARG;
+  if (pre->npart != post->npart) return NTT_ERR_ARG;
+  if (em->dim != 2 || metric->dim != 2) return NTT_ERR_UNSUPPORTED;
+  if (shape_order != 1 && shape_order != 2) return NTT_ERR_UNSUPPORTED;
+  if (ctx->pusher_flags != NTT_PUSHER_BORIS) return NTT_ERR_UNSUPPORTED;
+  if (precision != 32 && precision != 64) return NTT_ERR_ARG;
+  if (!(ctx->dt > 0.0f)) return NTT_ERR_ARG;
+  ParticleView P, Q;
+  ntt_status s = make_particle_view(pre, 2, true, &P);
+  if (s != NTT_OK) return s;
+  s = make_particle_view(post, 2, true, &Q);
+  if (s != NTT_OK) return s;
+  FieldView F;
+  s = make_field_view(em, 6, &F);
+  if (s != NTT_OK) return s;
+  if (P.n == 0) return NTT_OK;
+  const EmView   ev{ F.d, F.N1, F.N2, F.cstride };
+  const int64_t* ids  = static_cast<const int64_t*>(pre->pld_i);
+  uint32_t*      err  = device_error_word();
+  hipStream_t    st   = as_stream(stream);
+  const unsigned grid = grid_for(P.n, kVjpChunk);
+  const float    ndth = HALF * (ctx->charge / ctx->mass) * ctx->omegaB0 * ctx->dt;
+  auto run = [&](auto tag) {
+    using R = decltype(tag);
+    AdjField<R> L{ static_cast<R*>(lam_em), F.n1, F.n2 };
+    AdjField<R> LJ{ const_cast<R*>(static_cast<const R*>(lam_j)), F.n1, F.n2 };
+    const auto  po = padj<R>(post_adj), pr = padj<R>(pre_adj);
+    const R     inv_dt = R(1) / static_cast<R>(ctx->dt);
     if (shape_order == 1)
-      k_deposit_vjp<1, float><<<grid, 256, 0, st>>>(P, L, charge, ONE / dt, pa,
-                                                    static_cast<float*>(lx_prev1),
-                                                    static_cast<float*>(lx_prev2));
+      k_push_vjp<1, R, true><<<grid, kVjpThreads, 0, st>>>(P, ids, ev, F.ngh, ndth, ctx->dt,
+                                                           metric->dx, po, pr, L, err, Q, LJ,
+                                                           static_cast<R>(ctx->charge), inv_dt);
     else
-      k_deposit_vjp<2, float><<<grid, 256, 0, st>>>(P, L, charge, ONE / dt, pa,
-                                                    static_cast<float*>(lx_prev1),
-                                                    static_cast<float*>(lx_prev2));
-  }
+      k_push_vjp<2, R, true><<<grid, kVjpThreads, 0, st>>>(P, ids, ev, F.ngh, ndth, ctx->dt,
+                                                           metric->dx, po, pr, L, err, Q, LJ,
+                                                           static_cast<R>(ctx->charge), inv_dt);
+  };
+  if (precision == 64) run(double{});
+  else run(float{});
   return check_launch();
 }
 

@@ -173,19 +173,41 @@ class SRPICEngine:
         return arr
 
     def step_forward(self, stream=None) -> None:
+        self.maybe_sort(stream)
+        self.advance(stream)
+
+    def maybe_sort(self, stream=None) -> bool:
+        """The tile re-sort that opens a step every sort_interval steps; True if it ran."""
         p = self.params
         if p.sort_interval and self.step > 0 and self.step % p.sort_interval == 0:
             self.sort(stream=stream)
-        em, cur, buff = (self.fields.abi(x) for x in ("em", "cur", "buff"))
-        sp = self.species_abi()
-        prm = self.step_params()
-        metric = _abi.NttMinkowski(p.dim, p.dx)
+            return True
+        return False
+
+    def _step_args(self):
+        """The ABI structures of ntt_sr_step, rebuilt only when a particle count,
+        a tile-offset array, the weight bound or a parameter has changed."""
+        key = (tuple(sp.npart for sp in self.species),
+               tuple(t.data_ptr() if t is not None else 0 for t in self.tile_offsets),
+               self.weight_max, tuple(self.params.__dict__.values()))
+        if getattr(self, "_args_key", None) != key:
+            p = self.params
+            self._args = (tuple(self.fields.abi(x) for x in ("em", "cur", "buff")),
+                          self.species_abi(), self.step_params(),
+                          _abi.NttMinkowski(p.dim, p.dx))
+            self._args_key = key
+        return self._args
+
+    def advance(self, stream=None) -> None:
+        """One ntt_sr_step (the step without its opening re-sort)."""
+        (em, cur, buff), sp, prm, metric = self._step_args()
+        prm.step = self.step
         _abi.check(self.lib.ntt_sr_step(C.byref(em), C.byref(cur), C.byref(buff), sp,
                                         len(self.species), C.byref(prm), C.byref(metric),
                                         self._step_ws.data_ptr(), self._step_ws.numel(),
                                         _stream(stream)), "ntt_sr_step")
         self.step += 1
-        self.time += p.dt
+        self.time += self.params.dt
 
     # -- spatial sort ----------------------------------------------------------------
     def sort(self, stream=None, compact: bool = False) -> None:

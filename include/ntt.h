@@ -354,10 +354,16 @@ NTT_API ntt_status ntt_sr_step_workspace_size(const ntt_field_t* cur, uint64_t* 
 
 /* Per-particle adjoints (d loss / d state) of a 2D species: offsets dx1,
  * dx2 (the cell index is held fixed) and momenta u1..3; float or double
- * arrays of npart entries, as the precision argument says. */
+ * arrays of `size` entries, as the precision argument says.
+ * Indexing: when the particles carry an ID payload (pld_i != NULL), particle
+ * p's adjoint is entry pld_i[p], so the adjoint follows a particle through
+ * re-sorts; IDs outside [0, size) skip the particle and raise
+ * NTT_DEVERR_OUT_OF_BOUNDS.  Without a payload it is entry p, and size must
+ * be at least npart. */
 typedef struct ntt_particle_adjoint {
   void *lx1, *lx2;
   void *lu1, *lu2, *lu3;
+  int64_t size;
 } ntt_particle_adjoint_t;
 
 #define NTT_VJP_FARADAY 0
@@ -387,13 +393,33 @@ NTT_API ntt_status ntt_sr_push_vjp(const ntt_sr_pusher_ctx_t*   ctx,
  * species: `post` holds the particles after the push (i_prev, dx_prev, i, dx,
  * u, weight), lam_j the adjoint of the folded current (3 components, active
  * cells, [comp][n2][n1]).  Adds to post_adj (positions and momenta after
- * the push) and writes the adjoint of the pre-push offsets to lx_prev1/2.
- * No reference counterpart. */
+ * the push) and writes the adjoint of the pre-push offsets to pre_adj->lx1/2
+ * (pre_adj->lu* are not used).  No reference counterpart. */
 NTT_API ntt_status ntt_deposit_vjp(const ntt_particles_t* post, const int32_t n[2],
                                    float charge, float dt, int32_t shape_order,
                                    int32_t precision, const void* lam_j,
-                                   ntt_particle_adjoint_t* post_adj, void* lx_prev1,
-                                   void* lx_prev2, void* stream);
+                                   ntt_particle_adjoint_t* post_adj,
+                                   ntt_particle_adjoint_t* pre_adj, void* stream);
+
+/* The deposit VJP and the push VJP of one species and one step in one pass:
+ * `pre` and `post` are the particles before and after that step's push (same
+ * order, same count).  Equivalent to ntt_deposit_vjp(post, ..., post_adj,
+ * pre_adj) followed by ntt_sr_push_vjp(pre, ..., post_adj, pre_adj), except
+ * that post_adj is only read (the deposit's contribution is carried in
+ * registers) and pre_adj->lx* are written, not accumulated.  The grid size
+ * and charge come from em and ctx.  No reference counterpart. */
+NTT_API ntt_status ntt_sr_particle_vjp(const ntt_sr_pusher_ctx_t*    ctx,
+                                       const ntt_particles_t*        pre,
+                                       const ntt_particles_t*        post,
+                                       const ntt_field_t*            em,
+                                       const ntt_minkowski_t*        metric,
+                                       int32_t                       shape_order,
+                                       int32_t                       precision,
+                                       const void*                   lam_j,
+                                       const ntt_particle_adjoint_t* post_adj,
+                                       ntt_particle_adjoint_t*       pre_adj,
+                                       void*                         lam_em,
+                                       void*                         stream);
 
 /* Transposed 2D periodic field operators on active-cell adjoints
  * ([comp][n2][n1], 3 components each):

@@ -132,7 +132,7 @@ def _particles(n, npart, rng, spread=0.45):
 
 def _padj(t5):
     from entity_amd import _abi
-    return _abi.NttParticleAdjoint(*(x.data_ptr() for x in t5))
+    return _abi.NttParticleAdjoint(*(x.data_ptr() for x in t5), t5[0].numel())
 
 
 @pytest.mark.parametrize("order", [1, 2])
@@ -268,19 +268,97 @@ def test_deposit_vjp_matches_finite_differences(hip, order):
     fd = float((lamJ * (run(h) - run(-h))).sum()) / (2 * h)
     p = fill(Particles(2, npart, device="cuda"), 0.0)
     post = [torch.zeros(npart, dtype=torch.float64, device="cuda") for _ in range(5)]
-    lxp = [torch.zeros(npart, dtype=torch.float64, device="cuda") for _ in range(2)]
+    pre = [torch.zeros(npart, dtype=torch.float64, device="cuda") for _ in range(5)]
     lj = torch.from_numpy(lamJ).cuda()
     lib = _abi.load(False)
     nn = (C.c_int32 * 2)(*n)
     _abi.check(lib.ntt_deposit_vjp(C.byref(p.abi()), nn, charge, dt, order, 64, lj.data_ptr(),
-                                   C.byref(_padj(post)), lxp[0].data_ptr(), lxp[1].data_ptr(),
-                                   _stream()), "ntt_deposit_vjp")
+                                   C.byref(_padj(post)), C.byref(_padj(pre)), _stream()),
+               "ntt_deposit_vjp")
     post = [x.cpu().numpy() for x in post]
-    lxp = [x.cpu().numpy() for x in lxp]
+    lxp = [x.cpu().numpy() for x in pre[:2]]
     ad = float((lxp[0] * v[0]).sum() + (lxp[1] * v[1]).sum() + (post[0] * v[2]).sum() +
                (post[1] * v[3]).sum())
     ad += sum(float((post[2 + c] * v_u[c]).sum()) for c in range(3))
     assert abs(ad - fd) <= 2e-3 * max(abs(fd), 1.0), (ad, fd)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_fused_particle_vjp_and_ids(hip, order):
+    """ntt_sr_particle_vjp equals ntt_deposit_vjp followed by ntt_sr_push_vjp;
+    with an ID payload the adjoint arrays are indexed by ID (a permutation of
+    the particles gives the permuted adjoint); an out-of-range ID is reported."""
+    from entity_amd import _abi
+    from entity_amd.containers import GridShape, Particles, field_abi
+    from entity_amd.kernels import (Minkowski, PusherBoundaries, PusherContext,
+                                    communicate_fields, device_errors, sr)
+    n = (24, 16)
+    shape = GridShape(2, n)
+    rng = np.random.default_rng(40 + order)
+    npart = 3000
+    pre = _particles(n, npart, rng)
+    em = torch.from_numpy((rng.standard_normal((6, n[1] + 2 * G, n[0] + 2 * G)) * 0.2)
+                          .astype(np.float32)).cuda()
+    communicate_fields(em, shape, 0, 6)
+    ctx = PusherContext(_abi.PUSHER_BORIS, 1.0, -1.0, 0.35, 2.0, n[0], n[1])
+    metric = Minkowski(2, 1.0)
+    post = Particles(2, npart, device="cuda").load(**{k: v for k, v in pre.to_host().items()})
+    sr.Pusher_kernel(ctx, PusherBoundaries(), post, em, shape, metric, order)()
+    lam_j = torch.from_numpy(rng.standard_normal((3, n[1], n[0]))).cuda()
+    lam_post = [torch.from_numpy(rng.standard_normal(npart)).cuda() for _ in range(5)]
+    lib = _abi.load(False)
+    nn = (C.c_int32 * 2)(*n)
+    fe = field_abi(em, shape)
+
+    def zeros(k):
+        return [torch.zeros(npart, dtype=torch.float64, device="cuda") for _ in range(k)]
+
+    # separate: deposit VJP (adds to a copy of lam_post), then push VJP
+    post_sep = [x.clone() for x in lam_post]
+    pre_sep, lem_sep = zeros(5), torch.zeros((6, n[1], n[0]), dtype=torch.float64, device="cuda")
+    _abi.check(lib.ntt_deposit_vjp(C.byref(post.abi()), nn, -1.0, 0.35, order, 64,
+                                   lam_j.data_ptr(), C.byref(_padj(post_sep)),
+                                   C.byref(_padj(pre_sep)), _stream()), "ntt_deposit_vjp")
+    _abi.check(lib.ntt_sr_push_vjp(C.byref(ctx.abi()), C.byref(pre.abi()), C.byref(fe),
+                                   C.byref(metric.abi()), order, 64, C.byref(_padj(post_sep)),
+                                   C.byref(_padj(pre_sep)), lem_sep.data_ptr(), _stream()),
+               "ntt_sr_push_vjp")
+    # fused
+    pre_fus, lem_fus = zeros(5), torch.zeros_like(lem_sep)
+    _abi.check(lib.ntt_sr_particle_vjp(C.byref(ctx.abi()), C.byref(pre.abi()), C.byref(post.abi()),
+                                       C.byref(fe), C.byref(metric.abi()), order, 64,
+                                       lam_j.data_ptr(), C.byref(_padj(lam_post)),
+                                       C.byref(_padj(pre_fus)), lem_fus.data_ptr(), _stream()),
+               "ntt_sr_particle_vjp")
+    for a, b in zip(pre_sep, pre_fus):
+        np.testing.assert_allclose(b.cpu().numpy(), a.cpu().numpy(), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(lem_fus.cpu().numpy(), lem_sep.cpu().numpy(), rtol=1e-11,
+                               atol=1e-11 * float(lem_sep.abs().max()))
+    # IDs: particle k carries ID perm[k]; adjoints live at their IDs
+    perm = torch.from_numpy(rng.permutation(npart)).cuda()
+    pre.pld_i = perm.clone()
+    post.pld_i = perm.clone()
+    lam_post_id = [torch.empty_like(x) for x in lam_post]
+    for a, b in zip(lam_post_id, lam_post):
+        a[perm] = b
+    pre_id, lem_id = zeros(5), torch.zeros_like(lem_sep)
+    _abi.check(lib.ntt_sr_particle_vjp(C.byref(ctx.abi()), C.byref(pre.abi()), C.byref(post.abi()),
+                                       C.byref(fe), C.byref(metric.abi()), order, 64,
+                                       lam_j.data_ptr(), C.byref(_padj(lam_post_id)),
+                                       C.byref(_padj(pre_id)), lem_id.data_ptr(), _stream()),
+               "ntt_sr_particle_vjp")
+    for a, b in zip(pre_fus, pre_id):
+        np.testing.assert_allclose(b[perm].cpu().numpy(), a.cpu().numpy(), rtol=1e-12, atol=1e-12)
+    assert device_errors() == 0
+    # an ID beyond the adjoint arrays: that particle is skipped and reported
+    pre.pld_i[5] = npart + 3
+    pre_bad = zeros(5)
+    _abi.check(lib.ntt_sr_particle_vjp(C.byref(ctx.abi()), C.byref(pre.abi()), C.byref(post.abi()),
+                                       C.byref(fe), C.byref(metric.abi()), order, 64,
+                                       lam_j.data_ptr(), C.byref(_padj(lam_post_id)),
+                                       C.byref(_padj(pre_bad)), lem_id.data_ptr(), _stream()),
+               "ntt_sr_particle_vjp")
+    assert device_errors() & _abi.DEVERR_OUT_OF_BOUNDS
 
 
 def test_particle_vjps_reject_bad_arguments(hip):
@@ -288,14 +366,21 @@ def test_particle_vjps_reject_bad_arguments(hip):
     from entity_amd.containers import Particles
     lib = _abi.load(False)
     p = Particles(2, 8, device="cuda")
+    p.npart = 8
     a = [torch.zeros(8, device="cuda") for _ in range(5)]
+    b = [torch.zeros(8, device="cuda") for _ in range(5)]
     nn = (C.c_int32 * 2)(8, 8)
     lj = torch.zeros(3 * 64, device="cuda")
     st = lib.ntt_deposit_vjp(C.byref(p.abi()), nn, 1.0, 0.1, 3, 32, lj.data_ptr(),
-                             C.byref(_padj(a)), a[0].data_ptr(), a[1].data_ptr(), None)
+                             C.byref(_padj(a)), C.byref(_padj(b)), None)
     assert st == _abi.NTT_ERR_UNSUPPORTED
     st = lib.ntt_deposit_vjp(C.byref(p.abi()), nn, 1.0, 0.1, 1, 32, None,
-                             C.byref(_padj(a)), a[0].data_ptr(), a[1].data_ptr(), None)
+                             C.byref(_padj(a)), C.byref(_padj(b)), None)
+    assert st == _abi.NTT_ERR_ARG
+    # adjoint arrays indexed by position must hold every particle
+    short = _abi.NttParticleAdjoint(*(x.data_ptr() for x in a), 4)
+    st = lib.ntt_deposit_vjp(C.byref(p.abi()), nn, 1.0, 0.1, 1, 32, lj.data_ptr(),
+                             C.byref(short), C.byref(_padj(b)), None)
     assert st == _abi.NTT_ERR_ARG
 
 
@@ -304,15 +389,16 @@ N_BOX = (32, 24)
 SIGN = (1.0, -1.0, 0.0)
 
 
-def _weibel(ud, b, order=2, n_filters=1):
+def _weibel(ud, b, order=2, n_filters=1, sort_interval=0):
     """Two counter-streaming electron beams (u_x shifted by +-u_d) over a
-    positron background, with a seed B_z = b cos(2 pi y / Ly)."""
+    positron background, with a seed B_z = b cos(2 pi y / Ly).  With a
+    re-sort interval the particles carry IDs."""
     from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
     prm = SRParams(dim=2, n=N_BOX, shape_order=order, n_filters=n_filters, ppc0=8.0,
-                   larmor0=1.0, skindepth0=3.0, sort_interval=0, tile=(8, 8))
+                   larmor0=1.0, skindepth0=3.0, sort_interval=sort_interval, tile=(8, 8))
     sp = [SpeciesSpec("e1", 1.0, -1.0), SpeciesSpec("e2", 1.0, -1.0), SpeciesSpec("p", 1.0, 1.0)]
     ppc = 4
-    eng = SRPICEngine(prm, sp, N_BOX[0] * N_BOX[1] * ppc)
+    eng = SRPICEngine(prm, sp, N_BOX[0] * N_BOX[1] * ppc, with_payload=sort_interval > 0)
     eng.inject_uniform(ppc, temperature=0.01, seed=5)
     for s, sp in zip(SIGN, eng.species):   # u_x += sign * u_d on a thermal plasma
         sp.ux1[: sp.npart] += s * ud
@@ -366,6 +452,22 @@ def test_adjoint_precisions_and_checkpoints_agree(hip):
     assert f32[2] == pytest.approx(ref[2], rel=1e-3)
 
 
+def test_gradient_through_resorts(hip):
+    """Re-sorting every 2 steps (inside segments and at their starts)
+    permutes the particles; with IDs the adjoint follows them, and since the
+    forward does not depend on particle order
+    (integer current sums) the loss is identical and the gradients agree to
+    round-off with the run that never re-sorts."""
+    from entity_amd.kernels import device_errors
+    ud, b, nsteps = 0.4, 0.05, 10
+    ref = _grads(ud, b, nsteps, 4, 64)
+    srt = _grads(ud, b, nsteps, 4, 64, sort_interval=2)
+    assert device_errors() == 0
+    assert srt[0] == ref[0]
+    assert srt[1] == pytest.approx(ref[1], rel=1e-9)
+    assert srt[2] == pytest.approx(ref[2], rel=1e-9)
+
+
 def test_autograd_function(hip):
     from entity_amd.adjoint import FieldEnergyLoss
 
@@ -393,5 +495,5 @@ def test_adjoint_rejects_unsupported_setups(hip):
         SRAdjoint(eng, 2)
     prm = SRParams(dim=2, n=(16, 16), shape_order=1, sort_interval=4)
     eng = SRPICEngine(prm, [SpeciesSpec("e", 1.0, -1.0)], 16)
-    with pytest.raises(ValueError):
+    with pytest.raises(ValueError):   # re-sorting without particle IDs
         SRAdjoint(eng, 2)
