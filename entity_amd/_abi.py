@@ -104,7 +104,7 @@ class NttBox(C.Structure):
     _fields_ = [("em", NttField), ("cur", NttField), ("buff", NttField),
                 ("species", C.POINTER(NttSpecies)), ("nspecies", C.c_int32),
                 ("prm", NttSrStepParams), ("metric", NttMinkowski), ("workspace", C.c_void_p),
-                ("workspace_bytes", C.c_uint64)]
+                ("workspace_bytes", C.c_uint64), ("tile", C.c_int32 * 3)]
 
 
 class NttPlasmaInit(C.Structure):
@@ -187,6 +187,7 @@ SIGNATURES: dict[str, tuple] = {
     "ntt_sr_step_workspace_size": (C.c_int, [_P(NttField), _P(C.c_uint64)]),
     "ntt_ensemble_create": (C.c_int, [_P(NttBox), C.c_int32, _P(C.c_void_p)]),
     "ntt_ensemble_step": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
+    "ntt_ensemble_sort": (C.c_int, [C.c_void_p, C.c_void_p]),
     "ntt_ensemble_destroy": (C.c_int, [C.c_void_p]),
     "ntt_sort_workspace_size": (C.c_int, [C.c_int64, C.c_int32, _P(C.c_int32), _P(C.c_int32),
                                           _P(C.c_uint64)]),

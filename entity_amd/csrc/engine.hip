@@ -245,6 +245,13 @@ struct ntt_ensemble {
   };
   std::vector<FieldView> ghost_E;   // step 0: E/B ghost fill per box
   Seg                    far, zero, jfix, amp, copy;
+  // batched tile re-sort of every (box, species) whose box names a tile
+  Seg                    sort;      // device SortSeg[n]
+  int64_t*               sort_pbase = nullptr;
+  int64_t*               sort_kbase = nullptr;
+  int64_t*               sort_bbase = nullptr;
+  int64_t                sort_ptot = 0, sort_ktot = 0, sort_blocks = 0;
+  void*                  sort_ws = nullptr;
   std::vector<Seg>       filter_pass;
   std::vector<Group>     groups;
   std::vector<void*>     allocations;
@@ -293,6 +300,8 @@ extern "C" ntt_status ntt_ensemble_create(const ntt_box_t* boxes, int32_t nboxes
     std::vector<int64_t> chunk0;
   };
   std::vector<GroupHost> groups;
+  std::vector<SortSeg>   sorts;
+  std::vector<int64_t>   pbase{ 0 }, kbase{ 0 }, bbase{ 0 };
   const size_t asz = fused_args_size();
   ntt_status s = NTT_OK;
   for (int b = 0; b < nboxes && s == NTT_OK; ++b) {
@@ -363,6 +372,28 @@ extern "C" ntt_status ntt_ensemble_create(const ntt_box_t* boxes, int32_t nboxes
         nb_copy += flat_seg_blocks(n);
       }
     }
+    if (B.tile[0] > 0) {
+      const int32_t n3[3]    = { B.em.n[0], B.em.n[1], P.D == 3 ? B.em.n[2] : 1 };
+      const int32_t tile3[3] = { B.tile[0], B.tile[1], P.D == 3 ? B.tile[2] : 1 };
+      for (int sp = 0; sp < B.nspecies && s == NTT_OK; ++sp) {
+        SortSeg g{};
+        // ntt_ensemble_sort rewrites the offsets the species hands to the step
+        s = make_sort_seg(&B.species[sp].prtls, P.D, n3, tile3,
+                          const_cast<uint32_t*>(B.species[sp].tile_offsets), &g);
+        if (s != NTT_OK) break;
+        // the step's chunking reads these offsets: they must cover every tile
+        if (B.species[sp].tile_offsets != nullptr &&
+            B.species[sp].ntiles != static_cast<int64_t>(g.dead_key)) {
+          s = NTT_ERR_ARG;
+          break;
+        }
+        sorts.push_back(g);
+        pbase.push_back(pbase.back() + g.n);
+        kbase.push_back(kbase.back() + static_cast<int64_t>(g.dead_key) + 1);
+        bbase.push_back(bbase.back() + sort_blocks(g.n));
+      }
+      if (s != NTT_OK) break;
+    }
     const int ppt = prm.particles_per_thread > 0 ? prm.particles_per_thread : 4;
     for (int sp = 0; sp < B.nspecies && s == NTT_OK; ++sp) {
       const Prep& p = P.prep[sp];
@@ -395,6 +426,26 @@ extern "C" ntt_status ntt_ensemble_create(const ntt_box_t* boxes, int32_t nboxes
     ntt_ensemble::Seg seg;
     s = upload_segs(e, passes[f], nb_pass[f], &seg);
     e->filter_pass.push_back(seg);
+  }
+  if (s == NTT_OK && !sorts.empty()) {
+    s = upload_segs(e, sorts, 0, &e->sort);
+    void* pb = nullptr;
+    void* kb = nullptr;
+    if (s == NTT_OK) s = upload(e, pbase.data(), pbase.size() * sizeof(int64_t), &pb);
+    void* bb = nullptr;
+    if (s == NTT_OK) s = upload(e, kbase.data(), kbase.size() * sizeof(int64_t), &kb);
+    if (s == NTT_OK) s = upload(e, bbase.data(), bbase.size() * sizeof(int64_t), &bb);
+    e->sort_pbase  = static_cast<int64_t*>(pb);
+    e->sort_kbase  = static_cast<int64_t*>(kb);
+    e->sort_bbase  = static_cast<int64_t*>(bb);
+    e->sort_ptot   = pbase.back();
+    e->sort_ktot   = kbase.back();
+    e->sort_blocks = bbase.back();
+    if (s == NTT_OK) {
+      const int64_t bytes = sort_batched_bytes(e->sort_ptot, e->sort_ktot);
+      if (hipMalloc(&e->sort_ws, bytes) != hipSuccess) s = NTT_ERR_HIP;
+      else e->allocations.push_back(e->sort_ws);
+    }
   }
   for (auto& gh : groups) {
     if (s != NTT_OK) break;
@@ -440,6 +491,16 @@ extern "C" ntt_status ntt_ensemble_step(ntt_ensemble_t e, int64_t step, void* st
   flat(FLAT_COPY_F32, e->copy);
   fields(SEG_FARADAY, e->far);
   fields(SEG_AMPERE, e->amp);
+  return check_launch();
+}
+
+extern "C" ntt_status ntt_ensemble_sort(ntt_ensemble_t e, void* stream) {
+  if (e == nullptr) return NTT_ERR_ARG;
+  if (e->sort.n == 0) return NTT_OK;
+  ntt_status s = sort_batched(static_cast<const SortSeg*>(e->sort.dev), e->sort_pbase,
+                              e->sort_kbase, e->sort_bbase, e->sort_blocks, e->sort.n,
+                              e->sort_ptot, e->sort_ktot, e->sort_ws, as_stream(stream));
+  if (s != NTT_OK) return s;
   return check_launch();
 }
 

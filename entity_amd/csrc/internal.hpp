@@ -69,6 +69,30 @@ enum { SEG_FARADAY = 0, SEG_AMPERE = 1, SEG_FILTER = 2, SEG_JFIX = 3 };
 int64_t field_seg_blocks(const FieldView& F, int32_t* gx, int32_t* gy);
 void    launch_fields_batched(int op, const FieldSeg* segs, int nseg, int64_t nblocks,
                               hipStream_t st);
+// sort.hip: one tile sort over the species of many boxes at once.  Every
+// (box, species) is a segment: its particles occupy [pbase, pbase + n) of a
+// global index space and its keys [kbase, kbase + ntiles + 1) of a global key
+// space, so one counting sort over the union sorts every segment in place.
+constexpr int kSortArrays = 19;
+struct SortSeg {
+  void*     arr[kSortArrays];  // i1..3, i*_prev, dx1..3, dx*_prev, u1..3, weight, phi, tag, pld_i
+  int64_t   n;
+  int32_t   dim, ext[3], tile[3], nt[3];
+  uint32_t  dead_key;          // = ntiles
+  uint32_t* tile_offsets;      // ntiles + 1 entries, written (may be null)
+};
+ntt_status make_sort_seg(const ntt_particles_t* p, int32_t dim, const int32_t n[3],
+                         const int32_t tile[3], uint32_t* tile_offsets, SortSeg* out);
+// workspace of sort_batched for ptot particles and ktot keys in all
+int64_t    sort_batched_bytes(int64_t ptot, int64_t ktot);
+// workgroups a segment of n particles takes
+int64_t    sort_blocks(int64_t n);
+// segs, pbase, kbase, bbase: device arrays (nseg, then nseg + 1 prefix sums
+// of particles, keys and sort_blocks); nblocks = bbase[nseg]
+ntt_status sort_batched(const SortSeg* segs, const int64_t* pbase, const int64_t* kbase,
+                        const int64_t* bbase, int64_t nblocks, int nseg, int64_t ptot,
+                        int64_t ktot, void* ws, hipStream_t st);
+
 struct FlatSeg {
   void*       dst;
   const void* src;

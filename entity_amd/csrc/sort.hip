@@ -259,7 +259,197 @@ void permute_all(void* const* arrs, const int32_t* bytes, int narr, char* tmp,
   }
 }
 
+// ---- batched: the segments of many boxes in one pass ---------------------------
+
+// segment holding global index g of a prefix array base[0..nseg] (base[0] = 0)
+__device__ __forceinline__ int find_seg(const int64_t* __restrict__ base, int nseg, int64_t g) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (base[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ uint32_t seg_key(const SortSeg& s, int64_t p) {
+  if (static_cast<const int16_t*>(s.arr[17])[p] == NTT_TAG_DEAD) return s.dead_key;
+  int c[3] = { static_cast<const int32_t*>(s.arr[0])[p], static_cast<const int32_t*>(s.arr[1])[p],
+               s.dim == 3 ? static_cast<const int32_t*>(s.arr[2])[p] : 0 };
+  uint32_t key = 0, mul = 1;
+  for (int d = 0; d < s.dim; ++d) {
+    const int x = min(max(c[d], 0), s.ext[d] - 1);
+    key += mul * static_cast<uint32_t>(x / s.tile[d]);
+    mul *= static_cast<uint32_t>(s.nt[d]);
+  }
+  return key;
+}
+
+// Every workgroup works inside one segment (bbase: first block of each
+// segment), so the segment index and its table entry are wave-uniform.
+struct SegBlock {
+  int     s;
+  int64_t p;  // particle index inside the segment
+};
+__device__ __forceinline__ SegBlock seg_block(const int64_t* __restrict__ bbase, int nseg) {
+  const int s = __builtin_amdgcn_readfirstlane(find_seg(bbase, nseg, blockIdx.x));
+  return SegBlock{ s, (static_cast<int64_t>(blockIdx.x) - bbase[s]) * kB + threadIdx.x };
+}
+
+__global__ __launch_bounds__(kB) void k_keys_batched(const SortSeg* __restrict__ segs,
+                                                     const int64_t* __restrict__ pbase,
+                                                     const int64_t* __restrict__ kbase,
+                                                     const int64_t* __restrict__ bbase, int nseg,
+                                                     uint32_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ counts) {
+  const SegBlock b      = seg_block(bbase, nseg);
+  const SortSeg& sg     = segs[b.s];
+  const bool     active = b.p < sg.n;
+  uint32_t       k      = 0;
+  if (active) {
+    k = static_cast<uint32_t>(kbase[b.s]) + seg_key(sg, b.p);
+    keys[pbase[b.s] + b.p] = k;
+  }
+  aggregated_increment(counts, k, active);
+}
+
+// tile offsets of every segment: the global exclusive prefix minus pbase
+__global__ __launch_bounds__(kB) void k_offsets_batched(const SortSeg* __restrict__ segs,
+                                                        const int64_t* __restrict__ pbase,
+                                                        const int64_t* __restrict__ kbase,
+                                                        int nseg, int64_t ktot,
+                                                        const uint32_t* __restrict__ prefix) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (k >= ktot) return;
+  const int s = find_seg(kbase, nseg, k);
+  uint32_t* off = segs[s].tile_offsets;
+  if (off) off[k - kbase[s]] = static_cast<uint32_t>(prefix[k] - pbase[s]);
+}
+
+// the element size of each of the kSortArrays arrays
+__device__ __forceinline__ int arr_bytes(int a) { return a == 17 ? 2 : (a == 18 ? 8 : 4); }
+
+__device__ __forceinline__ void move_elem(void* dst, int64_t to, const void* src, int64_t from,
+                                          int bytes) {
+  if (bytes == 2) static_cast<int16_t*>(dst)[to] = static_cast<const int16_t*>(src)[from];
+  else if (bytes == 4) static_cast<int32_t*>(dst)[to] = static_cast<const int32_t*>(src)[from];
+  else static_cast<int64_t*>(dst)[to] = static_cast<const int64_t*>(src)[from];
+}
+
+// arrays a0 .. a0 + kBatch - 1 of every segment: tmp slot k <- src[perm]
+__global__ __launch_bounds__(kB) void k_gather_batched(const SortSeg* __restrict__ segs,
+                                                       const int64_t* __restrict__ pbase,
+                                                       const int64_t* __restrict__ bbase, int nseg,
+                                                       int64_t ptot, int a0,
+                                                       const uint32_t* __restrict__ perm,
+                                                       char* __restrict__ tmp) {
+  const SegBlock b  = seg_block(bbase, nseg);
+  const SortSeg& sg = segs[b.s];
+  if (b.p >= sg.n) return;
+  const int64_t g = pbase[b.s] + b.p;
+  const int64_t q = static_cast<int64_t>(perm[g]) - pbase[b.s];
+#pragma unroll
+  for (int k = 0; k < kBatch; ++k) {
+    const int a = a0 + k;
+    if (a >= kSortArrays || sg.arr[a] == nullptr) continue;
+    move_elem(tmp + 8 * ptot * k, g, sg.arr[a], q, arr_bytes(a));
+  }
+}
+
+__global__ __launch_bounds__(kB) void k_copy_batched(const SortSeg* __restrict__ segs,
+                                                     const int64_t* __restrict__ pbase,
+                                                     const int64_t* __restrict__ bbase, int nseg,
+                                                     int64_t ptot, int a0,
+                                                     const char* __restrict__ tmp) {
+  const SegBlock b  = seg_block(bbase, nseg);
+  const SortSeg& sg = segs[b.s];
+  if (b.p >= sg.n) return;
+  const int64_t p = b.p;
+  const int64_t g = pbase[b.s] + p;
+#pragma unroll
+  for (int k = 0; k < kBatch; ++k) {
+    const int a = a0 + k;
+    if (a >= kSortArrays || sg.arr[a] == nullptr) continue;
+    move_elem(sg.arr[a], p, tmp + 8 * ptot * k, g, arr_bytes(a));
+  }
+}
+
 }  // namespace
+
+ntt_status make_sort_seg(const ntt_particles_t* p, int32_t dim, const int32_t n[3],
+                         const int32_t tile[3], uint32_t* tile_offsets, SortSeg* out) {
+  ParticleView pv;
+  ntt_status s = make_particle_view(p, dim, true, &pv);
+  if (s != NTT_OK) return s;
+  TileKey K;
+  Layout  L;
+  s = make_layout(pv.n, dim, n, tile, &K, &L);
+  if (s != NTT_OK) return s;
+  SortSeg g{};
+  void* arrs[kSortArrays] = {
+    p->i1, p->i2, dim == 3 ? p->i3 : nullptr,
+    p->i1_prev, p->i2_prev, dim == 3 ? p->i3_prev : nullptr,
+    p->dx1, p->dx2, dim == 3 ? p->dx3 : nullptr,
+    p->dx1_prev, p->dx2_prev, dim == 3 ? p->dx3_prev : nullptr,
+    p->ux1, p->ux2, p->ux3, p->weight, p->phi, p->tag, p->pld_i
+  };
+  for (int a = 0; a < kSortArrays; ++a) g.arr[a] = arrs[a];
+  g.n   = pv.n;
+  g.dim = dim;
+  for (int d = 0; d < 3; ++d) {
+    g.ext[d]  = K.n[d];
+    g.tile[d] = K.t[d];
+    g.nt[d]   = K.nt[d];
+  }
+  g.dead_key     = K.dead_key;
+  g.tile_offsets = tile_offsets;
+  *out = g;
+  return NTT_OK;
+}
+
+int64_t sort_blocks(int64_t n) { return (n + kB - 1) / kB; }
+
+int64_t sort_batched_bytes(int64_t ptot, int64_t ktot) {
+  int64_t off = 0;
+  off = align256(off + 4 * ptot);                    // keys
+  off = align256(off + 4 * ptot);                    // perm
+  off = align256(off + 4 * ktot);                    // counts
+  off = align256(off + 4 * ktot);                    // cursor
+  off = align256(off + 4 * scan_tmp_elems(ktot));    // scan
+  off = align256(off + 8 * kBatch * ptot);           // staging
+  return off;
+}
+
+ntt_status sort_batched(const SortSeg* segs, const int64_t* pbase, const int64_t* kbase,
+                        const int64_t* bbase, int64_t nblocks, int nseg, int64_t ptot,
+                        int64_t ktot, void* ws, hipStream_t st) {
+  if (ptot == 0 || nseg == 0 || nblocks == 0) return NTT_OK;
+  if (nblocks > 0x7FFFFFFFll) return NTT_ERR_ARG;
+  const unsigned nb = static_cast<unsigned>(nblocks);
+  if (ptot > 0xFFFFFFFFll || ktot > 0xFFFFFFFFll) return NTT_ERR_ARG;
+  char*     w    = static_cast<char*>(ws);
+  int64_t   off  = 0;
+  uint32_t* keys = reinterpret_cast<uint32_t*>(w + off); off = align256(off + 4 * ptot);
+  uint32_t* perm = reinterpret_cast<uint32_t*>(w + off); off = align256(off + 4 * ptot);
+  uint32_t* cnt  = reinterpret_cast<uint32_t*>(w + off); off = align256(off + 4 * ktot);
+  uint32_t* cur  = reinterpret_cast<uint32_t*>(w + off); off = align256(off + 4 * ktot);
+  uint32_t* scn  = reinterpret_cast<uint32_t*>(w + off); off = align256(off + 4 * scan_tmp_elems(ktot));
+  char*     tmp  = w + off;
+  k_zero_u32<<<blocks_for(ktot) > 4096 ? 4096 : blocks_for(ktot), kB, 0, st>>>(cnt, ktot);
+  k_keys_batched<<<nb, kB, 0, st>>>(segs, pbase, kbase, bbase, nseg, keys, cnt);
+  ntt_status s = scan_exclusive(cnt, ktot, scn, st);
+  if (s != NTT_OK) return s;
+  if (hipMemcpyAsync(cur, cnt, 4 * ktot, hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return NTT_ERR_HIP;
+  k_scatter<<<blocks_for(ptot), kB, 0, st>>>(keys, ptot, cur, perm);
+  k_offsets_batched<<<blocks_for(ktot), kB, 0, st>>>(segs, pbase, kbase, nseg, ktot, cnt);
+  for (int a0 = 0; a0 < kSortArrays; a0 += kBatch) {
+    k_gather_batched<<<nb, kB, 0, st>>>(segs, pbase, bbase, nseg, ptot, a0, perm, tmp);
+    k_copy_batched<<<nb, kB, 0, st>>>(segs, pbase, bbase, nseg, ptot, a0, tmp);
+  }
+  return check_launch();
+}
+
 }  // namespace ntt
 
 using namespace ntt;

@@ -2,10 +2,13 @@
 
 ``SREnsemble`` advances many ``SRPICEngine`` boxes (any sizes, shape orders,
 pushers, filter counts and particle loads) together: each phase of the step
-is one launch over all boxes (``ntt_ensemble_step``), and a step can be
-captured once in a HIP graph and replayed.  Every box's fields and particles
-equal those of ``SRPICEngine.step_forward`` on that box alone, bit for bit.
-The reference runs such parameter scans as separate simulations.
+is one launch over all boxes (``ntt_ensemble_step``), and so is the tile
+re-sort of every box (``ntt_ensemble_sort``, run when any box's interval is
+due).  A step, with or without the re-sort, is captured once in a HIP graph
+and replayed.  Every box's fields equal those of ``SRPICEngine.step_forward``
+on that box alone bit for bit, and its particles as a set (the order inside
+a tile may differ).  The reference runs such parameter scans as separate
+simulations.
 """
 from __future__ import annotations
 
@@ -27,7 +30,7 @@ class SREnsemble:
         self.engines = list(engines)
         self.lib = engines[0].lib
         self.step = engines[0].step
-        self._graph = None
+        self._graphs = {}
         self._plan = C.c_void_p(None)
         self._build()
 
@@ -46,13 +49,16 @@ class SREnsemble:
             b.metric = _abi.NttMinkowski(eng.params.dim, eng.params.dx)
             b.workspace = eng._step_ws.data_ptr()
             b.workspace_bytes = eng._step_ws.numel()
+            if eng.params.sort_interval:
+                t = list(eng.params.tile) + [1] * (3 - eng.params.dim)
+                b.tile[0], b.tile[1], b.tile[2] = t
         if self._plan.value:
             self.lib.ntt_ensemble_destroy(self._plan)
         plan = C.c_void_p(None)
         _abi.check(self.lib.ntt_ensemble_create(boxes, len(self.engines), C.byref(plan)),
                    "ntt_ensemble_create")
         self._plan = plan
-        self._graph = None
+        self._graphs = {}
 
     def __del__(self):
         try:
@@ -61,11 +67,16 @@ class SREnsemble:
         except Exception:
             pass
 
-    def _sort_due(self) -> None:
-        for eng in self.engines:
-            si = eng.params.sort_interval
-            if si and self.step > 0 and self.step % si == 0:
-                eng.sort()
+    def _sort_due(self) -> bool:
+        """True when some box's re-sort interval is due at this step (then every
+        box that re-sorts at all is re-sorted, in one batched pass)."""
+        return any(e.params.sort_interval and self.step > 0 and
+                   self.step % e.params.sort_interval == 0 for e in self.engines)
+
+    def _enqueue(self, sort: bool, stream) -> None:
+        if sort:
+            _abi.check(self.lib.ntt_ensemble_sort(self._plan, stream), "ntt_ensemble_sort")
+        _abi.check(self.lib.ntt_ensemble_step(self._plan, self.step, stream), "ntt_ensemble_step")
 
     def _advance_counters(self) -> None:
         self.step += 1
@@ -74,31 +85,29 @@ class SREnsemble:
             eng.time += eng.params.dt
 
     def step_forward(self, stream=None) -> None:
-        """One step of every box (re-sorts first where a box's interval is due)."""
-        self._sort_due()
-        _abi.check(self.lib.ntt_ensemble_step(self._plan, self.step, _stream(stream)),
-                   "ntt_ensemble_step")
+        """One step of every box (after the batched re-sort when one is due)."""
+        self._enqueue(self._sort_due(), _stream(stream))
         self._advance_counters()
 
     def capture(self) -> None:
-        """Records one ensemble step (step > 0) in a HIP graph; step_graph()
-        then replays it.  Re-sorts stay outside the graph."""
+        """Records the ensemble step (step > 0) in two HIP graphs, without and
+        with the batched re-sort in front; step_graph() replays the one the
+        step needs."""
         if self.step == 0:
             raise ValueError("capture after the first step (step 0 also fills ghosts)")
-        s = torch.cuda.Stream()
-        s.wait_stream(torch.cuda.current_stream())
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.stream(s):
-            with torch.cuda.graph(g, stream=s):
-                _abi.check(self.lib.ntt_ensemble_step(self._plan, self.step, s.cuda_stream),
-                           "ntt_ensemble_step")
-        torch.cuda.current_stream().wait_stream(s)
-        self._graph = g
+        for sort in (False, True):
+            s = torch.cuda.Stream()
+            s.wait_stream(torch.cuda.current_stream())
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.stream(s):
+                with torch.cuda.graph(g, stream=s):
+                    self._enqueue(sort, s.cuda_stream)
+            torch.cuda.current_stream().wait_stream(s)
+            self._graphs[sort] = g
 
     def step_graph(self) -> None:
-        """One step by replaying the captured graph."""
-        if self._graph is None:
+        """One step by replaying a captured graph."""
+        if not self._graphs:
             raise RuntimeError("capture() first")
-        self._sort_due()
-        self._graph.replay()
+        self._graphs[self._sort_due()].replay()
         self._advance_counters()

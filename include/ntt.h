@@ -443,6 +443,9 @@ typedef struct ntt_box {
   ntt_minkowski_t      metric;
   void*                workspace;        /* ntt_sr_step_workspace_size bytes  */
   uint64_t             workspace_bytes;
+  int32_t              tile[3];          /* tile of ntt_ensemble_sort; tile[0]
+                                            = 0: the ensemble does not sort
+                                            this box                          */
 } ntt_box_t;
 
 typedef struct ntt_ensemble* ntt_ensemble_t;
@@ -463,6 +466,15 @@ NTT_API ntt_status ntt_ensemble_create(const ntt_box_t* boxes, int32_t nboxes,
  * Enqueues kernels only (no allocation, copy or synchronisation), so the
  * call can be captured in a HIP graph. */
 NTT_API ntt_status ntt_ensemble_step(ntt_ensemble_t ens, int64_t step, void* stream);
+
+/* The tile re-sort (ntt_sort_particles, without compaction) of every species
+ * of every box whose ntt_box_t.tile was set, as one counting sort over all of
+ * them: a fixed number of launches whatever the number of boxes.  Writes
+ * each species' tile offsets.  Particle order inside a tile may differ from
+ * ntt_sort_particles; results of later steps do not depend on it.  Enqueues
+ * kernels and device-to-device copies only (graph-capturable); the sort
+ * workspace is allocated by ntt_ensemble_create.  No reference counterpart. */
+NTT_API ntt_status ntt_ensemble_sort(ntt_ensemble_t ens, void* stream);
 
 NTT_API ntt_status ntt_ensemble_destroy(ntt_ensemble_t ens);
 

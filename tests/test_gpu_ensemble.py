@@ -84,3 +84,71 @@ def test_ensemble_graph_replays(hip):
         for a, b in zip(ens_boxes, ref_boxes):
             assert_same(a, b)
     assert device_errors() == 0
+
+
+def test_ensemble_sort_orders_every_box(hip):
+    """ntt_ensemble_sort over boxes of different sizes, dimensions of tiles and
+    species: every species ends grouped by tile with dead particles last,
+    every particle carried with all its arrays, and tile offsets equal to the
+    counts of the tiles."""
+    from entity_amd.ensemble import SREnsemble
+    specs = [BOXES[0], BOXES[1], BOXES[3]]
+    boxes = [make_box(s, sort_interval=4) for s in specs]
+    rng = np.random.default_rng(9)
+    befores = []
+    for eng in boxes:
+        for sp in eng.species:
+            n = sp.npart
+            sp.pld_i = torch.arange(sp.maxnpart, device="cuda")
+            perm = torch.from_numpy(rng.permutation(n)).cuda()
+            for v in sp.arrays().values():
+                v.copy_(v[perm].clone())
+            dead = torch.from_numpy(rng.random(n) < 0.05).cuda()
+            sp.tag[:n][dead] = 0
+            befores.append(sp.to_host())
+    ens = SREnsemble(boxes)
+    _abi_check = __import__("entity_amd._abi", fromlist=["check"]).check
+    _abi_check(ens.lib.ntt_ensemble_sort(ens._plan, torch.cuda.current_stream().cuda_stream),
+               "ntt_ensemble_sort")
+    torch.cuda.synchronize()
+    k = 0
+    for eng in boxes:
+        t1, t2 = eng.params.tile
+        nt1 = (eng.params.n[0] + t1 - 1) // t1
+        for s, sp in enumerate(eng.species):
+            before, after = befores[k], sp.to_host()
+            k += 1
+            inv = np.empty(len(before["pld_i"]), dtype=np.int64)
+            inv[before["pld_i"]] = np.arange(len(before["pld_i"]))
+            src = inv[after["pld_i"]]
+            for name, arr in before.items():
+                np.testing.assert_array_equal(after[name], arr[src], err_msg=name)
+            alive = int((after["tag"] == 1).sum())
+            assert (after["tag"][:alive] == 1).all() and (after["tag"][alive:] == 0).all()
+            key = after["i1"][:alive] // t1 + nt1 * (after["i2"][:alive] // t2)
+            assert (np.diff(key) >= 0).all()
+            off = eng.tile_offsets[s].cpu().numpy()
+            counts = np.bincount(key, minlength=len(off) - 1)
+            np.testing.assert_array_equal(off, np.concatenate([[0], np.cumsum(counts)]))
+
+
+def test_ensemble_graph_replays_with_resorts(hip):
+    """Re-sorting every 2 steps inside the replayed graphs: fields bit-identical
+    to the boxes stepped alone, particles equal as sets."""
+    from entity_amd.ensemble import SREnsemble
+    from entity_amd.kernels import device_errors
+    ens_boxes = [make_box(s, sort_interval=2) for s in BOXES[:4]]
+    ref_boxes = [make_box(s, sort_interval=2) for s in BOXES[:4]]
+    ens = SREnsemble(ens_boxes)
+    ens.step_forward()
+    for b in ref_boxes:
+        b.step_forward()
+    ens.capture()
+    for r in range(6):
+        ens.step_graph()
+        for b in ref_boxes:
+            b.step_forward()
+        _ = (torch.ones(1000, device="cuda") * r).sum().item()
+        for a, b in zip(ens_boxes, ref_boxes):
+            assert_same(a, b)
+    assert device_errors() == 0
