@@ -140,11 +140,14 @@ __global__ __launch_bounds__(kB) void k_scatter(const uint32_t* __restrict__ key
 }
 
 // up to kBatch particle arrays permuted per pass, each staged in a slot of
-// 8 bytes per particle of the scratch buffer
+// its own element size per particle of the scratch buffer (slot k starts at
+// off[k] * n bytes, off[k] a multiple of the element size; at most 8 * kBatch
+// bytes per particle in all)
 constexpr int kBatch = 4;
 struct ArrayBatch {
   void*   arr[kBatch];
   int32_t bytes[kBatch];  // element size: 2, 4 or 8
+  int32_t off[kBatch];    // sum of the element sizes of the slots before
   int32_t count;
 };
 
@@ -162,7 +165,7 @@ __global__ __launch_bounds__(kB) void k_gather_batch(ArrayBatch b, char* __restr
 #pragma unroll
   for (int k = 0; k < kBatch; ++k) {
     if (k >= b.count) break;
-    void* slot = tmp + 8 * n * k;
+    void* slot = tmp + n * b.off[k];
     if (b.bytes[k] == 2) gather_one<int16_t>(slot, b.arr[k], p, q);
     else if (b.bytes[k] == 4) gather_one<int32_t>(slot, b.arr[k], p, q);
     else gather_one<int64_t>(slot, b.arr[k], p, q);
@@ -176,7 +179,7 @@ __global__ __launch_bounds__(kB) void k_copy_batch(ArrayBatch b, const char* __r
 #pragma unroll
   for (int k = 0; k < kBatch; ++k) {
     if (k >= b.count) break;
-    const void* slot = tmp + 8 * n * k;
+    const void* slot = tmp + n * b.off[k];
     if (b.bytes[k] == 2) gather_one<int16_t>(b.arr[k], slot, p, p);
     else if (b.bytes[k] == 4) gather_one<int32_t>(b.arr[k], slot, p, p);
     else gather_one<int64_t>(b.arr[k], slot, p, p);
@@ -249,6 +252,9 @@ void permute_all(void* const* arrs, const int32_t* bytes, int narr, char* tmp,
     if (k < narr && arrs[k] != nullptr) {
       b.arr[b.count]   = arrs[k];
       b.bytes[b.count] = bytes[k];
+      // slots start at multiples of their element size (aligned accesses)
+      const int end    = b.count ? b.off[b.count - 1] + b.bytes[b.count - 1] : 0;
+      b.off[b.count]   = (end + bytes[k] - 1) / bytes[k] * bytes[k];
       ++b.count;
     }
     if (b.count == kBatch || (k == narr && b.count > 0)) {
@@ -348,11 +354,15 @@ __global__ __launch_bounds__(kB) void k_gather_batched(const SortSeg* __restrict
   if (b.p >= sg.n) return;
   const int64_t g = pbase[b.s] + b.p;
   const int64_t q = static_cast<int64_t>(perm[g]) - pbase[b.s];
+  int64_t off = 0;  // slot offset in bytes per particle (the element sizes before)
 #pragma unroll
   for (int k = 0; k < kBatch; ++k) {
     const int a = a0 + k;
-    if (a >= kSortArrays || sg.arr[a] == nullptr) continue;
-    move_elem(tmp + 8 * ptot * k, g, sg.arr[a], q, arr_bytes(a));
+    if (a >= kSortArrays) break;
+    const int sz = arr_bytes(a);
+    off = (off + sz - 1) / sz * sz;  // aligned slot
+    if (sg.arr[a] != nullptr) move_elem(tmp + ptot * off, g, sg.arr[a], q, sz);
+    off += sz;
   }
 }
 
@@ -366,11 +376,15 @@ __global__ __launch_bounds__(kB) void k_copy_batched(const SortSeg* __restrict__
   if (b.p >= sg.n) return;
   const int64_t p = b.p;
   const int64_t g = pbase[b.s] + p;
+  int64_t off = 0;
 #pragma unroll
   for (int k = 0; k < kBatch; ++k) {
     const int a = a0 + k;
-    if (a >= kSortArrays || sg.arr[a] == nullptr) continue;
-    move_elem(sg.arr[a], p, tmp + 8 * ptot * k, g, arr_bytes(a));
+    if (a >= kSortArrays) break;
+    const int sz = arr_bytes(a);
+    off = (off + sz - 1) / sz * sz;
+    if (sg.arr[a] != nullptr) move_elem(sg.arr[a], p, tmp + ptot * off, g, sz);
+    off += sz;
   }
 }
 
