@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--gr", type=int, default=1, help="also time the GR pusher runs")
     ap.add_argument("--gr-steps", type=int, default=5)
     ap.add_argument("--adjoint", type=int, default=1, help="also time the adjoint (gradient) run")
+    ap.add_argument("--autotune", type=int, default=1, help="also run the launch autotuner")
     return ap.parse_args()
 
 
@@ -365,6 +366,49 @@ def adjoint_leg(args):
                                   "b": abs(gb64 - fd_b) / abs(fd_b)}}
 
 
+def autotune_leg(args):
+    """Autotuner on a 2D periodic 256 x 256 box, e-/e+ 16 ppc per species
+    (2,097,152 particles), theta 0.1, counter-drift u = 0.57735, order 1, Boris,
+    1 filter pass: every candidate (tile shape x re-sort cadence, and the
+    unsorted layout x chunk size) runs 16 timed steps after 32 steps that age
+    the layout; reports the fastest and its gain over the default (8x8 tiles,
+    re-sort every 4)."""
+    import torch
+    from entity_amd.autotune import Candidate, candidates_for, engine_runner, tune
+    from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
+    n, ppc = (256, 256), 16
+    ref = SRPICEngine(SRParams(dim=2, n=n, tile=(8, 8)), [SpeciesSpec("e-", 1.0, -1.0),
+                                                          SpeciesSpec("e+", 1.0, 1.0)],
+                      n[0] * n[1] * ppc)
+    ref.inject_uniform(ppc, temperature=0.1, drifts=[(0.57735, 0, 0), (-0.57735, 0, 0)], seed=11)
+    prts = [sp.to_host() for sp in ref.species]
+    del ref
+
+    def make(c):
+        prm = SRParams(dim=2, n=n, shape_order=1, n_filters=1, ppc0=2.0 * ppc, larmor0=8.0,
+                       skindepth0=8.0, tile=c.tile, sort_interval=c.sort_interval,
+                       particles_per_thread=c.particles_per_thread)
+        eng = SRPICEngine(prm, [SpeciesSpec("e-", 1.0, -1.0), SpeciesSpec("e+", 1.0, 1.0)],
+                          n[0] * n[1] * ppc)
+        for sp, h in zip(eng.species, prts):
+            sp.load(**h)
+        if c.sort_interval:
+            eng.sort()
+        return eng
+
+    res = tune(engine_runner(make, steps=16, warmup=32), candidates_for(2))
+    default = Candidate((8, 8), 4)
+    t_def = res.ms_per_step.get(default)
+    torch.cuda.synchronize()
+    return {"workload": "2D periodic 256x256, e-/e+ 16 ppc per species (2,097,152 particles), "
+                        "order 1, Boris, 1 filter pass",
+            "best": {"tile": list(res.best.tile), "sort_interval": res.best.sort_interval,
+                     "particles_per_thread": res.best.particles_per_thread},
+            "best_ms_per_step": res.ms_per_step[res.best], "default_ms_per_step": t_def,
+            "gain_over_default": (t_def / res.ms_per_step[res.best]) if t_def else None,
+            "candidates": len(res.ms_per_step)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -470,6 +514,8 @@ def main():
         out["gr"] = gr_leg(args)
     if args.adjoint:
         out["adjoint"] = adjoint_leg(args)
+    if args.autotune:
+        out["autotune"] = autotune_leg(args)
     if args.cpu_baseline:
         ccells = tuple(int(x) for x in args.cpu_cells.split(","))
         out["cpu_baseline"] = cpu_baseline(args, ccells, args.order, pusher, args.filters)

@@ -49,6 +49,7 @@ class SRParams:
     fd: dict = field(default_factory=dict)
     sort_interval: int = 0          # 0: never re-sort
     tile: tuple = None              # sort / generation tile
+    particles_per_thread: int = 0   # chunk of the unsorted layout (0: the library's 4)
 
     def __post_init__(self):
         if self.extent is None:
@@ -156,6 +157,7 @@ class SRPICEngine:
         s.fieldsolver_enable, s.deposit_enable = int(p.fieldsolver_enable), int(p.deposit_enable)
         s.step = self.step
         s.weight_max = self.weight_max
+        s.particles_per_thread = p.particles_per_thread
         for k, v in p.fd.items():
             setattr(s.fd, k, v)
         for d in range(3):
@@ -214,6 +216,10 @@ class SRPICEngine:
         p = self.params
         n = (C.c_int32 * 3)(*(list(p.n) + [1] * (3 - p.dim)))
         tile = (C.c_int32 * 3)(*(list(p.tile) + [1] * (3 - p.dim)))
+        nt = _ntiles(self.lib, p)
+        if nt != self.ntiles:   # the tile shape changed: offsets of the new size
+            self.ntiles = nt
+            self.tile_offsets = [None] * len(self.species)
         need = max(_sort_ws_bytes(self.lib, sp.maxnpart, p.dim, n, tile) for sp in self.species)
         if self._sort_ws is None or self._sort_ws.numel() < need:
             self._sort_ws = torch.empty(need, dtype=torch.uint8, device=self.device)
