@@ -27,7 +27,14 @@ _U_NAMES = ("ux1", "ux2", "ux3")
 
 
 class Particles:
-    """Structure-of-arrays particle storage of one species on one device."""
+    """Structure-of-arrays particle storage of one species on one device.
+
+    ``version`` changes whenever an attribute is assigned (an array replaced,
+    npart changed), so holders of the C ABI view know when to rebuild it."""
+
+    def __setattr__(self, name, value):
+        object.__setattr__(self, "version", self.__dict__.get("version", 0) + 1)
+        object.__setattr__(self, name, value)
 
     def __init__(self, dim: int, maxnpart: int, *, device="cuda", label: str = "",
                  mass: float = 1.0, charge: float = 1.0, pusher: int = _abi.PUSHER_BORIS,
@@ -117,7 +124,12 @@ class GridShape:
 
 
 class Fields:
-    """em, cur and buff arrays of one domain (Yee staggering as in fields.h:40-52)."""
+    """em, cur and buff arrays of one domain (Yee staggering as in fields.h:40-52).
+    ``version`` changes whenever an array is replaced."""
+
+    def __setattr__(self, name, value):
+        object.__setattr__(self, "version", self.__dict__.get("version", 0) + 1)
+        object.__setattr__(self, name, value)
 
     def __init__(self, shape: GridShape, *, device="cuda"):
         self.shape = shape

@@ -330,6 +330,169 @@ __device__ __forceinline__ void deposit_vjp_acc(const ParticleView& P, int64_t p
       }
 }
 
+// The same vector-Jacobian product by a hand-written reverse sweep (a few
+// times the forward arithmetic instead of seven): the Esirkepov currents are
+// prefix sums of separable products, so their adjoint is a suffix sum of the
+// current adjoint times the other factor; the shapes' adjoints then go
+// through the B-spline derivatives, and the out-of-plane term through
+// v_z = u3 / gamma.  Equal to deposit_vjp_acc to round-off.
+template <int O, typename R>
+__device__ __forceinline__ void shape_d(int i, R di, int& imin, R* S, R* dS) {
+  // prtl_shape::order<false, O> (particle_shapes.hpp:544-588) and its derivative
+  if constexpr (O == 1) {
+    imin  = i;
+    S[0]  = R(1) - di;
+    S[1]  = di;
+    dS[0] = R(-1);
+    dS[1] = R(1);
+  } else {
+    if (di < R(0.5)) {
+      imin      = i - 1;
+      const R a = R(0.5) - di;
+      S[0]      = R(0.5) * a * a;
+      S[1]      = R(0.75) - di * di;
+      dS[0]     = -a;
+      dS[1]     = R(-2) * di;
+    } else {
+      imin      = i;
+      const R a = R(1.5) - di, b = R(1) - di;
+      S[0]      = R(0.5) * a * a;
+      S[1]      = R(0.75) - b * b;
+      dS[0]     = -a;
+      dS[1]     = R(2) * b;
+    }
+    S[2]  = R(1) - S[0] - S[1];
+    dS[2] = -dS[0] - dS[1];
+  }
+}
+
+template <int O, typename R>
+__device__ __forceinline__ void for_deposit_d(int i_init, R di_init, int i_fin, R di_fin,
+                                              int& i_min, int& di, R* iS, R* fS, R* diS,
+                                              R* dfS) {
+  int a, b;
+  R   iS_[O + 1], fS_[O + 1], diS_[O + 1], dfS_[O + 1];
+  shape_d<O, R>(i_init, di_init, a, iS_, diS_);
+  shape_d<O, R>(i_fin, di_fin, b, fS_, dfS_);
+#pragma unroll
+  for (int j = 0; j < O + 2; ++j) iS[j] = fS[j] = diS[j] = dfS[j] = R(0);
+  // particle_shapes.hpp:934-1024: the two stencils on one common range
+  const int si = (a > b) ? 1 : 0, sf = (a < b) ? 1 : 0;
+  i_min = (a < b) ? a : b;
+  di    = (a == b) ? O : O + 1;
+  // shifted copies with selects (static indices keep the arrays in registers)
+#pragma unroll
+  for (int j = 0; j < O + 2; ++j) {
+    const R i0 = (j < O + 1) ? iS_[j] : R(0), i1 = (j > 0) ? iS_[j - 1] : R(0);
+    const R d0 = (j < O + 1) ? diS_[j] : R(0), d1v = (j > 0) ? diS_[j - 1] : R(0);
+    const R f0 = (j < O + 1) ? fS_[j] : R(0), f1 = (j > 0) ? fS_[j - 1] : R(0);
+    const R e0 = (j < O + 1) ? dfS_[j] : R(0), e1 = (j > 0) ? dfS_[j - 1] : R(0);
+    iS[j]  = si ? i1 : i0;
+    diS[j] = si ? d1v : d0;
+    fS[j]  = sf ? f1 : f0;
+    dfS[j] = sf ? e1 : e0;
+  }
+}
+
+template <int O, typename R>
+__device__ __forceinline__ void deposit_vjp_rev(const ParticleView& P, int64_t p,
+                                                const AdjField<R>& lj, R charge, R inv_dt,
+                                                R* acc) {
+  constexpr int M = O + 2;
+  R   iS1[M], fS1[M], iS2[M], fS2[M], diS1[M], dfS1[M], diS2[M], dfS2[M];
+  int i1_min, d1, i2_min, d2;
+  for_deposit_d<O, R>(P.i1p[p], static_cast<R>(P.dx1p[p]), P.i1[p], static_cast<R>(P.dx1[p]),
+                      i1_min, d1, iS1, fS1, diS1, dfS1);
+  for_deposit_d<O, R>(P.i2p[p], static_cast<R>(P.dx2p[p]), P.i2[p], static_cast<R>(P.dx2[p]),
+                      i2_min, d2, iS2, fS2, diS2, dfS2);
+  const R u1 = P.ux1[p], u2 = P.ux2[p], u3 = P.ux3[p];
+  const R gam   = sqrt(R(1) + (u1 * u1 + u2 * u2 + u3 * u3));
+  const R vz    = u3 / gam;
+  const R coeff = static_cast<R>(P.w[p]) * charge;
+  const R c0    = -(coeff * inv_dt) * R(0.5);
+  const R third = static_cast<R>(0.333333f);
+  const bool    inner = i1_min >= 0 && i1_min + M <= lj.n1 && i2_min >= 0 && i2_min + M <= lj.n2;
+  const int64_t base  = static_cast<int64_t>(i1_min) + static_cast<int64_t>(lj.n1) * i2_min;
+  const int64_t cs    = static_cast<int64_t>(lj.n1) * lj.n2;
+  auto lam = [&](int c, int a, int b) -> R {
+    return inner ? lj.d[base + a + static_cast<int64_t>(lj.n1) * b + cs * c]
+                 : lj.get(c, i1_min + a, i2_min + b);
+  };
+  // adjoints of the shape values
+  R gi1[M], gf1[M], gi2[M], gf2[M];
+#pragma unroll
+  for (int j = 0; j < M; ++j) gi1[j] = gf1[j] = gi2[j] = gf2[j] = R(0);
+  // jx1(a, b) = sum_{a' <= a} c0 (fS1 - iS1)[a'] (fS2 + iS2)[b], a < d1, b <= d2
+#pragma unroll
+  for (int b = 0; b < M; ++b) {
+    if (b > d2) continue;
+    const R B = fS2[b] + iS2[b];
+    R       L = R(0), gB = R(0);
+#pragma unroll
+    for (int a = M - 1; a >= 0; --a) {
+      if (a < d1) L += lam(0, a, b);
+      const R gA = c0 * B * L;
+      gf1[a] += gA;
+      gi1[a] -= gA;
+      gB += (fS1[a] - iS1[a]) * L;
+    }
+    gf2[b] += c0 * gB;
+    gi2[b] += c0 * gB;
+  }
+  // jx2(a, b) = sum_{b' <= b} c0 (fS1 + iS1)[a] (fS2 - iS2)[b'], a <= d1, b < d2
+#pragma unroll
+  for (int a = 0; a < M; ++a) {
+    if (a > d1) continue;
+    const R A = fS1[a] + iS1[a];
+    R       L = R(0), gA = R(0);
+#pragma unroll
+    for (int b = M - 1; b >= 0; --b) {
+      if (b < d2) L += lam(1, a, b);
+      const R gB = c0 * A * L;
+      gf2[b] += gB;
+      gi2[b] -= gB;
+      gA += (fS2[b] - iS2[b]) * L;
+    }
+    gf1[a] += c0 * gA;
+    gi1[a] += c0 * gA;
+  }
+  // jx3(a, b) = coeff vz third (fS2[b] (iS1[a] / 2 + fS1[a]) + iS2[b] (fS1[a] / 2 + iS1[a]))
+  const R q3 = coeff * vz * third;
+  R       K3 = R(0);  // d<lam, jx3> / d(coeff vz)
+#pragma unroll
+  for (int a = 0; a < M; ++a)
+#pragma unroll
+    for (int b = 0; b < M; ++b) {
+      if (a > d1 || b > d2) continue;
+      const R l  = lam(2, a, b);
+      const R p1 = R(0.5) * iS1[a] + fS1[a], p2 = R(0.5) * fS1[a] + iS1[a];
+      K3 += l * (fS2[b] * p1 + iS2[b] * p2);
+      const R k = q3 * l;
+      gf2[b] += k * p1;
+      gi2[b] += k * p2;
+      gf1[a] += k * (fS2[b] + R(0.5) * iS2[b]);
+      gi1[a] += k * (R(0.5) * fS2[b] + iS2[b]);
+    }
+  K3 *= third;
+  R s0 = R(0), s1 = R(0), s2 = R(0), s3 = R(0);
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    s0 += gi1[j] * diS1[j];
+    s1 += gi2[j] * diS2[j];
+    s2 += gf1[j] * dfS1[j];
+    s3 += gf2[j] * dfS2[j];
+  }
+  acc[0] = s0;
+  acc[1] = s1;
+  acc[2] = s2;
+  acc[3] = s3;
+  // v_z = u3 / gamma
+  const R g3 = gam * gam * gam, kz = coeff * K3;
+  acc[4] = kz * (-u3 * u1 / g3);
+  acc[5] = kz * (-u3 * u2 / g3);
+  acc[6] = kz * (R(1) / gam - u3 * u3 / g3);
+}
+
 template <int O, typename R>
 __global__ __launch_bounds__(256) void k_deposit_vjp(ParticleView P, const int64_t* ids,
                                                      AdjField<R> lj, R charge, R inv_dt,
@@ -472,14 +635,14 @@ __global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const 
   __syncthreads();
 
   // ---- pass 2: per-particle VJP
-  using D11 = Dual<11, R>;
-  using D2  = Dual<2, R>;
+  using D9 = Dual<9, R>;
+  using D2 = Dual<2, R>;
   for (int64_t p = beg + tid; p < end; p += kVjpThreads) {
     const int64_t q = adj_index(ids, p, min(pre.size, post.size), err);
     if (q < 0) continue;
     R a7[7] = { R(0), R(0), R(0), R(0), R(0), R(0), R(0) };
     if constexpr (DEP) {
-      if (Q.tag[p] == NTT_TAG_ALIVE) deposit_vjp_acc<O, R>(Q, p, lj, charge, inv_dt, a7);
+      if (Q.tag[p] == NTT_TAG_ALIVE) deposit_vjp_rev<O, R>(Q, p, lj, charge, inv_dt, a7);
     }
     if (!valid(p)) {
       pre.lu1[q] = pre.lu2[q] = pre.lu3[q] = R(0);
@@ -514,28 +677,33 @@ __global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const 
     g[3] = gather(3, ip, jd, S1p, S2d);
     g[4] = gather(4, id, jp, S1d, S2p);
     g[5] = gather(5, id, jd, S1d, S2d);
-    // Boris + position update over the 11 inputs
-    D11 u[3] = { D11::var(static_cast<R>(P.ux1[p]), 2), D11::var(static_cast<R>(P.ux2[p]), 3),
-                 D11::var(static_cast<R>(P.ux3[p]), 4) };
-    D11 e[3], b[3];
+    // Boris + position update.  The offsets enter the update only through
+    // the gathered fields (handled by the Dual<2> gather) and additively
+    // (d dx'/d dx = 1), so the duals carry the 9 inputs u1..3, e1..3, b1..3.
+    D9 u[3] = { D9::var(static_cast<R>(P.ux1[p]), 0), D9::var(static_cast<R>(P.ux2[p]), 1),
+                D9::var(static_cast<R>(P.ux3[p]), 2) };
+    D9 e[3], b[3];
     for (int c = 0; c < 3; ++c) {
-      e[c] = D11::var(g[c].v, 5 + c);
-      b[c] = D11::var(g[3 + c].v, 8 + c);
+      e[c] = D9::var(g[c].v, 3 + c);
+      b[c] = D9::var(g[3 + c].v, 6 + c);
     }
     for (int c = 0; c < 2; ++c) {
       e[c] = e[c] * dxm;
       b[c] = b[c] * dxm;
     }
-    boris_t<D11, R>(ndth, u, e, b);
-    const D11 dtie = dt / tsqrt(R(1) + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
-    const D11 X1 = D11::var(static_cast<R>(P.dx1[p]), 0) + (u[0] / D11(dxm)) * dtie;
-    const D11 X2 = D11::var(static_cast<R>(P.dx2[p]), 1) + (u[1] / D11(dxm)) * dtie;
+    boris_t<D9, R>(ndth, u, e, b);
+    const D9 dtie = dt / tsqrt(R(1) + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+    const D9 X1 = (u[0] / D9(dxm)) * dtie;   // dx1' - dx1
+    const D9 X2 = (u[1] / D9(dxm)) * dtie;
     const R lo5[5] = { post.lx1[q] + a7[2], post.lx2[q] + a7[3], post.lu1[q] + a7[4],
                        post.lu2[q] + a7[5], post.lu3[q] + a7[6] };
+    // lin = (d/d dx1, d/d dx2, d/du1..3, d/de1..3, d/db1..3) of <lo5, (dx', u')>
     R lin[11];
-    for (int k = 0; k < 11; ++k)
-      lin[k] = lo5[0] * X1.d[k] + lo5[1] * X2.d[k] + lo5[2] * u[0].d[k] + lo5[3] * u[1].d[k] +
-               lo5[4] * u[2].d[k];
+    lin[0] = lo5[0];
+    lin[1] = lo5[1];
+    for (int k = 0; k < 9; ++k)
+      lin[2 + k] = lo5[0] * X1.d[k] + lo5[1] * X2.d[k] + lo5[2] * u[0].d[k] +
+                   lo5[3] * u[1].d[k] + lo5[4] * u[2].d[k];
     // adjoint of the gather: the position derivative of the gathered values,
     // and a scatter into the field adjoint
     R lx1 = lin[0], lx2 = lin[1];

@@ -187,9 +187,10 @@ class SRPICEngine:
         return False
 
     def _step_args(self):
-        """The ABI structures of ntt_sr_step, rebuilt only when a particle count,
-        a tile-offset array, the weight bound or a parameter has changed."""
-        key = (tuple(sp.npart for sp in self.species),
+        """The ABI structures of ntt_sr_step, rebuilt only when a particle array
+        or count (the containers' versions), a tile-offset array, the weight
+        bound or a parameter has changed."""
+        key = (tuple(sp.version for sp in self.species), self.fields.version,
                tuple(t.data_ptr() if t is not None else 0 for t in self.tile_offsets),
                self.weight_max, tuple(self.params.__dict__.values()))
         if getattr(self, "_args_key", None) != key:
