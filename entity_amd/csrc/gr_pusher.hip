@@ -443,7 +443,11 @@ __global__ __launch_bounds__(256) void k_gr_push(GRArgs g, KSMetric<R, KIND> M, 
     vu[1] = v[1];
     vu[2] = v[2];
     R vm[3], vmc[3];
+    // the iteration is a fixed map of (vu[0], vu[1]): once an iteration
+    // leaves them unchanged bit for bit, every later one does too, so
+    // stopping there gives exactly the niter-iteration result
     for (int it = 0; it < g.niter; ++it) {
+      const R prev0 = vu[0], prev1 = vu[1];
       vm[0] = R(0.5) * (v[0] + vu[0]);
       vm[1] = R(0.5) * (v[1] + vu[1]);
       vm[2] = v[2];
@@ -457,6 +461,7 @@ __global__ __launch_bounds__(256) void k_gr_push(GRArgs g, KSMetric<R, KIND> M, 
                            (R(0.5) / u0) *
                              (d.dt_h11 * sqr(vm[0]) + d.dt_h22 * sqr(vm[1]) + d.dt_h33 * sqr(vm[2]) +
                               R(2) * d.dt_h13 * vm[0] * vm[2]));
+      if (vu[0] == prev0 && vu[1] == prev1) break;
     }
   };
 
@@ -516,7 +521,8 @@ __global__ __launch_bounds__(256) void k_gr_push(GRArgs g, KSMetric<R, KIND> M, 
 
   // GeodesicCoordinatePush (gr.hpp:333-383)
   R xu0 = x0, xu1 = x1;
-  for (int it = 0; it < g.niter; ++it) {
+  for (int it = 0; it < g.niter; ++it) {   // exact early stop as in momentum_push
+    const R prev0 = xu0, prev1 = xu1;
     const R xm0 = R(0.5) * (x0 + xu0);
     const R xm1 = clamp_theta(R(0.5) * (x1 + xu1));
     MPt<R> mm;
@@ -526,6 +532,7 @@ __global__ __launch_bounds__(256) void k_gr_push(GRArgs g, KSMetric<R, KIND> M, 
     const R u0 = gamma_of(vp_upd, vc) / mm.alpha;
     xu0 = x0 + dt * (vc[0] / u0 - mm.beta1);
     xu1 = x1 + dt * (vc[1] / u0);
+    if (xu0 == prev0 && xu1 == prev1) break;
   }
 
   // from_Xi_to_i_di (gr.hpp:36-45) and boundaryConditions (gr.hpp:818-865)
