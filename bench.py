@@ -55,6 +55,7 @@ def parse():
     ap.add_argument("--gr-steps", type=int, default=5)
     ap.add_argument("--adjoint", type=int, default=1, help="also time the adjoint (gradient) run")
     ap.add_argument("--autotune", type=int, default=1, help="also run the launch autotuner")
+    ap.add_argument("--c0", type=int, default=1, help="also run the 2D 256^2 CPU-runnable case")
     return ap.parse_args()
 
 
@@ -409,6 +410,85 @@ def autotune_leg(args):
             "candidates": len(res.ms_per_step)}
 
 
+def c0_leg(args):
+    """The reference's CPU-runnable case on the GPU: 2D periodic 256 x 256,
+    dx = 1, CFL 0.5, e-/e+ beams at u_x = +-0.57735 (beta 0.5) over
+    theta = 0.1, 16 ppc per species x 4 species (4,194,304 particles), zero
+    initial fields, shape order 1, Boris, 1 filter pass, re-sort every 4
+    steps, 100 steps.  Reports particle-steps/s, cell-updates/s and the 2D
+    fused kernel against the HBM roofline (78 B per particle + 36 B per cell
+    per launch), with the oracle timed on the host for 10 steps of the same box."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from entity_amd import _abi
+    from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
+    n, ppc, nsteps = (256, 256), 16, 100
+    drifts = [(0.57735, 0, 0), (0.57735, 0, 0), (-0.57735, 0, 0), (-0.57735, 0, 0)]
+    prm = SRParams(dim=2, n=n, cfl=0.5, shape_order=1, n_filters=1, ppc0=4.0 * ppc, larmor0=8.0,
+                   skindepth0=8.0, sort_interval=4, tile=(8, 8))
+    sp = [SpeciesSpec(f"s{k}", 1.0, q) for k, q in enumerate((-1.0, 1.0, -1.0, 1.0))]
+    eng = SRPICEngine(prm, sp, n[0] * n[1] * ppc)
+    eng.inject_uniform(ppc, temperature=0.1, drifts=drifts, seed=101)
+    for _ in range(4):
+        eng.step_forward()
+    torch.cuda.synchronize()
+    lib = eng.lib
+    lib.ntt_profile_read(None, None, 1)
+    lib.ntt_profile_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        eng.step_forward()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    lib.ntt_profile_enable(0)
+    ms, nl = C.c_double(0), C.c_int64(0)
+    _abi.check(lib.ntt_profile_read(C.byref(ms), C.byref(nl), 1), "ntt_profile_read")
+    npart = 4 * n[0] * n[1] * ppc
+    per = n[0] * n[1] * ppc
+    avg = ms.value / max(nl.value, 1)
+    achieved = (per * 78 + n[0] * n[1] * 36) / (avg * 1e-3) / 1e9
+    out = {"workload": "2D periodic 256x256, e-/e+ beams u_x +-0.57735 over theta 0.1, "
+                       "4 species x 16 ppc (4,194,304 particles), order 1, Boris, 1 filter pass, "
+                       "re-sort every 4 steps, 100 steps",
+           "particles": npart, "steps": nsteps, "value": npart * nsteps / el,
+           "unit": "particle-steps/s", "ms_per_step": 1e3 * el / nsteps,
+           "cell_updates_per_s": n[0] * n[1] * nsteps / el,
+           "fused_kernel": {"avg_launch_ms": avg, "achieved_gbs": achieved,
+                            "frac": achieved / HBM_PEAK_GBS}}
+    if args.cpu_baseline:
+        out["cpu_baseline"] = c0_cpu_baseline(eng, n, ppc, drifts)
+    return out
+
+
+def c0_cpu_baseline(eng, n, ppc, drifts):
+    """The oracle (sequential C++, FP32, one core) on 10 steps of the same
+    box, from the GPU engine's state copied to the host."""
+    import numpy as np
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import ntt_oracle
+    p = eng.params
+    em = eng.fields.em.cpu().numpy().astype(np.float32)
+    cur = np.zeros((3,) + em.shape[1:], np.float32)
+    buff = np.zeros_like(cur)
+    species = [{"p": ntt_oracle.normalize_particles(sp.to_host(), "f32"), "mass": sp.mass,
+                "charge": sp.charge, "pusher": sp.pusher} for sp in eng.species]
+    prm = dict(dt=p.dt, correction=p.correction, omegaB0=p.omegaB0, q0=p.q0, B0=p.B0, V0=p.V0,
+               ppc0=p.ppc0, dx=p.dx, shape_order=p.shape_order, n_filters=p.n_filters,
+               fieldsolver_enable=1, deposit_enable=1)
+    nst = 10
+    t0 = time.perf_counter()
+    for k in range(nst):
+        prm["step"] = eng.step + k
+        rc = ntt_oracle.sr_step(em, cur, buff, species, dim=2, prm=prm,
+                                bc=((1, 1), (1, 1), (0, 0)))
+        assert rc == 0, rc
+    el = time.perf_counter() - t0
+    npart = sum(int(sp["p"]["_n"]) for sp in species)
+    return {"value": npart * nst / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{nst} full steps of the same 256x256 box ({npart} particles), oracle C++ "
+                      f"restatement in FP32, one core, {el:.1f} s"}
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -516,6 +596,8 @@ def main():
         out["adjoint"] = adjoint_leg(args)
     if args.autotune:
         out["autotune"] = autotune_leg(args)
+    if args.c0:
+        out["c0"] = c0_leg(args)
     if args.cpu_baseline:
         ccells = tuple(int(x) for x in args.cpu_cells.split(","))
         out["cpu_baseline"] = cpu_baseline(args, ccells, args.order, pusher, args.filters)
