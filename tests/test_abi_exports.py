@@ -46,3 +46,20 @@ def test_product_path_has_no_oracle_dependency():
         if f.suffix in (".py", ".hip", ".hpp", ".cpp", ".h"):
             txt = f.read_text()
             assert "ntt_oracle" not in txt and "libntt_oracle" not in txt, f
+
+
+def test_container_versions_track_assignments():
+    """The engine rebuilds its cached ABI view of the particles when an array
+    or the particle count of a container is assigned."""
+    from entity_amd.containers import Fields, GridShape, Particles
+    p = Particles(2, 8, device="cpu")
+    v0 = p.version
+    p.npart = 3
+    assert p.version > v0
+    v1 = p.version
+    p.pld_i = None
+    assert p.version > v1
+    f = Fields(GridShape(2, (4, 4)), device="cpu")
+    w = f.version
+    f.em = f.em.clone()
+    assert f.version > w
