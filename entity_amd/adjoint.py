@@ -8,7 +8,7 @@ kernels of the library (``ntt_sr_particle_vjp``, the fused form of
 ``checkpoint`` steps and recomputing the steps in between.
 
 Particle adjoints are indexed by particle ID when the engine carries an ID
-payload (``SRPICEngine(..., with_payload=True)``): the driver numbers the
+payload (``SRPICEngine(..., with_payload=True)``): SRAdjoint numbers the
 particles 0..npart-1 in ``pld_i`` at the start, and the tile re-sorts of the
 forward then permute particles without disturbing the adjoint.  Without a
 payload the adjoint is indexed by position and re-sorting must be off.
