@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--filters", type=int, default=2)
     ap.add_argument("--sort-interval", type=int, default=4)
     ap.add_argument("--tile", type=str, default="4,4,4")
+    ap.add_argument("--double-buffer", type=int, default=1,
+                    help="re-sort out of place into a second set of particle arrays")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-cells", type=str, default="64,64,64")
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -512,7 +514,8 @@ def main():
     npart = ncell * args.ppc
     t_init = time.perf_counter()
     eng = SRPICEngine(prm, [SpeciesSpec("e-", 1.0, -1.0, pusher),
-                            SpeciesSpec("e+", 1.0, 1.0, pusher)], npart)
+                            SpeciesSpec("e+", 1.0, 1.0, pusher)], npart,
+                      double_buffer=bool(args.double_buffer))
     eng.inject_uniform(args.ppc, temperature=0.3, seed=2024, position_seed=99)
     init_fields(eng)
     torch.cuda.synchronize()
@@ -564,10 +567,12 @@ def main():
                         f"{args.ppc} ppc per species ({total_particles} particles), theta=0.3, "
                         f"B0 z + 16-mode dB/B0=0.5, shape order {args.order}, {args.pusher} pusher, "
                         f"{args.filters} filter passes, CFL 0.5, re-sort every "
-                        f"{args.sort_interval} steps",
+                        f"{args.sort_interval} steps"
+                        + (" (out of place, double-buffered particles)" if args.double_buffer
+                           else ""),
             "cells": list(cells), "particles": total_particles, "shape_order": args.order,
             "pusher": args.pusher, "filters": args.filters, "sort_interval": args.sort_interval,
-            "tile": list(tile),
+            "tile": list(tile), "double_buffer": bool(args.double_buffer),
         },
         "cell_updates_per_s": ncell * args.steps / el,
         "roofline": {

@@ -194,6 +194,11 @@ SIGNATURES: dict[str, tuple] = {
     "ntt_sort_particles": (C.c_int, [_P(NttParticles), C.c_int32, _P(C.c_int32), _P(C.c_int32),
                                      C.c_void_p, C.c_uint64, C.c_void_p, _P(C.c_int64),
                                      C.c_void_p]),
+    "ntt_sort_particles_to": (C.c_int, [_P(NttParticles), _P(NttParticles), C.c_int32,
+                                        _P(C.c_int32), _P(C.c_int32), C.c_void_p, C.c_uint64,
+                                        C.c_void_p, _P(C.c_int64), C.c_void_p]),
+    "ntt_sort_workspace_size_to": (C.c_int, [C.c_int64, C.c_int32, _P(C.c_int32), _P(C.c_int32),
+                                             _P(C.c_uint64)]),
     "ntt_sort_ntiles": (C.c_int, [C.c_int32, _P(C.c_int32), _P(C.c_int32), _P(C.c_int64)]),
     "ntt_profile_enable": (C.c_int, [C.c_int32]),
     "ntt_profile_read": (C.c_int, [_P(C.c_double), _P(C.c_int64), C.c_int32]),

@@ -54,7 +54,7 @@ class _Snap:
         self.names = _MOVED + (_SORTED if with_sorted else ())
         self.has_pld = with_sorted and sp.pld_i is not None
         # int32 views of the species' arrays, made once per (species, kind)
-        key = (id(sp), n, with_sorted)
+        key = (id(sp), sp.version, with_sorted)
         parts = views.get(key)
         if parts is None:
             parts = [getattr(sp, k)[:n].view(torch.int32) for k in self.names]

@@ -76,6 +76,16 @@ class Particles:
         s.maxnpart = self.maxnpart
         return s
 
+    def swap_arrays(self, other: "Particles") -> None:
+        """Exchange every particle array with `other` (same capacity and
+        optional arrays) together with the particle counts."""
+        names = _INT_NAMES + _DX_NAMES + _U_NAMES + ("weight", "tag", "phi", "pld_i")
+        for name in names:
+            a, b = getattr(self, name), getattr(other, name)
+            setattr(self, name, b)
+            setattr(other, name, a)
+        self.npart, other.npart = other.npart, self.npart
+
     # -- host transfer (tests, init) ----------------------------------------
     def arrays(self) -> dict[str, torch.Tensor]:
         names = list(_INT_NAMES[: self.dim]) + list(_INT_NAMES[3:3 + self.dim])

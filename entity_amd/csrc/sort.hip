@@ -186,6 +186,30 @@ __global__ __launch_bounds__(kB) void k_copy_batch(ArrayBatch b, const char* __r
   }
 }
 
+// out of place: dst[k][p] = src[k][perm[p]] for up to kDirect arrays per pass
+constexpr int kDirect = 8;
+struct ArrayPairs {
+  const void* src[kDirect];
+  void*       dst[kDirect];
+  int32_t     bytes[kDirect];
+  int32_t     count;
+};
+
+__global__ __launch_bounds__(kB) void k_permute_direct(ArrayPairs b,
+                                                       const uint32_t* __restrict__ perm,
+                                                       int64_t n) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (p >= n) return;
+  const int64_t q = perm[p];
+#pragma unroll
+  for (int k = 0; k < kDirect; ++k) {
+    if (k >= b.count) break;
+    if (b.bytes[k] == 2) gather_one<int16_t>(b.dst[k], b.src[k], p, q);
+    else if (b.bytes[k] == 4) gather_one<int32_t>(b.dst[k], b.src[k], p, q);
+    else gather_one<int64_t>(b.dst[k], b.src[k], p, q);
+  }
+}
+
 // exclusive scan of n uint32 in place; tmp holds the block sums of every level
 ntt_status scan_exclusive(uint32_t* data, int64_t n, uint32_t* tmp, hipStream_t st) {
   const int64_t nb = (n + kScanTile - 1) / kScanTile;
@@ -215,7 +239,7 @@ struct Layout {
 int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
 ntt_status make_layout(int64_t npart, int32_t dim, const int32_t* n, const int32_t* tile,
-                       TileKey* K, Layout* L) {
+                       TileKey* K, Layout* L, bool staging = true) {
   if (dim != 2 && dim != 3) return NTT_ERR_UNSUPPORTED;
   if (npart < 0 || npart > 0xFFFFFFFFll) return NTT_ERR_ARG;
   int64_t ntiles = 1;
@@ -237,7 +261,7 @@ ntt_status make_layout(int64_t npart, int32_t dim, const int32_t* n, const int32
   L->counts = off; off = align256(off + 4 * L->nkeys);
   L->cursor = off; off = align256(off + 4 * L->nkeys);
   L->scan   = off; off = align256(off + 4 * scan_tmp_elems(L->nkeys));
-  L->tmp    = off; off = align256(off + 8 * kBatch * npart);
+  L->tmp    = off; off = align256(off + (staging ? 8 * kBatch * npart : 0));
   L->total  = off;
   return NTT_OK;
 }
@@ -490,17 +514,49 @@ extern "C" ntt_status ntt_sort_ntiles(int32_t dim, const int32_t n[3], const int
   return NTT_OK;
 }
 
-extern "C" ntt_status ntt_sort_particles(ntt_particles_t* prtls, int32_t dim, const int32_t n[3],
-                                         const int32_t tile[3], void* workspace,
-                                         uint64_t workspace_bytes, uint32_t* tile_offsets,
-                                         int64_t* npart_alive, void* stream) {
-  if (prtls == nullptr || n == nullptr || tile == nullptr) return NTT_ERR_ARG;
+namespace {
+
+// every SoA array of a species, in the order of SortSeg (coordinates beyond
+// dim are not used)
+void species_arrays(const ntt_particles_t* p, int32_t dim, void* arrs[kSortArrays]) {
+  void* a[kSortArrays] = {
+    p->i1, p->i2, dim == 3 ? p->i3 : nullptr,
+    p->i1_prev, p->i2_prev, dim == 3 ? p->i3_prev : nullptr,
+    p->dx1, p->dx2, dim == 3 ? p->dx3 : nullptr,
+    p->dx1_prev, p->dx2_prev, dim == 3 ? p->dx3_prev : nullptr,
+    p->ux1, p->ux2, p->ux3, p->weight, p->phi, p->tag, p->pld_i
+  };
+  for (int k = 0; k < kSortArrays; ++k) arrs[k] = a[k];
+}
+constexpr int32_t kArrBytes[kSortArrays] = { 4, 4, 4, 4, 4, 4, 4, 4, 4, 4,
+                                            4, 4, 4, 4, 4, 4, 4, 2, 8 };
+
+// ntt_sort_particles (dst == nullptr: in place, through the staging buffer)
+// and ntt_sort_particles_to (dst: every array gathered straight into dst)
+ntt_status sort_species(ntt_particles_t* src, ntt_particles_t* dst, int32_t dim,
+                        const int32_t n[3], const int32_t tile[3], void* workspace,
+                        uint64_t workspace_bytes, uint32_t* tile_offsets,
+                        int64_t* npart_alive, void* stream) {
+  if (src == nullptr || n == nullptr || tile == nullptr) return NTT_ERR_ARG;
   ParticleView pv;
-  ntt_status s = make_particle_view(prtls, dim, true, &pv);
+  ntt_status s = make_particle_view(src, dim, true, &pv);
   if (s != NTT_OK) return s;
+  void* sa[kSortArrays];
+  void* da[kSortArrays];
+  species_arrays(src, dim, sa);
+  if (dst != nullptr) {
+    if (dst->maxnpart > 0 && dst->maxnpart < src->npart) return NTT_ERR_CAPACITY;
+    species_arrays(dst, dim, da);
+    for (int k = 0; k < kSortArrays; ++k) {
+      if (sa[k] == nullptr) continue;
+      if (da[k] == nullptr) return NTT_ERR_ARG;
+      for (int j = 0; j < kSortArrays; ++j)
+        if (sa[j] != nullptr && da[k] == sa[j]) return NTT_ERR_ARG;  // no aliasing
+    }
+  }
   TileKey K;
   Layout  L;
-  s = make_layout(pv.n, dim, n, tile, &K, &L);
+  s = make_layout(pv.n, dim, n, tile, &K, &L, dst == nullptr);
   if (s != NTT_OK) return s;
   if (pv.n == 0) {
     if (tile_offsets != nullptr) {
@@ -535,16 +591,24 @@ extern "C" ntt_status ntt_sort_particles(ntt_particles_t* prtls, int32_t dim, co
       hipMemcpyAsync(tile_offsets, cnt, 4 * L.nkeys, hipMemcpyDeviceToDevice, st) != hipSuccess)
     return NTT_ERR_HIP;
 
-  // every SoA array of the species (coordinates beyond dim are not used)
-  void* arrs[19] = {
-    prtls->i1, prtls->i2, dim == 3 ? prtls->i3 : nullptr,
-    prtls->i1_prev, prtls->i2_prev, dim == 3 ? prtls->i3_prev : nullptr,
-    prtls->dx1, prtls->dx2, dim == 3 ? prtls->dx3 : nullptr,
-    prtls->dx1_prev, prtls->dx2_prev, dim == 3 ? prtls->dx3_prev : nullptr,
-    prtls->ux1, prtls->ux2, prtls->ux3, prtls->weight, prtls->phi, prtls->tag, prtls->pld_i
-  };
-  const int32_t bytes[19] = { 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 2, 8 };
-  permute_all(arrs, bytes, 19, static_cast<char*>(tmp), perm, np, st);
+  if (dst == nullptr) {
+    permute_all(sa, kArrBytes, kSortArrays, static_cast<char*>(tmp), perm, np, st);
+  } else {
+    ArrayPairs b{};
+    for (int k = 0; k <= kSortArrays; ++k) {
+      if (k < kSortArrays && sa[k] != nullptr) {
+        b.src[b.count]   = sa[k];
+        b.dst[b.count]   = da[k];
+        b.bytes[b.count] = kArrBytes[k];
+        ++b.count;
+      }
+      if (b.count == kDirect || (k == kSortArrays && b.count > 0)) {
+        k_permute_direct<<<blocks_for(np), kB, 0, st>>>(b, perm, np);
+        b = ArrayPairs{};
+      }
+    }
+    dst->npart = src->npart;
+  }
   s = check_launch();
   if (s != NTT_OK) return s;
   if (npart_alive != nullptr) {
@@ -555,5 +619,37 @@ extern "C" ntt_status ntt_sort_particles(ntt_particles_t* prtls, int32_t dim, co
     if (hipStreamSynchronize(st) != hipSuccess) return NTT_ERR_HIP;
     *npart_alive = alive;
   }
+  return NTT_OK;
+}
+
+}  // namespace
+
+extern "C" ntt_status ntt_sort_particles(ntt_particles_t* prtls, int32_t dim, const int32_t n[3],
+                                         const int32_t tile[3], void* workspace,
+                                         uint64_t workspace_bytes, uint32_t* tile_offsets,
+                                         int64_t* npart_alive, void* stream) {
+  return sort_species(prtls, nullptr, dim, n, tile, workspace, workspace_bytes, tile_offsets,
+                      npart_alive, stream);
+}
+
+extern "C" ntt_status ntt_sort_particles_to(const ntt_particles_t* src, ntt_particles_t* dst,
+                                            int32_t dim, const int32_t n[3],
+                                            const int32_t tile[3], void* workspace,
+                                            uint64_t workspace_bytes, uint32_t* tile_offsets,
+                                            int64_t* npart_alive, void* stream) {
+  if (dst == nullptr) return NTT_ERR_ARG;
+  return sort_species(const_cast<ntt_particles_t*>(src), dst, dim, n, tile, workspace,
+                      workspace_bytes, tile_offsets, npart_alive, stream);
+}
+
+extern "C" ntt_status ntt_sort_workspace_size_to(int64_t maxnpart, int32_t dim,
+                                                 const int32_t n[3], const int32_t tile[3],
+                                                 uint64_t* bytes) {
+  if (n == nullptr || tile == nullptr || bytes == nullptr) return NTT_ERR_ARG;
+  TileKey K;
+  Layout  L;
+  ntt_status s = make_layout(maxnpart, dim, n, tile, &K, &L, false);
+  if (s != NTT_OK) return s;
+  *bytes = static_cast<uint64_t>(L.total);
   return NTT_OK;
 }

@@ -89,7 +89,11 @@ class SRParams:
 
 class SRPICEngine:
     def __init__(self, params: SRParams, species: list[SpeciesSpec], maxnpart: int | list,
-                 *, device="cuda", with_payload=False, strict=False):
+                 *, device="cuda", with_payload=False, strict=False, double_buffer=False):
+        """double_buffer: re-sort out of place (ntt_sort_particles_to) into a
+        second set of particle arrays, then swap the sets -- about half the
+        memory traffic of the in-place sort, for twice the particle memory.
+        The species' array attributes then change at every re-sort."""
         self.params = params
         self.device = torch.device(device)
         self.shape = GridShape(params.dim, tuple(params.n))
@@ -102,6 +106,8 @@ class SRPICEngine:
             for s, m in zip(species, maxnpart)]
         self.step = 0
         self.time = 0.0
+        self.double_buffer = double_buffer
+        self._alt = None        # the second set of particle arrays (double_buffer)
         # bound on |weight| of every particle (sets the current's fixed-point scale)
         self.weight_max = 1.0
         self._sort_ws = None
@@ -221,21 +227,34 @@ class SRPICEngine:
         if nt != self.ntiles:   # the tile shape changed: offsets of the new size
             self.ntiles = nt
             self.tile_offsets = [None] * len(self.species)
-        need = max(_sort_ws_bytes(self.lib, sp.maxnpart, p.dim, n, tile) for sp in self.species)
+        need = max(_sort_ws_bytes(self.lib, sp.maxnpart, p.dim, n, tile, self.double_buffer)
+                   for sp in self.species)
         if self._sort_ws is None or self._sort_ws.numel() < need:
             self._sort_ws = torch.empty(need, dtype=torch.uint8, device=self.device)
+        if self.double_buffer and self._alt is None:
+            self._alt = [Particles(p.dim, sp.maxnpart, device=self.device, label=sp.label,
+                                   mass=sp.mass, charge=sp.charge, pusher=sp.pusher,
+                                   with_phi=sp.phi is not None, with_payload=sp.pld_i is not None)
+                         for sp in self.species]
         for k, sp in enumerate(self.species):
             abi = sp.abi()
             alive = C.c_int64(0)
             if self.tile_offsets[k] is None:
                 self.tile_offsets[k] = torch.zeros(self.ntiles + 1, dtype=torch.int32,
                                                    device=self.device)
-            _abi.check(self.lib.ntt_sort_particles(C.byref(abi), p.dim, n, tile,
-                                                   self._sort_ws.data_ptr(),
-                                                   self._sort_ws.numel(),
-                                                   self.tile_offsets[k].data_ptr(),
-                                                   C.byref(alive) if compact else None,
-                                                   _stream(stream)), "ntt_sort_particles")
+            common = (p.dim, n, tile, self._sort_ws.data_ptr(), self._sort_ws.numel(),
+                      self.tile_offsets[k].data_ptr(), C.byref(alive) if compact else None,
+                      _stream(stream))
+            if self.double_buffer:
+                alt = self._alt[k]
+                dst = alt.abi()
+                _abi.check(self.lib.ntt_sort_particles_to(C.byref(abi), C.byref(dst), *common),
+                           "ntt_sort_particles_to")
+                alt.npart = sp.npart      # dst now holds the sorted species
+                sp.swap_arrays(alt)
+            else:
+                _abi.check(self.lib.ntt_sort_particles(C.byref(abi), *common),
+                           "ntt_sort_particles")
             if compact:
                 sp.npart = int(alive.value)
 
@@ -248,10 +267,10 @@ def _ntiles(lib, p: SRParams) -> int:
     return int(v.value)
 
 
-def _sort_ws_bytes(lib, maxnpart, dim, n, tile) -> int:
+def _sort_ws_bytes(lib, maxnpart, dim, n, tile, out_of_place=False) -> int:
     b = C.c_uint64(0)
-    _abi.check(lib.ntt_sort_workspace_size(maxnpart, dim, n, tile, C.byref(b)),
-               "ntt_sort_workspace_size")
+    fn = lib.ntt_sort_workspace_size_to if out_of_place else lib.ntt_sort_workspace_size
+    _abi.check(fn(maxnpart, dim, n, tile, C.byref(b)), "ntt_sort_workspace_size")
     return int(b.value)
 
 

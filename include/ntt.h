@@ -514,6 +514,31 @@ NTT_API ntt_status ntt_sort_particles(ntt_particles_t* prtls,
                                       int64_t*         npart_alive,
                                       void*            stream);
 
+/* The same sort out of place: every array of src (npart entries) is gathered
+ * in tile order straight into the matching array of dst, which must not
+ * alias any array of src and must hold npart entries; dst->npart is set.
+ * It moves each array once instead of staging it, about half the memory
+ * traffic of ntt_sort_particles, at the price of a second set of arrays (the
+ * caller swaps the two sets).  `workspace` must hold
+ * ntt_sort_workspace_size_to() bytes (no staging buffer).  Same key, tile
+ * offsets and npart_alive as ntt_sort_particles.  No reference counterpart
+ * as an interface (the reference sorts in place). */
+NTT_API ntt_status ntt_sort_particles_to(const ntt_particles_t* src,
+                                         ntt_particles_t*       dst,
+                                         int32_t                dim,
+                                         const int32_t          n[3],
+                                         const int32_t          tile[3],
+                                         void*                  workspace,
+                                         uint64_t               workspace_bytes,
+                                         uint32_t*              tile_offsets,
+                                         int64_t*               npart_alive,
+                                         void*                  stream);
+NTT_API ntt_status ntt_sort_workspace_size_to(int64_t       maxnpart,
+                                              int32_t       dim,
+                                              const int32_t n[3],
+                                              const int32_t tile[3],
+                                              uint64_t*     bytes);
+
 /* number of tiles of the key used by ntt_sort_particles (tile_offsets of the
  * sort then holds ntiles + 1 entries, the last one counting the live particles) */
 NTT_API ntt_status ntt_sort_ntiles(int32_t dim, const int32_t n[3], const int32_t tile[3],
