@@ -20,13 +20,21 @@ BOXES = [  # (n, order, pusher, filters, ppc, theta, drift, seed)
 ]
 
 
+BOXES_3D = [
+    ((16, 12, 8), 2, VAY, 2, 4, 0.3, 0.3, 11),
+    ((12, 8, 16), 1, BORIS, 1, 3, 0.1, 0.6, 12),
+]
+
+
 def make_box(spec, sort_interval=4):
     from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
     n, order, alg, nf, ppc, theta, drift, seed = spec
-    prm = SRParams(dim=2, n=n, shape_order=order, n_filters=nf, ppc0=2.0 * ppc, tile=(8, 8),
+    dim = len(n)
+    prm = SRParams(dim=dim, n=n, shape_order=order, n_filters=nf, ppc0=2.0 * ppc,
+                   tile=(8, 8) if dim == 2 else (4, 4, 4),
                    larmor0=0.5, skindepth0=2.0, sort_interval=sort_interval)
     eng = SRPICEngine(prm, [SpeciesSpec("e-", 1.0, -1.0, alg), SpeciesSpec("e+", 1.0, 1.0, alg)],
-                      n[0] * n[1] * ppc)
+                      int(np.prod(n)) * ppc)
     eng.inject_uniform(ppc, temperature=theta, drifts=[(drift, 0, 0), (-drift, 0, 0)], seed=seed)
     rng = np.random.default_rng(seed)
     em = (rng.standard_normal(eng.fields.em.shape) * 0.05).astype(np.float32)
@@ -37,6 +45,8 @@ def make_box(spec, sort_interval=4):
 def rows(host):
     keys = ["i1", "i2", "dx1", "dx2", "i1_prev", "i2_prev", "dx1_prev", "dx2_prev",
             "ux1", "ux2", "ux3", "weight", "tag"]
+    if "i3" in host:
+        keys += ["i3", "dx3", "i3_prev", "dx3_prev"]
     m = np.stack([np.asarray(host[k]).astype(np.float64) for k in keys], 1)
     return m[np.lexsort(m.T[::-1])]
 
@@ -152,3 +162,27 @@ def test_ensemble_graph_replays_with_resorts(hip):
         for a, b in zip(ens_boxes, ref_boxes):
             assert_same(a, b)
     assert device_errors() == 0
+
+
+def test_ensemble_mixes_2d_and_3d_boxes(hip):
+    """2D and 3D boxes (different dimension groups of the batched fused
+    kernel, and 3D tiles in the batched re-sort) in one ensemble, replayed
+    from graphs with re-sorts every 2 steps: every box equals itself stepped
+    alone."""
+    from entity_amd.ensemble import SREnsemble
+    from entity_amd.kernels import device_errors
+    specs = BOXES[:2] + BOXES_3D
+    ens_boxes = [make_box(s, sort_interval=2) for s in specs]
+    ref_boxes = [make_box(s, sort_interval=2) for s in specs]
+    ens = SREnsemble(ens_boxes)
+    ens.step_forward()
+    for b in ref_boxes:
+        b.step_forward()
+    ens.capture()
+    for _ in range(5):
+        ens.step_graph()
+        for b in ref_boxes:
+            b.step_forward()
+    assert device_errors() == 0
+    for a, b in zip(ens_boxes, ref_boxes):
+        assert_same(a, b)
