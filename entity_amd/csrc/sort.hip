@@ -19,8 +19,8 @@ namespace ntt {
 namespace {
 
 constexpr int kB = 256;
-// out-of-place permutation by destination (coalesced reads, writes scattered
-// inside a tile): measured faster than by source at 3D order 2, 1.07e9 particles
+// permutations by destination (coalesced reads, writes scattered inside a
+// tile): measured faster than by source at 3D order 2, 1.07e9 particles
 #ifndef NTT_SORT_SCATTER
 #define NTT_SORT_SCATTER 1
 #endif
@@ -142,8 +142,8 @@ __global__ __launch_bounds__(kB) void k_scatter(uint32_t* __restrict__ keys, int
   const uint32_t key    = active ? keys[p] : 0u;
   const uint32_t pos    = aggregated_increment(cursor, key, active);
   if (active) {
-    perm[pos] = static_cast<uint32_t>(p);
     if (inverse) keys[p] = pos;  // the key is not needed any more: keep the destination
+    else perm[pos] = static_cast<uint32_t>(p);
   }
 }
 
@@ -164,19 +164,22 @@ __device__ __forceinline__ void gather_one(void* dst, const void* src, int64_t t
   static_cast<T*>(dst)[to] = static_cast<const T*>(src)[from];
 }
 
+// stage the permuted arrays: slot[p] = arr[perm[p]], or by destination
+// (by_dest) slot[inv[p]] = arr[p] with coalesced reads
 __global__ __launch_bounds__(kB) void k_gather_batch(ArrayBatch b, char* __restrict__ tmp,
                                                      const uint32_t* __restrict__ perm,
-                                                     int64_t n) {
+                                                     int64_t n, int by_dest) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
   if (p >= n) return;
-  const int64_t q = perm[p];
+  const int64_t q  = perm[p];
+  const int64_t to = by_dest ? q : p, from = by_dest ? p : q;
 #pragma unroll
   for (int k = 0; k < kBatch; ++k) {
     if (k >= b.count) break;
     void* slot = tmp + n * b.off[k];
-    if (b.bytes[k] == 2) gather_one<int16_t>(slot, b.arr[k], p, q);
-    else if (b.bytes[k] == 4) gather_one<int32_t>(slot, b.arr[k], p, q);
-    else gather_one<int64_t>(slot, b.arr[k], p, q);
+    if (b.bytes[k] == 2) gather_one<int16_t>(slot, b.arr[k], to, from);
+    else if (b.bytes[k] == 4) gather_one<int32_t>(slot, b.arr[k], to, from);
+    else gather_one<int64_t>(slot, b.arr[k], to, from);
   }
 }
 
@@ -294,7 +297,7 @@ unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kB - 1) / kB)
 
 // applies perm to every listed array (dst[p] = src[perm[p]]), kBatch at a time
 void permute_all(void* const* arrs, const int32_t* bytes, int narr, char* tmp,
-                 const uint32_t* perm, int64_t n, hipStream_t st) {
+                 const uint32_t* perm, int64_t n, hipStream_t st, int by_dest) {
   ArrayBatch b{};
   for (int k = 0; k <= narr; ++k) {
     if (k < narr && arrs[k] != nullptr) {
@@ -306,7 +309,7 @@ void permute_all(void* const* arrs, const int32_t* bytes, int narr, char* tmp,
       ++b.count;
     }
     if (b.count == kBatch || (k == narr && b.count > 0)) {
-      k_gather_batch<<<blocks_for(n), kB, 0, st>>>(b, tmp, perm, n);
+      k_gather_batch<<<blocks_for(n), kB, 0, st>>>(b, tmp, perm, n, by_dest);
       k_copy_batch<<<blocks_for(n), kB, 0, st>>>(b, tmp, n);
       b = ArrayBatch{};
     }
@@ -391,17 +394,21 @@ __device__ __forceinline__ void move_elem(void* dst, int64_t to, const void* src
 }
 
 // arrays a0 .. a0 + kBatch - 1 of every segment: tmp slot k <- src[perm]
+// by_dest: perm holds each particle's global destination, and the slot is
+// written there (coalesced reads); otherwise the slot gathers from perm
 __global__ __launch_bounds__(kB) void k_gather_batched(const SortSeg* __restrict__ segs,
                                                        const int64_t* __restrict__ pbase,
                                                        const int64_t* __restrict__ bbase, int nseg,
                                                        int64_t ptot, int a0,
                                                        const uint32_t* __restrict__ perm,
-                                                       char* __restrict__ tmp) {
+                                                       char* __restrict__ tmp, int by_dest) {
   const SegBlock b  = seg_block(bbase, nseg);
   const SortSeg& sg = segs[b.s];
   if (b.p >= sg.n) return;
-  const int64_t g = pbase[b.s] + b.p;
-  const int64_t q = static_cast<int64_t>(perm[g]) - pbase[b.s];
+  const int64_t g0 = pbase[b.s] + b.p;
+  // g: slot index (global), q: source index (in the segment)
+  const int64_t g = by_dest ? static_cast<int64_t>(perm[g0]) : g0;
+  const int64_t q = by_dest ? b.p : static_cast<int64_t>(perm[g0]) - pbase[b.s];
   int64_t off = 0;  // slot offset in bytes per particle (the element sizes before)
 #pragma unroll
   for (int k = 0; k < kBatch; ++k) {
@@ -503,10 +510,11 @@ ntt_status sort_batched(const SortSeg* segs, const int64_t* pbase, const int64_t
   if (s != NTT_OK) return s;
   if (hipMemcpyAsync(cur, cnt, 4 * ktot, hipMemcpyDeviceToDevice, st) != hipSuccess)
     return NTT_ERR_HIP;
-  k_scatter<<<blocks_for(ptot), kB, 0, st>>>(keys, ptot, cur, perm, 0);
+  k_scatter<<<blocks_for(ptot), kB, 0, st>>>(keys, ptot, cur, perm, NTT_SORT_SCATTER);
   k_offsets_batched<<<blocks_for(ktot), kB, 0, st>>>(segs, pbase, kbase, nseg, ktot, cnt);
   for (int a0 = 0; a0 < kSortArrays; a0 += kBatch) {
-    k_gather_batched<<<nb, kB, 0, st>>>(segs, pbase, bbase, nseg, ptot, a0, perm, tmp);
+    k_gather_batched<<<nb, kB, 0, st>>>(segs, pbase, bbase, nseg, ptot, a0,
+                                        NTT_SORT_SCATTER ? keys : perm, tmp, NTT_SORT_SCATTER);
     k_copy_batched<<<nb, kB, 0, st>>>(segs, pbase, bbase, nseg, ptot, a0, tmp);
   }
   return check_launch();
@@ -610,14 +618,14 @@ ntt_status sort_species(ntt_particles_t* src, ntt_particles_t* dst, int32_t dim,
   if (s != NTT_OK) return s;
   if (hipMemcpyAsync(cur, cnt, 4 * L.nkeys, hipMemcpyDeviceToDevice, st) != hipSuccess)
     return NTT_ERR_HIP;
-  k_scatter<<<blocks_for(np), kB, 0, st>>>(keys, np, cur, perm,
-                                            dst != nullptr && NTT_SORT_SCATTER);
+  k_scatter<<<blocks_for(np), kB, 0, st>>>(keys, np, cur, perm, NTT_SORT_SCATTER);
   if (tile_offsets != nullptr &&
       hipMemcpyAsync(tile_offsets, cnt, 4 * L.nkeys, hipMemcpyDeviceToDevice, st) != hipSuccess)
     return NTT_ERR_HIP;
 
   if (dst == nullptr) {
-    permute_all(sa, kArrBytes, kSortArrays, static_cast<char*>(tmp), perm, np, st);
+    permute_all(sa, kArrBytes, kSortArrays, static_cast<char*>(tmp),
+                NTT_SORT_SCATTER ? keys : perm, np, st, NTT_SORT_SCATTER);
   } else {
     ArrayPairs b{};
     for (int k = 0; k <= kSortArrays; ++k) {
