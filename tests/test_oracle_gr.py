@@ -99,6 +99,38 @@ def test_textbook_kerr_schild(oracle, name):
     np.testing.assert_allclose(m["beta1"] * Jr, z / (1 + z), rtol=1e-12, atol=1e-300)
 
 
+@pytest.mark.parametrize("case", ["ks", "qks"])
+def test_reference_ks_qks_cases(oracle, case):
+    """The two grids of the reference's own metric test (tests/metrics/ks-qks.cpp:177-193):
+    KerrSchild 64 x 54, r in [0.8, 50], a = 0.95, and QKerrSchild 32 x 42,
+    r in [0.8, 10], r0 = -2, h = 0, a = 0.8; at every cell centre h_ij and
+    h^ij equal the textbook Kerr-Schild expressions (the reference checks them
+    in the physical basis to 10 ulp; here in code coordinates to 1e-12)."""
+    import ntt_oracle as o
+    if case == "ks":
+        prm = o.gr_params("kerr_schild", (64, 54), ((0.8, 50.0), (0.0, math.pi)), a=0.95)
+    else:
+        prm = o.gr_params("qkerr_schild", (32, 42), ((0.8, 10.0), (0.0, math.pi)), a=0.8,
+                          r0=-2.0, h=0.0)
+    i, j = np.meshgrid(np.arange(prm.nx1), np.arange(prm.nx2), indexing="ij")
+    x1, x2 = i.ravel() + 0.5, j.ravel() + 0.5
+    m = oracle.gr_metric(prm, x1, x2)
+    r, th, a = m["r"], m["theta"], prm.a
+    S = r ** 2 + (a * np.cos(th)) ** 2
+    z = 2 * r / S
+    D = r ** 2 - 2 * r + a ** 2
+    A = (r ** 2 + a ** 2) ** 2 - (a * np.sin(th)) ** 2 * D
+    Jr, Jt, _, _ = _jacobian(prm, x1, x2)
+    np.testing.assert_allclose(m["h_11"], (1 + z) * Jr ** 2, rtol=1e-12)
+    np.testing.assert_allclose(m["h_22"], S * Jt ** 2, rtol=1e-12)
+    np.testing.assert_allclose(m["h_33"], A * np.sin(th) ** 2 / S, rtol=1e-12)
+    np.testing.assert_allclose(m["h_13"], -a * (1 + z) * np.sin(th) ** 2 * Jr, rtol=1e-12)
+    np.testing.assert_allclose(m["h11"], A / (S * (S + 2 * r)) / Jr ** 2, rtol=1e-12)
+    np.testing.assert_allclose(m["h22"], 1 / S / Jt ** 2, rtol=1e-12)
+    np.testing.assert_allclose(m["h33"], 1 / (S * np.sin(th) ** 2), rtol=1e-12)
+    np.testing.assert_allclose(m["h13"], a / S / Jr, rtol=1e-12)
+
+
 @pytest.mark.parametrize("name", list(METRICS))
 def test_upper_is_inverse_of_lower(oracle, name):
     prm = METRICS[name]()
