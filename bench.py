@@ -235,12 +235,70 @@ def batched_leg(args, sizes=(64, 128, 256, 512), label=None):
             "speedup_vs_plain_loop": t_plain / t_batched}
 
 
+def _qks(N, rmin, rmax, a):
+    """Quasi-spherical Kerr-Schild grid with h = 0, r0 = 0 (r = e^chi, chi uniform
+    in x1; theta uniform in x2), in float64 numpy: functions of code coordinates
+    (x1, x2) returning r, theta, dr/dx1, dth/dx2, h_11, h_22, h_33, h_13,
+    beta^1 (kerr_schild.h:143-330 and qkerr_schild.h:84-93 as formulas)."""
+    import numpy as np
+    chi0, chi1 = np.log(rmin), np.log(rmax)
+    dchi, dth = (chi1 - chi0) / N[0], np.pi / N[1]
+
+    def m(x1, x2):
+        r = np.exp(chi0 + x1 * dchi)
+        th = x2 * dth
+        drdx = r * dchi
+        S = r * r + (a * np.cos(th)) ** 2
+        z = 2 * r / S
+        D = r * r - 2 * r + a * a
+        A = (r * r + a * a) ** 2 - (a * np.sin(th)) ** 2 * D
+        s2 = np.sin(th) ** 2
+        return dict(r=r, th=th, drdx=drdx, dthdx=dth, h11=(1 + z) * drdx ** 2, h22=S * dth ** 2,
+                    h33=A * s2 / S, h13=-a * (1 + z) * s2 * drdx, beta1=z / (1 + z) / drdx)
+    return m
+
+
+def wald_vertical_b(N, rmin, rmax, a, B0=1.0):
+    """Contravariant code components of Wald's uniform field aligned with the
+    spin (pgens/wald/pgen.hpp:41-110): A_phi = B0/2 (h_33 + 2 a h_13 beta^1),
+    A_r = B0/2 (h_13 + 2 a h_11 beta^1), B^i = curl A / sqrt(det h), each on
+    its staggered position with ghost cells (2), as float32 arrays [3][N2+4][N1+4]."""
+    import numpy as np
+    m = _qks(N, rmin, rmax, a)
+    G = 2
+    i = np.arange(N[0] + 2 * G, dtype=np.float64)[None, :] - G
+    j = np.arange(N[1] + 2 * G, dtype=np.float64)[:, None] - G
+
+    def A3(x1, x2):
+        q = m(x1, x2)
+        return 0.5 * B0 * (q["h33"] + 2 * a * q["h13"] * q["beta1"])
+
+    def A1(x1, x2):
+        q = m(x1, x2)
+        return 0.5 * B0 * (q["h13"] + 2 * a * q["h11"] * q["beta1"])
+
+    def sqrt_det(x1, x2):
+        q = m(x1, x2)
+        return np.sqrt(q["h22"] * (q["h11"] * q["h33"] - q["h13"] ** 2))
+
+    with np.errstate(divide="ignore", invalid="ignore"):
+        b1 = (A3(i, j + 1.0) - A3(i, j)) / sqrt_det(i, j + 0.5)            # at (i, j + 1/2)
+        b2 = -(A3(i + 1.0, j) - A3(i, j)) / sqrt_det(i + 0.5, j)          # at (i + 1/2, j)
+        b3 = -(A1(i + 0.5, j + 1.0) - A1(i + 0.5, j)) / sqrt_det(i + 0.5, j + 0.5)
+    b2 = np.where(np.isfinite(b2), b2, 0.0)  # on the axis rows
+    b1 = np.where(np.isfinite(b1), b1, 0.0)
+    b3 = np.where(np.isfinite(b3), b3, 0.0)
+    return np.stack([b1 + 0 * i, b2 + 0 * j, b3]).astype(np.float32), m
+
+
 def gr_leg(args):
     """GR pusher: 16 runs on 768 x 768 quasi-spherical Kerr-Schild grids
     (r in [0.9 r_h, 50], theta in [0, pi], h = 0), spins linspace(0, 0.99, 16),
-    e-/e+ at 8 ppc per species, theta = 0.1 covariant thermal momenta,
-    synthetic static D and B, FP32 storage with FP64 metric and pusher, first
-    order shape, 10 iterations, absorbing r boundaries and axis reflection."""
+    Wald's uniform magnetic field aligned with the spin (B0 = 1, no electric
+    field), e-/e+ at 8 ppc per species with theta = 0.1 thermal momenta in the
+    local frame (covariant code components scaled by sqrt(h_ii)), FP32 storage
+    with FP64 metric and pusher, first order shape, 10 iterations, absorbing r
+    boundaries and axis reflection."""
     import numpy as np
     import torch
     from entity_amd import _abi
@@ -258,8 +316,19 @@ def gr_leg(args):
     for r, a in enumerate(np.linspace(0.0, 0.99, 16)):
         rh = 1 + np.sqrt(1 - a * a)
         met = gr.KerrSchild(_abi.METRIC_QKERR_SCHILD, N, (0.9 * rh, 50.0), (0.0, np.pi), a=float(a))
-        em = (torch.randn((6, ext[1], ext[0]), device=dev, generator=gen) * 0.01)
-        em0 = (torch.randn((6, ext[1], ext[0]), device=dev, generator=gen) * 0.01)
+        bw, mfun = wald_vertical_b(N, 0.9 * rh, 50.0, float(a))
+        em = torch.zeros((6, ext[1], ext[0]), device=dev)
+        em[3:6] = torch.from_numpy(bw).to(dev)
+        em0 = em.clone()
+        # sqrt(h_ii) at the cell centres: local-frame momenta to covariant code components
+        ic = np.arange(N[0]) + 0.5
+        jc = np.arange(N[1]) + 0.5
+        q = mfun(ic[None, :], jc[:, None])
+        sc = [torch.from_numpy(np.sqrt(q[k]).astype(np.float32).ravel()).to(dev)
+              for k in ("h11", "h22", "h33")]
+        sig = float(np.sqrt(0.1))   # theta = 0.1, non-relativistic thermal spread
+        # CFL 0.5 on the smallest cell (the innermost ring)
+        dt = float(0.5 * min(q["drdx"].min(), (q["r"] * q["dthdx"]).min()) / np.sqrt(2))
         sps = []
         for q in (-1.0, 1.0):
             P = Particles(2, npart, device=dev, with_phi=True)
@@ -267,12 +336,12 @@ def gr_leg(args):
             P.load(i1=(cell % N[0]).to(torch.int32), i2=(cell // N[0]).to(torch.int32),
                    dx1=torch.rand(npart, device=dev, generator=gen),
                    dx2=torch.rand(npart, device=dev, generator=gen),
-                   ux1=torch.randn(npart, device=dev, generator=gen) * 0.3,
-                   ux2=torch.randn(npart, device=dev, generator=gen) * 0.3,
-                   ux3=torch.randn(npart, device=dev, generator=gen) * 0.3,
+                   ux1=torch.randn(npart, device=dev, generator=gen) * sig * sc[0][cell],
+                   ux2=torch.randn(npart, device=dev, generator=gen) * sig * sc[1][cell],
+                   ux3=torch.randn(npart, device=dev, generator=gen) * sig * sc[2][cell],
                    weight=torch.ones(npart, device=dev), tag=torch.ones(npart, dtype=torch.int16,
                                                                         device=dev))
-            ctx = gr.PusherContext(_abi.PUSHER_BORIS, 1.0, q, 0.02, 1.0, N[0], N[1])
+            ctx = gr.PusherContext(_abi.PUSHER_BORIS, 1.0, q, dt, 1.0, N[0], N[1])
             bc = PusherBoundaries([(_abi.PRTL_BC_ABSORB, _abi.PRTL_BC_ABSORB),
                                    (_abi.PRTL_BC_AXIS, _abi.PRTL_BC_AXIS), (0, 0)])
             sps.append(gr.Pusher_kernel(ctx, bc, P, em, em0, shape, met, 1))
@@ -292,8 +361,9 @@ def gr_leg(args):
     torch.cuda.synchronize()
     ms = s0.elapsed_time(s1) / K
     total = 16 * 2 * npart
-    return {"workload": "16 QKS runs 768x768, spins 0..0.99, e-/e+ 8 ppc/species "
-                        "(151M particles), FP32 storage / FP64 metric, 10 iterations",
+    return {"workload": "16 QKS runs 768x768 (r in [0.9 r_h, 50]), spins 0..0.99, Wald field "
+                        "B0 = 1, e-/e+ 8 ppc/species theta 0.1 (151M particles), CFL 0.5, "
+                        "FP32 storage / FP64 metric, 10 iterations",
             "particles": total, "steps": K, "value": total / (ms * 1e-3),
             "unit": "particle-steps/s", "ms_per_step": ms,
             "hbm_gbs": total * 78 / (ms * 1e-3) / 1e9}

@@ -54,11 +54,23 @@ class SREnsemble:
                 b.tile[0], b.tile[1], b.tile[2] = t
         if self._plan.value:
             self.lib.ntt_ensemble_destroy(self._plan)
+        # the plan holds the device pointers of every box: remember the
+        # containers' versions to catch arrays replaced behind its back
+        self._versions = self._current_versions()
         plan = C.c_void_p(None)
         _abi.check(self.lib.ntt_ensemble_create(boxes, len(self.engines), C.byref(plan)),
                    "ntt_ensemble_create")
         self._plan = plan
         self._graphs = {}
+
+    def _current_versions(self):
+        return [(e.fields.version, tuple(sp.version for sp in e.species)) for e in self.engines]
+
+    def _check_plan(self) -> None:
+        if self._current_versions() != self._versions:
+            raise RuntimeError("a box's arrays or particle counts changed since the ensemble was "
+                               "planned (e.g. a box stepped or re-sorted on its own): rebuild "
+                               "the SREnsemble")
 
     def __del__(self):
         try:
@@ -86,6 +98,7 @@ class SREnsemble:
 
     def step_forward(self, stream=None) -> None:
         """One step of every box (after the batched re-sort when one is due)."""
+        self._check_plan()
         self._enqueue(self._sort_due(), _stream(stream))
         self._advance_counters()
 
@@ -109,5 +122,6 @@ class SREnsemble:
         """One step by replaying a captured graph."""
         if not self._graphs:
             raise RuntimeError("capture() first")
+        self._check_plan()
         self._graphs[self._sort_due()].replay()
         self._advance_counters()

@@ -186,3 +186,20 @@ def test_ensemble_mixes_2d_and_3d_boxes(hip):
     assert device_errors() == 0
     for a, b in zip(ens_boxes, ref_boxes):
         assert_same(a, b)
+
+
+def test_ensemble_detects_boxes_changed_behind_its_back(hip):
+    """A box re-sorted out of place on its own (its arrays swapped) no longer
+    matches the plan's pointers: the ensemble refuses to step until rebuilt."""
+    from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
+    from entity_amd.ensemble import SREnsemble
+    prm = SRParams(dim=2, n=(32, 32), sort_interval=4, tile=(8, 8))
+    eng = SRPICEngine(prm, [SpeciesSpec("e", 1.0, -1.0)], 32 * 32 * 2, double_buffer=True)
+    eng.inject_uniform(2, temperature=0.1)
+    ens = SREnsemble([eng])
+    ens.step_forward()
+    eng.sort()
+    with pytest.raises(RuntimeError):
+        ens.step_forward()
+    ens._build()
+    ens.step_forward()
