@@ -140,6 +140,62 @@ def test_ten_steps_close(hip, oracle, case):
     np.testing.assert_allclose(flds1["em"], em_ref, rtol=0, atol=1e-4 * escale)
 
 
+def oracle_step_f64(oracle, eng, parts, flds, nsteps):
+    """The oracle in double precision (momenta, weights and fields in FP64;
+    offsets dx stay FP32 as in the reference's FP64 build)."""
+    p = eng.params
+    sp = [{"p": oracle.normalize_particles(host, "f64"), "mass": s.mass, "charge": s.charge,
+           "pusher": s.pusher} for s, host in zip(eng.species, parts)]
+    em = flds["em"].astype(np.float64)
+    cur = flds["cur"].astype(np.float64)
+    buff = np.zeros_like(cur)
+    prm = dict(dt=p.dt, correction=p.correction, omegaB0=p.omegaB0, q0=p.q0, B0=p.B0, V0=p.V0,
+               ppc0=p.ppc0, dx=p.dx, shape_order=p.shape_order, n_filters=p.n_filters,
+               fieldsolver_enable=1, deposit_enable=1)
+    for k in range(nsteps):
+        prm["step"] = k
+        assert oracle.sr_step(em, cur, buff, sp, dim=p.dim, prm=prm, bc=((1, 1),) * 3,
+                              prec="f64") == 0
+    return sp, em
+
+
+@pytest.mark.parametrize("case", [CASES[1], CASES[4]], ids=["2d-o1", "3d-o2-vay"])
+@pytest.mark.parametrize("nsteps", [1, 10])
+def test_steps_vs_fp64_oracle(hip, oracle, case, nsteps):
+    """The product (FP32) build against the FP64 oracle after 1 and 10 steps.
+    Stated tolerance: positions within 2e-5 (1 step) / 2e-4 (10 steps) cells,
+    momenta within 2e-5 / 2e-4 (relative + absolute), fields within 2e-5 / 2e-4
+    of their largest value; after one step the cell indices agree except for
+    particles within 4 float ulps of a cell face."""
+    dim, n, order, alg, nf = case
+    tol = 2e-5 if nsteps == 1 else 2e-4
+    eng = make_engine(dim, n, order, alg, nf)
+    parts0, flds0 = snapshot(eng)
+    for _ in range(nsteps):
+        eng.step_forward()
+    parts1, flds1 = snapshot(eng)
+    sp, em_ref = oracle_step_f64(oracle, eng, parts0, flds0, nsteps)
+    for got, ref in zip(parts1, sp):
+        q = ref["p"]
+        x_got = np.stack([got[f"i{d+1}"] + got[f"dx{d+1}"].astype(np.float64) for d in range(dim)])
+        x_ref = np.stack([q[f"i{d+1}"] + q[f"dx{d+1}"].astype(np.float64) for d in range(dim)])
+        dist = np.abs(x_got - x_ref)
+        for d in range(dim):
+            dist[d] = np.minimum(dist[d], n[d] - dist[d])
+        assert dist.max() < tol, dist.max()
+        for k in ("ux1", "ux2", "ux3"):
+            np.testing.assert_allclose(got[k], q[k], rtol=tol, atol=tol, err_msg=k)
+        if nsteps == 1:
+            flagged = np.zeros(len(q["tag"]), bool)
+            for d in range(dim):
+                flagged |= near_face(q[f"dx{d+1}"]) | near_face(got[f"dx{d+1}"])
+            for d in range(dim):
+                mism = got[f"i{d+1}"] != q[f"i{d+1}"]
+                assert not (mism & ~flagged).any()
+    escale = np.abs(em_ref).max()
+    np.testing.assert_allclose(flds1["em"], em_ref, rtol=0, atol=tol * escale)
+
+
 def _rho(parts, dim, n, order, charges):
     """charge density on primal nodes (periodic), textbook B-splines"""
     so = 1 if order == 0 else order
