@@ -516,6 +516,86 @@ __global__ __launch_bounds__(256) void k_deposit_vjp(ParticleView P, const int64
   post.lu3[q] += acc[6];
 }
 
+// Reverse sweep of Boris (sr.hpp:337-368) and the position update of the 2D
+// push: given the adjoint lo5 of (dx1', dx2', u1', u2', u3'), writes to out[0..8]
+// the adjoint of (u1..3, e1..3, b1..3), e and b being the gathered fields
+// before the in-plane dxm scaling.  A few times the forward arithmetic; equal
+// to the dual-number form to round-off.
+template <typename R>
+__device__ __forceinline__ void boris_vjp_rev(R coeff, R dt, R dxm, const R* u, const R* e,
+                                              const R* b, const R* lo5, R* out) {
+  auto cross = [](const R* x, const R* y, R* z) {
+    z[0] = x[1] * y[2] - x[2] * y[1];
+    z[1] = x[2] * y[0] - x[0] * y[2];
+    z[2] = x[0] * y[1] - x[1] * y[0];
+  };
+  // forward, keeping the intermediates
+  const R es[3] = { e[0] * dxm, e[1] * dxm, e[2] }, bs[3] = { b[0] * dxm, b[1] * dxm, b[2] };
+  R e0[3], u0[3], b0[3], v[3], u1[3], up[3], t[3];
+  for (int c = 0; c < 3; ++c) {
+    e0[c] = coeff * es[c];
+    u0[c] = u[c] + e0[c];
+  }
+  // one division per reciprocal: in double each one is a long instruction chain
+  const R idxm = R(1) / dxm;
+  const R ig0  = R(1) / sqrt(R(1) + (u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2]));
+  const R C1   = coeff * ig0;
+  for (int c = 0; c < 3; ++c) b0[c] = bs[c] * C1;
+  const R C2 = R(2) / (R(1) + (b0[0] * b0[0] + b0[1] * b0[1] + b0[2] * b0[2]));
+  cross(u0, b0, t);
+  for (int c = 0; c < 3; ++c) {
+    v[c]  = u0[c] + t[c];
+    u1[c] = v[c] * C2;
+  }
+  cross(u1, b0, t);
+  for (int c = 0; c < 3; ++c) up[c] = u0[c] + t[c] + e0[c];
+  const R igp  = R(1) / sqrt(R(1) + (up[0] * up[0] + up[1] * up[1] + up[2] * up[2]));
+  const R dtie = dt * igp;
+  // reverse
+  R lup[3] = { lo5[2], lo5[3], lo5[4] };
+  // X_k = up_k / dxm * dtie (k = 1, 2), dtie = dt / gamma(up)
+  lup[0] += lo5[0] * dtie * idxm;
+  lup[1] += lo5[1] * dtie * idxm;
+  const R ldtie = (lo5[0] * up[0] + lo5[1] * up[1]) * idxm;
+  const R kg    = -ldtie * dt * (igp * igp * igp);
+  for (int c = 0; c < 3; ++c) lup[c] += kg * up[c];
+  // up = u0 + u1 x b0 + e0
+  R lu0[3], le0[3], lu1[3], lb0[3];
+  cross(b0, lup, lu1);              // d<l, u1 x b0>/d u1 = b0 x l
+  cross(lup, u1, lb0);              // d<l, u1 x b0>/d b0 = l x u1
+  for (int c = 0; c < 3; ++c) {
+    lu0[c] = lup[c];
+    le0[c] = lup[c];
+  }
+  // u1 = v C2, v = u0 + u0 x b0
+  R lv[3];
+  const R lC2 = lu1[0] * v[0] + lu1[1] * v[1] + lu1[2] * v[2];
+  for (int c = 0; c < 3; ++c) lv[c] = lu1[c] * C2;
+  cross(b0, lv, t);                 // d<lv, u0 x b0>/d u0 = b0 x lv
+  for (int c = 0; c < 3; ++c) lu0[c] += lv[c] + t[c];
+  cross(lv, u0, t);                 // d<lv, u0 x b0>/d b0 = lv x u0
+  for (int c = 0; c < 3; ++c) lb0[c] += t[c];
+  // C2 = 2 / (1 + |b0|^2): dC2/db0 = -C2^2 b0
+  for (int c = 0; c < 3; ++c) lb0[c] += lC2 * (-C2 * C2 * b0[c]);
+  // b0 = bs C1, C1 = coeff / gamma(u0)
+  R lbs[3];
+  const R lC1 = lb0[0] * bs[0] + lb0[1] * bs[1] + lb0[2] * bs[2];
+  for (int c = 0; c < 3; ++c) lbs[c] = lb0[c] * C1;
+  const R k1 = -lC1 * coeff * (ig0 * ig0 * ig0);
+  for (int c = 0; c < 3; ++c) lu0[c] += k1 * u0[c];
+  // u0 = u + e0, e0 = coeff es
+  for (int c = 0; c < 3; ++c) le0[c] += lu0[c];
+  out[0] = lu0[0];
+  out[1] = lu0[1];
+  out[2] = lu0[2];
+  out[3] = coeff * le0[0] * dxm;
+  out[4] = coeff * le0[1] * dxm;
+  out[5] = coeff * le0[2];
+  out[6] = lbs[0] * dxm;
+  out[7] = lbs[1] * dxm;
+  out[8] = lbs[2];
+}
+
 // ---- push VJP -------------------------------------------------------------------
 // Forward of one particle (sr_move, 2D, Boris): e, b gathered at (i, dx),
 // in-plane components times dxm, Boris, dx' = dx + (u'/dxm) dt/gamma'.
@@ -677,33 +757,39 @@ __global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const 
     g[3] = gather(3, ip, jd, S1p, S2d);
     g[4] = gather(4, id, jp, S1d, S2p);
     g[5] = gather(5, id, jd, S1d, S2d);
-    // Boris + position update.  The offsets enter the update only through
-    // the gathered fields (handled by the Dual<2> gather) and additively
-    // (d dx'/d dx = 1), so the duals carry the 9 inputs u1..3, e1..3, b1..3.
-    D9 u[3] = { D9::var(static_cast<R>(P.ux1[p]), 0), D9::var(static_cast<R>(P.ux2[p]), 1),
-                D9::var(static_cast<R>(P.ux3[p]), 2) };
-    D9 e[3], b[3];
-    for (int c = 0; c < 3; ++c) {
-      e[c] = D9::var(g[c].v, 3 + c);
-      b[c] = D9::var(g[3 + c].v, 6 + c);
-    }
-    for (int c = 0; c < 2; ++c) {
-      e[c] = e[c] * dxm;
-      b[c] = b[c] * dxm;
-    }
-    boris_t<D9, R>(ndth, u, e, b);
-    const D9 dtie = dt / tsqrt(R(1) + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
-    const D9 X1 = (u[0] / D9(dxm)) * dtie;   // dx1' - dx1
-    const D9 X2 = (u[1] / D9(dxm)) * dtie;
     const R lo5[5] = { post.lx1[q] + a7[2], post.lx2[q] + a7[3], post.lu1[q] + a7[4],
                        post.lu2[q] + a7[5], post.lu3[q] + a7[6] };
-    // lin = (d/d dx1, d/d dx2, d/du1..3, d/de1..3, d/db1..3) of <lo5, (dx', u')>
+    // lin = (d/d dx1, d/d dx2, d/du1..3, d/de1..3, d/db1..3) of <lo5, (dx', u')>.
+    // The offsets enter the update only through the gathered fields (handled
+    // by the Dual<2> gather) and additively (d dx'/d dx = 1).
     R lin[11];
     lin[0] = lo5[0];
     lin[1] = lo5[1];
-    for (int k = 0; k < 9; ++k)
-      lin[2 + k] = lo5[0] * X1.d[k] + lo5[1] * X2.d[k] + lo5[2] * u[0].d[k] +
-                   lo5[3] * u[1].d[k] + lo5[4] * u[2].d[k];
+    const R uin[3] = { static_cast<R>(P.ux1[p]), static_cast<R>(P.ux2[p]),
+                       static_cast<R>(P.ux3[p]) };
+    const R ein[3] = { g[0].v, g[1].v, g[2].v }, bin[3] = { g[3].v, g[4].v, g[5].v };
+    if constexpr (DEP) {
+      boris_vjp_rev<R>(ndth, dt, dxm, uin, ein, bin, lo5, lin + 2);
+    } else {
+      // forward-mode reference: dual numbers over the 9 inputs u1..3, e1..3, b1..3
+      D9 u[3] = { D9::var(uin[0], 0), D9::var(uin[1], 1), D9::var(uin[2], 2) };
+      D9 e[3], b[3];
+      for (int c = 0; c < 3; ++c) {
+        e[c] = D9::var(ein[c], 3 + c);
+        b[c] = D9::var(bin[c], 6 + c);
+      }
+      for (int c = 0; c < 2; ++c) {
+        e[c] = e[c] * dxm;
+        b[c] = b[c] * dxm;
+      }
+      boris_t<D9, R>(ndth, u, e, b);
+      const D9 dtie = dt / tsqrt(R(1) + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+      const D9 X1 = (u[0] / D9(dxm)) * dtie;   // dx1' - dx1
+      const D9 X2 = (u[1] / D9(dxm)) * dtie;
+      for (int k = 0; k < 9; ++k)
+        lin[2 + k] = lo5[0] * X1.d[k] + lo5[1] * X2.d[k] + lo5[2] * u[0].d[k] +
+                     lo5[3] * u[1].d[k] + lo5[4] * u[2].d[k];
+    }
     // adjoint of the gather: the position derivative of the gathered values,
     // and a scatter into the field adjoint
     R lx1 = lin[0], lx2 = lin[1];
