@@ -241,10 +241,16 @@ template <typename R>
 struct AdjField {
   R*      d;
   int32_t n1, n2;
+  // indices are a few cells outside [0, n): one conditional add or subtract,
+  // and the division only for a lane still outside (n below the reach)
+  __device__ __forceinline__ static int wrap(int x, int n) {
+    x = (x < 0) ? x + n : x;
+    x = (x >= n) ? x - n : x;
+    if (static_cast<unsigned>(x) >= static_cast<unsigned>(n)) x = ((x % n) + n) % n;
+    return x;
+  }
   __device__ __forceinline__ int64_t at(int c, int i, int j) const {
-    i = ((i % n1) + n1) % n1;
-    j = ((j % n2) + n2) % n2;
-    return static_cast<int64_t>(i) + static_cast<int64_t>(n1) * j +
+    return static_cast<int64_t>(wrap(i, n1)) + static_cast<int64_t>(n1) * wrap(j, n2) +
            static_cast<int64_t>(n1) * n2 * c;
   }
   __device__ __forceinline__ R get(int c, int i, int j) const { return d[at(c, i, j)]; }
@@ -366,6 +372,33 @@ __device__ __forceinline__ void shape_d(int i, R di, int& imin, R* S, R* dS) {
   }
 }
 
+// prtl_shape::order<true, O> (particle_shapes.hpp:544-588), the staggered
+// shape, and its derivative in di
+template <int O, typename R>
+__device__ __forceinline__ void shape_sd(int i, R di, int& imin, R* S, R* dS) {
+  if constexpr (O == 1) {
+    if (di < R(0.5)) {
+      imin = i - 1;
+      S[0] = R(0.5) - di;
+    } else {
+      imin = i;
+      S[0] = R(1.5) - di;
+    }
+    S[1]  = R(1) - S[0];
+    dS[0] = R(-1);
+    dS[1] = R(1);
+  } else {
+    imin      = i - 1;
+    const R a = R(1) - di;
+    S[0]      = R(0.5) * (a * a);
+    S[2]      = R(0.5) * (di * di);
+    S[1]      = R(1) - S[0] - S[2];
+    dS[0]     = -a;
+    dS[2]     = di;
+    dS[1]     = -dS[0] - dS[2];
+  }
+}
+
 template <int O, typename R>
 __device__ __forceinline__ void for_deposit_d(int i_init, R di_init, int i_fin, R di_fin,
                                               int& i_min, int& di, R* iS, R* fS, R* diS,
@@ -411,13 +444,16 @@ __device__ __forceinline__ void deposit_vjp_rev(const ParticleView& P, int64_t p
   const R coeff = static_cast<R>(P.w[p]) * charge;
   const R c0    = -(coeff * inv_dt) * R(0.5);
   const R third = static_cast<R>(0.333333f);
-  const bool    inner = i1_min >= 0 && i1_min + M <= lj.n1 && i2_min >= 0 && i2_min + M <= lj.n2;
-  const int64_t base  = static_cast<int64_t>(i1_min) + static_cast<int64_t>(lj.n1) * i2_min;
-  const int64_t cs    = static_cast<int64_t>(lj.n1) * lj.n2;
-  auto lam = [&](int c, int a, int b) -> R {
-    return inner ? lj.d[base + a + static_cast<int64_t>(lj.n1) * b + cs * c]
-                 : lj.get(c, i1_min + a, i2_min + b);
-  };
+  // the slots' periodic columns and row offsets, wrapped once per particle
+  // (offsets within one component stay below n1 * n2 < 2^31)
+  int col[M], row[M];
+#pragma unroll
+  for (int a = 0; a < M; ++a) {
+    col[a] = AdjField<R>::wrap(i1_min + a, lj.n1);
+    row[a] = AdjField<R>::wrap(i2_min + a, lj.n2) * lj.n1;
+  }
+  const int64_t cs = static_cast<int64_t>(lj.n1) * lj.n2;
+  auto lam = [&](int c, int a, int b) -> R { return lj.d[cs * c + (col[a] + row[b])]; };
   // adjoints of the shape values
   R gi1[M], gf1[M], gi2[M], gf2[M];
 #pragma unroll
@@ -610,9 +646,18 @@ __device__ __forceinline__ void boris_vjp_rev(R coeff, R dt, R dxm, const R* u, 
 // 20x the throughput of the 32-bit float LDS atomic on this chip), which is
 // then added to the field adjoint with one global atomic per non-zero entry.
 // Chunks that do not fit scatter with global atomics.
-constexpr int kVjpThreads = 256, kVjpWaves = kVjpThreads / 64, kVjpPerThread = 2;
+// Particles per thread and window side: two particles per thread and a 32 x 32
+// window measured no slower than one per thread with a 24 x 24 window at
+// three or four waves per SIMD (2D order 2, 128 x 128, 262,144 particles).
+#ifndef NTT_VJP_PER_THREAD
+#define NTT_VJP_PER_THREAD 2
+#endif
+constexpr int kVjpThreads = 256, kVjpWaves = kVjpThreads / 64, kVjpPerThread = NTT_VJP_PER_THREAD;
 constexpr int kVjpChunk = kVjpThreads * kVjpPerThread;
-constexpr int kVW = 32;
+#ifndef NTT_VJP_WINDOW
+#define NTT_VJP_WINDOW 32
+#endif
+constexpr int kVW = NTT_VJP_WINDOW;
 
 __device__ __forceinline__ int vjp_wave_min(int v) {
 #pragma unroll
@@ -733,41 +778,89 @@ __global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const 
       continue;
     }
     const int i = P.i1[p] + ngh, j = P.i2[p] + ngh;
-    const D2  x1 = D2::var(static_cast<R>(P.dx1[p]), 0), x2 = D2::var(static_cast<R>(P.dx2[p]), 1);
-    // gather with position derivatives: e_c, b_c as functions of (dx1, dx2)
-    D2  S1p[O + 1], S1d[O + 1], S2p[O + 1], S2d[O + 1];
+    // gathered E, B (gv) and their derivatives in dx1 (g1) and dx2 (g2); the
+    // shapes (values) are kept for the scatter
+    R   gv[6], g1[6], g2[6];
+    R   S1p[O + 1], S1d[O + 1], S2p[O + 1], S2d[O + 1];
     int ip, id, jp, jd;
-    shape_t<false, O, D2, R>(i, x1, ip, S1p);
-    shape_t<false, O, D2, R>(j, x2, jp, S2p);
-    shape_t<true, O, D2, R>(i, x1, id, S1d);
-    shape_t<true, O, D2, R>(j, x2, jd, S2d);
-    auto gather = [&](int c, int imin, int jmin, const D2* A, const D2* B) -> D2 {
-      D2 e(R(0));
-      for (int b = 0; b < O + 1; b++) {
-        D2 c0(R(0));
-        for (int a = 0; a < O + 1; a++) c0 = c0 + A[a] * static_cast<R>(F(c, imin + a, jmin + b));
-        e = e + c0 * B[b];
+    if constexpr (DEP) {
+      // separable stencils: e = sum_b B_b sum_a A_a f, with the derivatives
+      // taken through A (dx1) and B (dx2) only
+      R T1p[O + 1], T1d[O + 1], T2p[O + 1], T2d[O + 1];
+      shape_d<O, R>(i, static_cast<R>(P.dx1[p]), ip, S1p, T1p);
+      shape_d<O, R>(j, static_cast<R>(P.dx2[p]), jp, S2p, T2p);
+      shape_sd<O, R>(i, static_cast<R>(P.dx1[p]), id, S1d, T1d);
+      shape_sd<O, R>(j, static_cast<R>(P.dx2[p]), jd, S2d, T2d);
+      auto gather = [&](int c, int imin, int jmin, const R* A, const R* dA, const R* B,
+                        const R* dB) {
+        const float* f = F.d + F.cs * c + (imin + static_cast<int64_t>(F.N1) * jmin);
+        R e = R(0), e1 = R(0), e2 = R(0);
+#pragma unroll
+        for (int b = 0; b < O + 1; b++) {
+          R r = R(0), r1 = R(0);
+#pragma unroll
+          for (int a = 0; a < O + 1; a++) {
+            const R v = static_cast<R>(f[a + F.N1 * b]);
+            r += A[a] * v;
+            r1 += dA[a] * v;
+          }
+          e += r * B[b];
+          e1 += r1 * B[b];
+          e2 += r * dB[b];
+        }
+        gv[c] = e;
+        g1[c] = e1;
+        g2[c] = e2;
+      };
+      gather(0, id, jp, S1d, T1d, S2p, T2p);
+      gather(1, ip, jd, S1p, T1p, S2d, T2d);
+      gather(2, ip, jp, S1p, T1p, S2p, T2p);
+      gather(3, ip, jd, S1p, T1p, S2d, T2d);
+      gather(4, id, jp, S1d, T1d, S2p, T2p);
+      gather(5, id, jd, S1d, T1d, S2d, T2d);
+    } else {
+      // forward-mode reference: dual numbers over (dx1, dx2)
+      const D2 x1 = D2::var(static_cast<R>(P.dx1[p]), 0), x2 = D2::var(static_cast<R>(P.dx2[p]), 1);
+      D2 D1p[O + 1], D1d[O + 1], D2p[O + 1], D2d[O + 1];
+      shape_t<false, O, D2, R>(i, x1, ip, D1p);
+      shape_t<false, O, D2, R>(j, x2, jp, D2p);
+      shape_t<true, O, D2, R>(i, x1, id, D1d);
+      shape_t<true, O, D2, R>(j, x2, jd, D2d);
+      auto gather = [&](int c, int imin, int jmin, const D2* A, const D2* B) {
+        D2 e(R(0));
+        for (int b = 0; b < O + 1; b++) {
+          D2 c0(R(0));
+          for (int a = 0; a < O + 1; a++) c0 = c0 + A[a] * static_cast<R>(F(c, imin + a, jmin + b));
+          e = e + c0 * B[b];
+        }
+        gv[c] = e.v;
+        g1[c] = e.d[0];
+        g2[c] = e.d[1];
+      };
+      gather(0, id, jp, D1d, D2p);
+      gather(1, ip, jd, D1p, D2d);
+      gather(2, ip, jp, D1p, D2p);
+      gather(3, ip, jd, D1p, D2d);
+      gather(4, id, jp, D1d, D2p);
+      gather(5, id, jd, D1d, D2d);
+      for (int a = 0; a < O + 1; a++) {
+        S1p[a] = D1p[a].v;
+        S1d[a] = D1d[a].v;
+        S2p[a] = D2p[a].v;
+        S2d[a] = D2d[a].v;
       }
-      return e;
-    };
-    D2 g[6];
-    g[0] = gather(0, id, jp, S1d, S2p);
-    g[1] = gather(1, ip, jd, S1p, S2d);
-    g[2] = gather(2, ip, jp, S1p, S2p);
-    g[3] = gather(3, ip, jd, S1p, S2d);
-    g[4] = gather(4, id, jp, S1d, S2p);
-    g[5] = gather(5, id, jd, S1d, S2d);
+    }
     const R lo5[5] = { post.lx1[q] + a7[2], post.lx2[q] + a7[3], post.lu1[q] + a7[4],
                        post.lu2[q] + a7[5], post.lu3[q] + a7[6] };
     // lin = (d/d dx1, d/d dx2, d/du1..3, d/de1..3, d/db1..3) of <lo5, (dx', u')>.
-    // The offsets enter the update only through the gathered fields (handled
-    // by the Dual<2> gather) and additively (d dx'/d dx = 1).
+    // The offsets enter the update only through the gathered fields (the
+    // gather derivatives above) and additively (d dx'/d dx = 1).
     R lin[11];
     lin[0] = lo5[0];
     lin[1] = lo5[1];
     const R uin[3] = { static_cast<R>(P.ux1[p]), static_cast<R>(P.ux2[p]),
                        static_cast<R>(P.ux3[p]) };
-    const R ein[3] = { g[0].v, g[1].v, g[2].v }, bin[3] = { g[3].v, g[4].v, g[5].v };
+    const R ein[3] = { gv[0], gv[1], gv[2] }, bin[3] = { gv[3], gv[4], gv[5] };
     if constexpr (DEP) {
       boris_vjp_rev<R>(ndth, dt, dxm, uin, ein, bin, lo5, lin + 2);
     } else {
@@ -795,32 +888,48 @@ __global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const 
     R lx1 = lin[0], lx2 = lin[1];
     for (int c = 0; c < 6; ++c) {
       const R l = lin[5 + c];
-      lx1 += l * g[c].d[0];
-      lx2 += l * g[c].d[1];
+      lx1 += l * g1[c];
+      lx2 += l * g2[c];
     }
     // cell offset of this particle's unwrapped frame (0 or -n)
     const int sh1 = (rot[0] && P.i1[p] >= n1 - (n1 >> 1)) ? -n1 : 0;
     const int sh2 = (rot[1] && P.i2[p] >= n2 - (n2 >> 1)) ? -n2 : 0;
-    auto scatter = [&](int c, int imin, int jmin, const D2* A, const D2* B, R l) {
-      if (l == R(0)) return;
-      for (int bb = 0; bb < O + 1; bb++)
-        for (int a = 0; a < O + 1; a++) {
-          const R   v  = l * A[a].v * B[bb].v;
-          const int ia = imin + a - ngh, ja = jmin + bb - ngh;  // active cell (unwrapped)
-          if (fits) {
-            const int x = ia + sh1 - wlo[0], y = ja + sh2 - wlo[1];
-            atomicAdd(&win[(c * ext[1] + y) * kVW + x], static_cast<double>(v));
-          } else {
-            lem.add(c, ia, ja, v);
-          }
+    if (fits) {  // workgroup-uniform
+      auto scatter = [&](int c, int imin, int jmin, const R* A, const R* B, R l) {
+        if (l == R(0)) return;
+        // window row (c, y) holds cells x = 0..ext[0]-1 of component c
+        double* w = win + ((c * ext[1] + (jmin - ngh + sh2 - wlo[1])) * kVW +
+                           (imin - ngh + sh1 - wlo[0]));
+#pragma unroll
+        for (int bb = 0; bb < O + 1; bb++) {
+          const R lb = l * B[bb];
+#pragma unroll
+          for (int a = 0; a < O + 1; a++)
+            atomicAdd(w + (bb * kVW + a), static_cast<double>(lb * A[a]));
         }
-    };
-    scatter(0, id, jp, S1d, S2p, lin[5]);
-    scatter(1, ip, jd, S1p, S2d, lin[6]);
-    scatter(2, ip, jp, S1p, S2p, lin[7]);
-    scatter(3, ip, jd, S1p, S2d, lin[8]);
-    scatter(4, id, jp, S1d, S2p, lin[9]);
-    scatter(5, id, jd, S1d, S2d, lin[10]);
+      };
+      scatter(0, id, jp, S1d, S2p, lin[5]);
+      scatter(1, ip, jd, S1p, S2d, lin[6]);
+      scatter(2, ip, jp, S1p, S2p, lin[7]);
+      scatter(3, ip, jd, S1p, S2d, lin[8]);
+      scatter(4, id, jp, S1d, S2p, lin[9]);
+      scatter(5, id, jd, S1d, S2d, lin[10]);
+    } else {
+      auto scatter = [&](int c, int imin, int jmin, const R* A, const R* B, R l) {
+        if (l == R(0)) return;
+        for (int bb = 0; bb < O + 1; bb++) {
+          const R lb = l * B[bb];
+          for (int a = 0; a < O + 1; a++)
+            lem.add(c, imin + a - ngh, jmin + bb - ngh, lb * A[a]);
+        }
+      };
+      scatter(0, id, jp, S1d, S2p, lin[5]);
+      scatter(1, ip, jd, S1p, S2d, lin[6]);
+      scatter(2, ip, jp, S1p, S2p, lin[7]);
+      scatter(3, ip, jd, S1p, S2d, lin[8]);
+      scatter(4, id, jp, S1d, S2p, lin[9]);
+      scatter(5, id, jd, S1d, S2d, lin[10]);
+    }
     if constexpr (DEP) {
       pre.lx1[q] = a7[0] + lx1;
       pre.lx2[q] = a7[1] + lx2;

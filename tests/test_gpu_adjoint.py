@@ -283,8 +283,9 @@ def test_deposit_vjp_matches_finite_differences(hip, order):
     assert abs(ad - fd) <= 2e-3 * max(abs(fd), 1.0), (ad, fd)
 
 
+@pytest.mark.parametrize("n", [(24, 16), (48, 40)])
 @pytest.mark.parametrize("order", [1, 2])
-def test_fused_particle_vjp_and_ids(hip, order):
+def test_fused_particle_vjp_and_ids(hip, order, n):
     """ntt_sr_particle_vjp equals ntt_deposit_vjp followed by ntt_sr_push_vjp;
     with an ID payload the adjoint arrays are indexed by ID (a permutation of
     the particles gives the permuted adjoint); an out-of-range ID is reported."""
@@ -292,7 +293,6 @@ def test_fused_particle_vjp_and_ids(hip, order):
     from entity_amd.containers import GridShape, Particles, field_abi
     from entity_amd.kernels import (Minkowski, PusherBoundaries, PusherContext,
                                     communicate_fields, device_errors, sr)
-    n = (24, 16)
     shape = GridShape(2, n)
     rng = np.random.default_rng(40 + order)
     npart = 3000
