@@ -8,9 +8,8 @@
 // tiles of tile[0] x tile[1] (x tile[2]) cells, numbered x fastest; dead
 // particles get the key one past the last tile.
 //
-// Steps: key + histogram (wavefront-aggregated atomics) -> exclusive scan of
-// the histogram -> scatter of particle indices (aggregated atomic cursor per
-// key, so the order inside a tile is the arrival order) -> gather of the SoA
+// Steps: key + ranked histogram -> exclusive scan of the histogram -> each
+// particle's destination (prefix of its key + its rank) -> gather of the SoA
 // arrays, four per pass, through a scratch buffer.  All scratch lives in the
 // caller's workspace.
 #include "internal.hpp"
@@ -74,21 +73,72 @@ __device__ __forceinline__ uint32_t aggregated_increment(uint32_t* counters, uin
   return result;
 }
 
+// Ranked histogram: every particle gets its key and its rank among the
+// particles of that key, so that the scatter needs no atomics (destination =
+// exclusive prefix of its key + rank).  A workgroup takes kB * R consecutive
+// particles.  Keys within kHist of a reference key (the workgroup's first key
+// minus kHist / 4: a nearly sorted species moves few particles to lower
+// tiles) are counted in LDS, and the workgroup then reserves each non-empty
+// bin's range with one global atomic; other keys use wavefront-aggregated
+// global atomics.  This keeps the atomics on the few counters of a small
+// species from queuing behind each other.
+constexpr int kHist = 1024;
+
+// base, n: the workgroup's first particle and the particle count (in the
+// numbering of key_of); keys and rank are written at gofs + p.
+template <int R, class KeyOf>
+__device__ __forceinline__ void ranked_keys(int64_t base, int64_t n, int64_t gofs, KeyOf key_of,
+                                            uint32_t* keys, uint32_t* rank, uint32_t* counts,
+                                            uint32_t* hist, uint32_t* kref_s) {
+  for (int b = threadIdx.x; b < kHist; b += kB) hist[b] = 0;
+  uint32_t k[R];
+  bool     act[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t p = base + r * kB + threadIdx.x;
+    act[r]          = p < n;
+    k[r]            = act[r] ? key_of(p) : 0u;
+    if (act[r]) keys[gofs + p] = k[r];
+  }
+  if (threadIdx.x == 0) *kref_s = k[0] >= kHist / 4 ? k[0] - kHist / 4 : 0u;
+  __syncthreads();
+  const uint32_t kref = *kref_s;
+  uint32_t       loc[R];
+  bool           win[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    win[r]   = act[r] && k[r] - kref < static_cast<uint32_t>(kHist);
+    loc[r]   = aggregated_increment(hist, win[r] ? k[r] - kref : 0u, win[r]);
+    const uint32_t g = aggregated_increment(counts, k[r], act[r] && !win[r]);
+    if (act[r] && !win[r]) rank[gofs + base + r * kB + threadIdx.x] = g;
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kHist; b += kB) {
+    const uint32_t c = hist[b];
+    if (c != 0u) hist[b] = atomicAdd(counts + kref + b, c);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (win[r]) rank[gofs + base + r * kB + threadIdx.x] = hist[k[r] - kref] + loc[r];
+}
+
+constexpr int kKeyItems = 4;  // particles per thread of the single-species key pass
+
 __global__ __launch_bounds__(kB) void k_keys(const int32_t* __restrict__ i1,
                                              const int32_t* __restrict__ i2,
                                              const int32_t* __restrict__ i3,
                                              const int16_t* __restrict__ tag, int64_t n,
                                              TileKey K, uint32_t* __restrict__ keys,
+                                             uint32_t* __restrict__ rank,
                                              uint32_t* __restrict__ counts) {
-  const int64_t p      = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
-  const bool    active = p < n;
-  uint32_t      k      = 0;
-  if (active) {
-    if (tag[p] == NTT_TAG_DEAD) k = K.dead_key;
-    else k = K(i1[p], i2[p], K.dim == 3 ? i3[p] : 0);
-    keys[p] = k;
-  }
-  aggregated_increment(counts, k, active);
+  __shared__ uint32_t hist[kHist];
+  __shared__ uint32_t kref;
+  auto key_of = [&](int64_t p) -> uint32_t {
+    return tag[p] == NTT_TAG_DEAD ? K.dead_key : K(i1[p], i2[p], K.dim == 3 ? i3[p] : 0);
+  };
+  ranked_keys<kKeyItems>(static_cast<int64_t>(blockIdx.x) * (kB * kKeyItems), n, 0, key_of, keys,
+                         rank, counts, hist, &kref);
 }
 
 // block-level exclusive scan of kScanTile elements; writes block totals
@@ -134,17 +184,17 @@ __global__ __launch_bounds__(kB) void k_scan_add(uint32_t* __restrict__ data, in
   }
 }
 
+// destination = exclusive prefix of the key + rank; inverse writes it over
+// the key (keys[p] = destination), else perm[destination] = p
 __global__ __launch_bounds__(kB) void k_scatter(uint32_t* __restrict__ keys, int64_t n,
-                                                uint32_t* __restrict__ cursor,
+                                                const uint32_t* __restrict__ prefix,
+                                                const uint32_t* __restrict__ rank,
                                                 uint32_t* __restrict__ perm, int inverse) {
-  const int64_t  p      = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
-  const bool     active = p < n;
-  const uint32_t key    = active ? keys[p] : 0u;
-  const uint32_t pos    = aggregated_increment(cursor, key, active);
-  if (active) {
-    if (inverse) keys[p] = pos;  // the key is not needed any more: keep the destination
-    else perm[pos] = static_cast<uint32_t>(p);
-  }
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t pos = prefix[keys[p]] + rank[p];
+  if (inverse) keys[p] = pos;
+  else perm[pos] = static_cast<uint32_t>(p);
 }
 
 // up to kBatch particle arrays permuted per pass, each staged in a slot of
@@ -259,7 +309,7 @@ int64_t scan_tmp_elems(int64_t n) {
 }
 
 struct Layout {
-  int64_t keys, perm, counts, cursor, scan, tmp, total;  // byte offsets
+  int64_t keys, perm, rank, counts, scan, tmp, total;  // byte offsets
   int64_t nkeys;
 };
 
@@ -285,8 +335,11 @@ ntt_status make_layout(int64_t npart, int32_t dim, const int32_t* n, const int32
   int64_t off = 0;
   L->keys   = off; off = align256(off + 4 * npart);
   L->perm   = off; off = align256(off + 4 * npart);
+  // ranks share the permutation's buffer when the permutation is kept in the
+  // keys (by destination)
+  L->rank   = L->perm;
+  if (!NTT_SORT_SCATTER) { L->rank = off; off = align256(off + 4 * npart); }
   L->counts = off; off = align256(off + 4 * L->nkeys);
-  L->cursor = off; off = align256(off + 4 * L->nkeys);
   L->scan   = off; off = align256(off + 4 * scan_tmp_elems(L->nkeys));
   L->tmp    = off; off = align256(off + (staging ? 8 * kBatch * npart : 0));
   L->total  = off;
@@ -358,16 +411,16 @@ __global__ __launch_bounds__(kB) void k_keys_batched(const SortSeg* __restrict__
                                                      const int64_t* __restrict__ kbase,
                                                      const int64_t* __restrict__ bbase, int nseg,
                                                      uint32_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ rank,
                                                      uint32_t* __restrict__ counts) {
-  const SegBlock b      = seg_block(bbase, nseg);
-  const SortSeg& sg     = segs[b.s];
-  const bool     active = b.p < sg.n;
-  uint32_t       k      = 0;
-  if (active) {
-    k = static_cast<uint32_t>(kbase[b.s]) + seg_key(sg, b.p);
-    keys[pbase[b.s] + b.p] = k;
-  }
-  aggregated_increment(counts, k, active);
+  __shared__ uint32_t hist[kHist];
+  __shared__ uint32_t kref;
+  const SegBlock b    = seg_block(bbase, nseg);
+  const SortSeg& sg   = segs[b.s];
+  const uint32_t koff = static_cast<uint32_t>(kbase[b.s]);
+  auto key_of = [&](int64_t p) -> uint32_t { return koff + seg_key(sg, p); };
+  // one particle per thread: the workgroups follow the segments' block table
+  ranked_keys<1>(b.p - threadIdx.x, sg.n, pbase[b.s], key_of, keys, rank, counts, hist, &kref);
 }
 
 // tile offsets of every segment: the global exclusive prefix minus pbase
@@ -481,9 +534,9 @@ int64_t sort_blocks(int64_t n) { return (n + kB - 1) / kB; }
 int64_t sort_batched_bytes(int64_t ptot, int64_t ktot) {
   int64_t off = 0;
   off = align256(off + 4 * ptot);                    // keys
-  off = align256(off + 4 * ptot);                    // perm
+  off = align256(off + 4 * ptot);                    // perm (and ranks)
+  if (!NTT_SORT_SCATTER) off = align256(off + 4 * ptot);  // ranks
   off = align256(off + 4 * ktot);                    // counts
-  off = align256(off + 4 * ktot);                    // cursor
   off = align256(off + 4 * scan_tmp_elems(ktot));    // scan
   off = align256(off + 8 * kBatch * ptot);           // staging
   return off;
@@ -500,17 +553,16 @@ ntt_status sort_batched(const SortSeg* segs, const int64_t* pbase, const int64_t
   int64_t   off  = 0;
   uint32_t* keys = reinterpret_cast<uint32_t*>(w + off); off = align256(off + 4 * ptot);
   uint32_t* perm = reinterpret_cast<uint32_t*>(w + off); off = align256(off + 4 * ptot);
+  uint32_t* rank = perm;
+  if (!NTT_SORT_SCATTER) { rank = reinterpret_cast<uint32_t*>(w + off); off = align256(off + 4 * ptot); }
   uint32_t* cnt  = reinterpret_cast<uint32_t*>(w + off); off = align256(off + 4 * ktot);
-  uint32_t* cur  = reinterpret_cast<uint32_t*>(w + off); off = align256(off + 4 * ktot);
   uint32_t* scn  = reinterpret_cast<uint32_t*>(w + off); off = align256(off + 4 * scan_tmp_elems(ktot));
   char*     tmp  = w + off;
   k_zero_u32<<<blocks_for(ktot) > 4096 ? 4096 : blocks_for(ktot), kB, 0, st>>>(cnt, ktot);
-  k_keys_batched<<<nb, kB, 0, st>>>(segs, pbase, kbase, bbase, nseg, keys, cnt);
+  k_keys_batched<<<nb, kB, 0, st>>>(segs, pbase, kbase, bbase, nseg, keys, rank, cnt);
   ntt_status s = scan_exclusive(cnt, ktot, scn, st);
   if (s != NTT_OK) return s;
-  if (hipMemcpyAsync(cur, cnt, 4 * ktot, hipMemcpyDeviceToDevice, st) != hipSuccess)
-    return NTT_ERR_HIP;
-  k_scatter<<<blocks_for(ptot), kB, 0, st>>>(keys, ptot, cur, perm, NTT_SORT_SCATTER);
+  k_scatter<<<blocks_for(ptot), kB, 0, st>>>(keys, ptot, cnt, rank, perm, NTT_SORT_SCATTER);
   k_offsets_batched<<<blocks_for(ktot), kB, 0, st>>>(segs, pbase, kbase, nseg, ktot, cnt);
   for (int a0 = 0; a0 < kSortArrays; a0 += kBatch) {
     k_gather_batched<<<nb, kB, 0, st>>>(segs, pbase, bbase, nseg, ptot, a0,
@@ -607,18 +659,17 @@ ntt_status sort_species(ntt_particles_t* src, ntt_particles_t* dst, int32_t dim,
   uint32_t*   keys = reinterpret_cast<uint32_t*>(ws + L.keys);
   uint32_t*   perm = reinterpret_cast<uint32_t*>(ws + L.perm);
   uint32_t*   cnt  = reinterpret_cast<uint32_t*>(ws + L.counts);
-  uint32_t*   cur  = reinterpret_cast<uint32_t*>(ws + L.cursor);
+  uint32_t*   rank = reinterpret_cast<uint32_t*>(ws + L.rank);
   uint32_t*   scn  = reinterpret_cast<uint32_t*>(ws + L.scan);
   void*       tmp  = ws + L.tmp;
   const int64_t np = pv.n;
 
   k_zero_u32<<<blocks_for(L.nkeys) > 4096 ? 4096 : blocks_for(L.nkeys), kB, 0, st>>>(cnt, L.nkeys);
-  k_keys<<<blocks_for(np), kB, 0, st>>>(pv.i1, pv.i2, pv.i3, pv.tag, np, K, keys, cnt);
+  k_keys<<<static_cast<unsigned>((np + kB * kKeyItems - 1) / (kB * kKeyItems)), kB, 0, st>>>(
+    pv.i1, pv.i2, pv.i3, pv.tag, np, K, keys, rank, cnt);
   s = scan_exclusive(cnt, L.nkeys, scn, st);
   if (s != NTT_OK) return s;
-  if (hipMemcpyAsync(cur, cnt, 4 * L.nkeys, hipMemcpyDeviceToDevice, st) != hipSuccess)
-    return NTT_ERR_HIP;
-  k_scatter<<<blocks_for(np), kB, 0, st>>>(keys, np, cur, perm, NTT_SORT_SCATTER);
+  k_scatter<<<blocks_for(np), kB, 0, st>>>(keys, np, cnt, rank, perm, NTT_SORT_SCATTER);
   if (tile_offsets != nullptr &&
       hipMemcpyAsync(tile_offsets, cnt, 4 * L.nkeys, hipMemcpyDeviceToDevice, st) != hipSuccess)
     return NTT_ERR_HIP;
