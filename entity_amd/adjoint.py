@@ -46,9 +46,13 @@ _INT32 = ("i1", "i2", "i1_prev", "i2_prev")
 
 class _Snap:
     """Particle arrays of one species at one instant, packed in one int32
-    buffer (one launch to take), plus tag and IDs when re-sorts can move them."""
+    buffer (one launch to take), plus tag and IDs when re-sorts can move them.
+    ``base``: an earlier snapshot with the sorted arrays (weight, tag, IDs) of
+    the same particle order.  A push in the periodic box changes none of them,
+    so a snapshot taken after pushes (and before the next re-sort) reads them
+    there instead of copying them again."""
 
-    def __init__(self, sp, with_sorted: bool, views: dict):
+    def __init__(self, sp, with_sorted: bool, views: dict, base=None):
         n = sp.npart
         self.n = n
         self.names = _MOVED + (_SORTED if with_sorted else ())
@@ -63,7 +67,8 @@ class _Snap:
             views[key] = parts
         self.buf = torch.cat(parts)
         self.tag = sp.tag[:n].clone() if with_sorted else None
-        self.shared = sp   # weight / tag / IDs when they are not in the snapshot
+        # weight / tag / IDs when they are not in the snapshot
+        self.shared = base if base is not None else sp
         self._abi = None
 
     def array(self, name: str):
@@ -72,14 +77,15 @@ class _Snap:
             k = self.names.index(name)
             v = self.buf[k * n:(k + 1) * n]
             return v if name in _INT32 else v.view(torch.float32)
-        if name == "pld_i":
-            if self.has_pld:
-                k = len(self.names)
-                return self.buf[k * n:(k + 2) * n].view(torch.int64)
-            return self.shared.pld_i[:n] if self.shared.pld_i is not None else None
-        if name == "tag":
-            return self.tag if self.tag is not None else self.shared.tag[:n]
-        return getattr(self.shared, name)[:n]
+        if name == "pld_i" and self.has_pld:
+            k = len(self.names)
+            return self.buf[k * n:(k + 2) * n].view(torch.int64)
+        if name == "tag" and self.tag is not None:
+            return self.tag
+        if isinstance(self.shared, _Snap):
+            return self.shared.array(name)
+        a = getattr(self.shared, name)
+        return a[:n] if a is not None else None
 
     def abi(self, maxnpart: int) -> _abi.NttParticles:
         """The C ABI view (built once; pointers into the packed buffer)."""
@@ -126,7 +132,6 @@ class SRAdjoint:
             raise ValueError("precision is 32 or 64")
         self.eng, self.nsteps, self.K, self.prec = engine, nsteps, max(1, checkpoint), precision
         self.rdt = torch.float64 if precision == 64 else torch.float32
-        self.sorting = bool(p.sort_interval)
         n1, n2 = p.n
         self.nn = (C.c_int32 * 2)(n1, n2)
         # the step's field coefficients, in single precision as the step forms them
@@ -196,12 +201,13 @@ class SRAdjoint:
         e = self.eng
         recs = []
         before = [_Snap(sp, True, self._views) for sp in e.species]
+        sorted_ = before   # the snapshots that hold weight, tag and IDs in this order
         for _ in range(n0, n1):
             if e.maybe_sort():
-                before = [_Snap(sp, True, self._views) for sp in e.species]
+                before = sorted_ = [_Snap(sp, True, self._views) for sp in e.species]
             emh = self._em_half()
             e.advance()
-            after = [_Snap(sp, self.sorting, self._views) for sp in e.species]
+            after = [_Snap(sp, False, self._views, b) for sp, b in zip(e.species, sorted_)]
             recs.append((before, emh, after))
             before = after
         return recs
