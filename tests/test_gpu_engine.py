@@ -304,6 +304,55 @@ def test_sort_orders_by_tile_and_compacts(hip):
     assert (np.diff(t[:sp.npart]) >= 0).all()
 
 
+@pytest.mark.parametrize("layout", ["random", "nearly_sorted", "one_tile"])
+def test_sort_ranked_histogram_paths(hip, layout):
+    """The key pass ranks particles in LDS bins near each workgroup's first key
+    and with global atomics beyond them.  16384 tiles in random order send most
+    keys through the global path, a nearly sorted species through the bins, and
+    one tile for all particles piles every rank into one bin; each must give a
+    tile-ordered permutation of the same particles with consistent offsets."""
+    from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
+    n, t = (256, 256), (2, 2)
+    prm = SRParams(dim=2, n=n, tile=t)
+    eng = SRPICEngine(prm, [SpeciesSpec("e", 1.0, -1.0)], n[0] * n[1] * 2, with_payload=True)
+    eng.inject_uniform(2, temperature=0.1)
+    sp = eng.species[0]
+    npart = sp.npart
+    sp.pld_i[:npart] = torch.arange(npart, device="cuda")
+    if layout == "random":
+        perm = torch.randperm(npart, device="cuda")
+        for v in sp.arrays().values():
+            v.copy_(v[perm].clone())
+    elif layout == "nearly_sorted":
+        # move every 7th particle one cell in x (some change tiles)
+        sl = slice(0, npart, 7)
+        sp.i1[sl] = (sp.i1[sl] + 1) % n[0]
+    else:
+        sp.i1[:npart] = 0
+        sp.i2[:npart] = 1
+    dead = torch.rand(npart, device="cuda") < 0.05
+    sp.tag[:npart][dead] = 0
+    before = sp.to_host()
+    eng.sort(compact=True)
+    after = sp.to_host()
+    off = eng.tile_offsets[0].cpu().numpy().astype(np.int64)
+    torch.cuda.synchronize()
+    alive = int((before["tag"] == 1).sum())
+    assert sp.npart == alive
+    inv = np.empty(npart, dtype=np.int64)
+    inv[before["pld_i"]] = np.arange(npart)
+    src = inv[after["pld_i"]]   # to_host() holds the live particles after compaction
+    np.testing.assert_array_equal(np.sort(src), np.flatnonzero(before["tag"] == 1))
+    for name, arr in before.items():
+        np.testing.assert_array_equal(after[name], arr[src], err_msg=name)
+    key = after["i1"][:alive] // t[0] + (n[0] // t[0]) * (after["i2"][:alive] // t[1])
+    assert (np.diff(key) >= 0).all()
+    ntiles = (n[0] // t[0]) * (n[1] // t[1])
+    counts = np.bincount(key, minlength=ntiles)
+    np.testing.assert_array_equal(off[:ntiles], np.concatenate(([0], np.cumsum(counts)[:-1])))
+    assert off[ntiles] == alive
+
+
 def test_generator_statistics(hip):
     from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
     prm = SRParams(dim=2, n=(64, 64), tile=(8, 8))
