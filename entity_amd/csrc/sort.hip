@@ -622,9 +622,18 @@ ntt_status sort_species(ntt_particles_t* src, ntt_particles_t* dst, int32_t dim,
                         uint64_t workspace_bytes, uint32_t* tile_offsets,
                         int64_t* npart_alive, void* stream) {
   if (src == nullptr || n == nullptr || tile == nullptr) return NTT_ERR_ARG;
-  ParticleView pv;
-  ntt_status s = make_particle_view(src, dim, true, &pv);
-  if (s != NTT_OK) return s;
+  // the key needs the cell indices and the tags; every other array is
+  // permuted when given (NULL arrays are left as they are)
+  if (src->npart < 0 || (src->maxnpart > 0 && src->npart > src->maxnpart)) return NTT_ERR_ARG;
+  ParticleView pv{};
+  pv.n = src->npart;
+  if (pv.n > 0 && (!src->i1 || !src->tag || (dim >= 2 && !src->i2) || (dim >= 3 && !src->i3)))
+    return NTT_ERR_ARG;
+  pv.i1  = src->i1;
+  pv.i2  = src->i2;
+  pv.i3  = src->i3;
+  pv.tag = src->tag;
+  ntt_status s = NTT_OK;
   void* sa[kSortArrays];
   void* da[kSortArrays];
   species_arrays(src, dim, sa);

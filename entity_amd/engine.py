@@ -188,7 +188,7 @@ class SRPICEngine:
         """The tile re-sort that opens a step every sort_interval steps; True if it ran."""
         p = self.params
         if p.sort_interval and self.step > 0 and self.step % p.sort_interval == 0:
-            self.sort(stream=stream)
+            self.sort(stream=stream, keep_prev=False)   # the push that follows rewrites them
             return True
         return False
 
@@ -219,7 +219,10 @@ class SRPICEngine:
         self.time += self.params.dt
 
     # -- spatial sort ----------------------------------------------------------------
-    def sort(self, stream=None, compact: bool = False) -> None:
+    def sort(self, stream=None, compact: bool = False, keep_prev: bool = True) -> None:
+        """Tile re-sort of every species (ntt_sort_particles[_to]).  keep_prev=False
+        leaves the i*_prev / dx*_prev arrays of pushed species unpermuted, for a
+        sort the next push follows (the push rewrites them first)."""
         p = self.params
         n = (C.c_int32 * 3)(*(list(p.n) + [1] * (3 - p.dim)))
         tile = (C.c_int32 * 3)(*(list(p.tile) + [1] * (3 - p.dim)))
@@ -237,7 +240,8 @@ class SRPICEngine:
                                    with_phi=sp.phi is not None, with_payload=sp.pld_i is not None)
                          for sp in self.species]
         for k, sp in enumerate(self.species):
-            abi = sp.abi()
+            skip = not keep_prev and sp.pusher != _abi.PUSHER_NONE
+            abi = _without_prev(sp.abi()) if skip else sp.abi()
             alive = C.c_int64(0)
             if self.tile_offsets[k] is None:
                 self.tile_offsets[k] = torch.zeros(self.ntiles + 1, dtype=torch.int32,
@@ -247,7 +251,7 @@ class SRPICEngine:
                       _stream(stream))
             if self.double_buffer:
                 alt = self._alt[k]
-                dst = alt.abi()
+                dst = _without_prev(alt.abi()) if skip else alt.abi()
                 _abi.check(self.lib.ntt_sort_particles_to(C.byref(abi), C.byref(dst), *common),
                            "ntt_sort_particles_to")
                 alt.npart = sp.npart      # dst now holds the sorted species
@@ -257,6 +261,17 @@ class SRPICEngine:
                            "ntt_sort_particles")
             if compact:
                 sp.npart = int(alive.value)
+
+
+_PREV = ("i1_prev", "i2_prev", "i3_prev", "dx1_prev", "dx2_prev", "dx3_prev")
+
+
+def _without_prev(abi: _abi.NttParticles) -> _abi.NttParticles:
+    """A copy of the ABI view with the prev arrays left out (not permuted)."""
+    out = _abi.NttParticles.from_buffer_copy(abi)
+    for f in _PREV:
+        setattr(out, f, None)
+    return out
 
 
 def _ntiles(lib, p: SRParams) -> int:

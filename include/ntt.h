@@ -491,8 +491,11 @@ NTT_API ntt_status ntt_profile_read(double* ms_total, int64_t* launches, int32_t
  * Replaces Particles::SortSpatially + RemoveDead
  * (src/framework/containers/particles_sort.cpp:103-253) with the tile key of
  * sort::PositionToTileIndex (src/global/utils/sorting.h:65-137): tiles of
- * tile[0] x tile[1] (x tile[2]) cells numbered x fastest.  Every SoA array
- * (prev arrays, phi and pld_i included when non-NULL) is permuted.  The order
+ * tile[0] x tile[1] (x tile[2]) cells numbered x fastest.  Every non-NULL
+ * SoA array is permuted; only the cell indices and tag are required.  A
+ * caller whose next push rewrites the prev arrays (i*_prev, dx*_prev) may pass
+ * them as NULL: they are then left unpermuted, which saves their traffic (no
+ * reference counterpart: SortSpatially permutes every array).  The order
  * inside a tile is unspecified.  `workspace` must hold
  * ntt_sort_workspace_size() bytes.  When npart_alive is non-NULL the call
  * waits for the stream and returns the number of live particles (which lead
