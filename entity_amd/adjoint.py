@@ -57,19 +57,29 @@ class _Snap:
         self.n = n
         self.names = _MOVED + (_SORTED if with_sorted else ())
         self.has_pld = with_sorted and sp.pld_i is not None
+        # the int16 tags ride in the int32 buffer as n/2 words when n is even
+        self.tag_packed = with_sorted and n % 2 == 0
         # int32 views of the species' arrays, made once per (species, kind)
         key = (id(sp), sp.version, with_sorted)
         parts = views.get(key)
         if parts is None:
-            parts = [getattr(sp, k)[:n].view(torch.int32) for k in self.names]
-            if self.has_pld:
-                parts.append(sp.pld_i[:n].view(torch.int32))
+            parts = self._views_of(sp)
             views[key] = parts
         self.buf = torch.cat(parts)
-        self.tag = sp.tag[:n].clone() if with_sorted else None
+        self.tag = sp.tag[:n].clone() if with_sorted and not self.tag_packed else None
         # weight / tag / IDs when they are not in the snapshot
         self.shared = base if base is not None else sp
         self._abi = None
+
+    def _views_of(self, sp) -> list:
+        """int32 views of the species' arrays in buffer order."""
+        n = self.n
+        parts = [getattr(sp, k)[:n].view(torch.int32) for k in self.names]
+        if self.has_pld:
+            parts.append(sp.pld_i[:n].view(torch.int32))
+        if self.tag_packed:
+            parts.append(sp.tag[:n].view(torch.int32))
+        return parts
 
     def array(self, name: str):
         n = self.n
@@ -77,9 +87,12 @@ class _Snap:
             k = self.names.index(name)
             v = self.buf[k * n:(k + 1) * n]
             return v if name in _INT32 else v.view(torch.float32)
+        k = len(self.names) * n
         if name == "pld_i" and self.has_pld:
-            k = len(self.names)
-            return self.buf[k * n:(k + 2) * n].view(torch.int64)
+            return self.buf[k:k + 2 * n].view(torch.int64)
+        if name == "tag" and self.tag_packed:
+            k += 2 * n if self.has_pld else 0
+            return self.buf[k:k + n // 2].view(torch.int16)
         if name == "tag" and self.tag is not None:
             return self.tag
         if isinstance(self.shared, _Snap):
@@ -104,13 +117,28 @@ class _Snap:
         return self._abi
 
     def restore(self, sp) -> None:
-        for k in _MOVED + ("weight", "tag", "pld_i"):
-            src = self.array(k)
-            if src is None:
+        """Writes the snapshot back into the species (one multi-tensor copy for
+        the packed buffer)."""
+        dst, src, k = [], [], 0
+        for d in self._views_of(sp):
+            m = d.numel()
+            s = self.buf[k:k + m]
+            k += m
+            if s.data_ptr() != d.data_ptr():
+                dst.append(d)
+                src.append(s)
+        if dst:
+            torch._foreach_copy_(dst, src)
+        for name in ("weight", "tag", "pld_i"):
+            if name in self.names or (name == "pld_i" and self.has_pld) or \
+                    (name == "tag" and self.tag_packed):
                 continue
-            dst = getattr(sp, k)[: self.n]
-            if src.data_ptr() != dst.data_ptr():
-                dst.copy_(src)
+            a = self.array(name)
+            if a is None:
+                continue
+            d = getattr(sp, name)[: self.n]
+            if a.data_ptr() != d.data_ptr():
+                d.copy_(a)
 
 
 class SRAdjoint:
