@@ -381,6 +381,10 @@ extern "C" ntt_status ntt_ensemble_create(const ntt_box_t* boxes, int32_t nboxes
         s = make_sort_seg(&B.species[sp].prtls, P.D, n3, tile3,
                           const_cast<uint32_t*>(B.species[sp].tile_offsets), &g);
         if (s != NTT_OK) break;
+        // the step's push rewrites the prev arrays of a pushed species before
+        // anything reads them: the re-sort leaves them out
+        if (B.species[sp].pusher_flags != NTT_PUSHER_NONE)
+          for (int a : { 3, 4, 5, 9, 10, 11 }) g.arr[a] = nullptr;  // i*_prev, dx*_prev
         // the step's chunking reads these offsets: they must cover every tile
         if (B.species[sp].tile_offsets != nullptr &&
             B.species[sp].ntiles != static_cast<int64_t>(g.dead_key)) {

@@ -471,7 +471,9 @@ NTT_API ntt_status ntt_ensemble_step(ntt_ensemble_t ens, int64_t step, void* str
  * of every box whose ntt_box_t.tile was set, as one counting sort over all of
  * them: a fixed number of launches whatever the number of boxes.  Writes
  * each species' tile offsets.  Particle order inside a tile may differ from
- * ntt_sort_particles; results of later steps do not depend on it.  Enqueues
+ * ntt_sort_particles; results of later steps do not depend on it.  The prev
+ * arrays (i*_prev, dx*_prev) of species with a pusher are left unpermuted:
+ * the next ntt_ensemble_step rewrites them before reading them.  Enqueues
  * kernels and device-to-device copies only (graph-capturable); the sort
  * workspace is allocated by ntt_ensemble_create.  No reference counterpart. */
 NTT_API ntt_status ntt_ensemble_sort(ntt_ensemble_t ens, void* stream);

@@ -132,7 +132,10 @@ def test_ensemble_sort_orders_every_box(hip):
             inv[before["pld_i"]] = np.arange(len(before["pld_i"]))
             src = inv[after["pld_i"]]
             for name, arr in before.items():
-                np.testing.assert_array_equal(after[name], arr[src], err_msg=name)
+                if name.endswith("_prev"):   # left in place: the next push rewrites them
+                    np.testing.assert_array_equal(after[name], arr, err_msg=name)
+                else:
+                    np.testing.assert_array_equal(after[name], arr[src], err_msg=name)
             alive = int((after["tag"] == 1).sum())
             assert (after["tag"][:alive] == 1).all() and (after["tag"][alive:] == 0).all()
             key = after["i1"][:alive] // t1 + nt1 * (after["i2"][:alive] // t2)
