@@ -377,7 +377,8 @@ def adjoint_leg(args):
     IDs carry the adjoint through the re-sorts), 40 steps.  Loss
     L = sum |B|^2 / 2 at the last step; gradients dL/du_d and dL/db from the
     checkpointed reverse pass (checkpoint every 8 steps).  Reports the forward
-    and the gradient wall time, float vs double adjoint agreement, and central
+    and the float gradient wall time (medians of three runs) and the double
+    gradient's, float vs double adjoint agreement, and central
     finite differences (h = 1e-3) of the forward."""
     import torch
     from entity_amd.adjoint import SRAdjoint
@@ -417,9 +418,13 @@ def adjoint_leg(args):
         return L, time.perf_counter() - t0
 
     loss_of(ud0, b0)            # warm-up
-    L0, t_fwd = loss_of(ud0, b0)
+    # medians of three timed runs each (single runs vary by a few %)
+    fwd = [loss_of(ud0, b0) for _ in range(3)]
+    L0, t_fwd = fwd[0][0], sorted(t for _, t in fwd)[1]
     grads(32)                   # warm-up
-    L32, gu32, gb32, t32 = grads(32)
+    g32 = [grads(32) for _ in range(3)]
+    L32, gu32, gb32 = g32[0][:3]
+    t32 = sorted(g[3] for g in g32)[1]
     L64, gu64, gb64, t64 = grads(64)
     fd_u = (loss_of(ud0 + h, b0)[0] - loss_of(ud0 - h, b0)[0]) / (2 * h)
     fd_b = (loss_of(ud0, b0 + h)[0] - loss_of(ud0, b0 - h)[0]) / (2 * h)

@@ -42,6 +42,7 @@ _MOVED = ("i1", "i2", "i1_prev", "i2_prev", "dx1", "dx2", "dx1_prev", "dx2_prev"
 # float arrays only a re-sort changes (permutes)
 _SORTED = ("weight",)
 _INT32 = ("i1", "i2", "i1_prev", "i2_prev")
+_DX_PREV = ("dx1_prev", "dx2_prev")
 
 
 class _Snap:
@@ -52,15 +53,20 @@ class _Snap:
     so a snapshot taken after pushes (and before the next re-sort) reads them
     there instead of copying them again."""
 
-    def __init__(self, sp, with_sorted: bool, views: dict, base=None):
+    def __init__(self, sp, with_sorted: bool, views: dict, base=None, pre=None):
+        """pre: the snapshot of the same particles before the push that led
+        here; the push copies dx into dx_prev bit for bit, so this snapshot
+        reads its dx*_prev there instead of copying them."""
         n = sp.npart
         self.n = n
-        self.names = _MOVED + (_SORTED if with_sorted else ())
+        self.pre = pre
+        moved = _MOVED if pre is None else tuple(k for k in _MOVED if k not in _DX_PREV)
+        self.names = moved + (_SORTED if with_sorted else ())
         self.has_pld = with_sorted and sp.pld_i is not None
         # the int16 tags ride in the int32 buffer as n/2 words when n is even
         self.tag_packed = with_sorted and n % 2 == 0
         # int32 views of the species' arrays, made once per (species, kind)
-        key = (id(sp), sp.version, with_sorted)
+        key = (id(sp), sp.version, with_sorted, pre is None)
         parts = views.get(key)
         if parts is None:
             parts = self._views_of(sp)
@@ -69,7 +75,7 @@ class _Snap:
         self.tag = sp.tag[:n].clone() if with_sorted and not self.tag_packed else None
         # weight / tag / IDs when they are not in the snapshot
         self.shared = base if base is not None else sp
-        self._abi = None
+        self._abi = {}
 
     def _views_of(self, sp) -> list:
         """int32 views of the species' arrays in buffer order."""
@@ -87,6 +93,8 @@ class _Snap:
             k = self.names.index(name)
             v = self.buf[k * n:(k + 1) * n]
             return v if name in _INT32 else v.view(torch.float32)
+        if name in _DX_PREV and self.pre is not None:
+            return self.pre.array(name[:-len("_prev")])
         k = len(self.names) * n
         if name == "pld_i" and self.has_pld:
             return self.buf[k:k + 2 * n].view(torch.int64)
@@ -100,21 +108,22 @@ class _Snap:
         a = getattr(self.shared, name)
         return a[:n] if a is not None else None
 
-    def abi(self, maxnpart: int) -> _abi.NttParticles:
-        """The C ABI view (built once; pointers into the packed buffer)."""
-        if self._abi is None:
+    def abi(self, maxnpart: int, ids: bool = True) -> _abi.NttParticles:
+        """The C ABI view (built once; pointers into the packed buffer).
+        ids=False leaves pld_i out, so that kernels index adjoints by position."""
+        if ids not in self._abi:
             s = _abi.NttParticles()
             base, n = self.buf.data_ptr(), self.n
             for k, name in enumerate(self.names):
                 setattr(s, name, base + 4 * k * n)
-            for name in ("weight", "tag"):
+            for name in ("weight", "tag") + (_DX_PREV if self.pre is not None else ()):
                 if name not in self.names:
                     setattr(s, name, self.array(name).data_ptr())
-            ids = self.array("pld_i")
-            s.pld_i = ids.data_ptr() if ids is not None else None
+            pld = self.array("pld_i") if ids else None
+            s.pld_i = pld.data_ptr() if pld is not None else None
             s.npart, s.maxnpart = n, maxnpart
-            self._abi = s
-        return self._abi
+            self._abi[ids] = s
+        return self._abi[ids]
 
     def restore(self, sp) -> None:
         """Writes the snapshot back into the species (one multi-tensor copy for
@@ -215,8 +224,8 @@ class SRAdjoint:
 
     @staticmethod
     def _padj(d) -> _abi.NttParticleAdjoint:
-        return _abi.NttParticleAdjoint(*(d[k].data_ptr() for k in ("lx1", "lx2", "lu1", "lu2",
-                                                                   "lu3")), d["lx1"].numel())
+        """d: [5][n] = (lx1, lx2, lu1, lu2, lu3)"""
+        return _abi.NttParticleAdjoint(*(d[k].data_ptr() for k in range(5)), d.shape[1])
 
     def _prepare_reverse(self, post, pre):
         """ABI structs of the two particle-adjoint sets (built once)."""
@@ -235,7 +244,8 @@ class SRAdjoint:
                 before = sorted_ = [_Snap(sp, True, self._views) for sp in e.species]
             emh = self._em_half()
             e.advance()
-            after = [_Snap(sp, False, self._views, b) for sp, b in zip(e.species, sorted_)]
+            after = [_Snap(sp, False, self._views, b, a)
+                     for sp, b, a in zip(e.species, sorted_, before)]
             recs.append((before, emh, after))
             before = after
         return recs
@@ -275,9 +285,19 @@ class SRAdjoint:
         lam[3:6] = self._active_B().to(self.rdt)
         lamJ = torch.zeros((3, n2, n1), dtype=self.rdt, device=dev)
         lamJ2 = torch.zeros_like(lamJ)
-        post = [{k: torch.zeros(sp.npart, dtype=self.rdt, device=dev)
-                 for k in ("lx1", "lx2", "lu1", "lu2", "lu3")} for sp in e.species]
-        pre = [{k: torch.zeros_like(v) for k, v in d.items()} for d in post]
+        # particle adjoints [5][n] = (dx1, dx2, u1, u2, u3), held in the order of
+        # the particles of the step at hand, so the kernels read them coalesced.
+        # post_ids: the IDs of that order.  A re-sort, or a segment recomputed
+        # from its checkpoint (whose re-sorts may order tiles differently),
+        # changes the order: the adjoint then follows the particles by ID.
+        post = [torch.zeros((5, sp.npart), dtype=self.rdt, device=dev) for sp in e.species]
+        pre = [torch.zeros_like(d) for d in post]
+        tmp = [torch.empty_like(d) for d in post]
+        post_ids = [None] * len(e.species)
+
+        def reorder(s, d, ids_from, ids_to):
+            tmp[s].index_copy_(1, ids_from, d)                 # by ID
+            torch.index_select(tmp[s], 1, ids_to, out=d)       # in the new order
         metric = _abi.NttMinkowski(2, e.params.dx)
         ctxs = [_abi.NttSrPusherCtx(sp.mass, sp.charge, e.params.dt, e.params.omegaB0, n1, n2, 1,
                                     sp.pusher) for sp in e.species]
@@ -303,18 +323,28 @@ class SRAdjoint:
                 fe = field_abi(emh, e.shape)
                 # particles: deposit and push VJPs of each species in one pass
                 for s, sp in enumerate(e.species):
+                    ids_q = after[s].array("pld_i")
+                    if post_ids[s] is not None and post_ids[s].data_ptr() != ids_q.data_ptr():
+                        reorder(s, post[s], post_ids[s], ids_q)
+                    post_ids[s] = before[s].array("pld_i")   # the order pre is written in
                     _abi.check(self.lib.ntt_sr_particle_vjp(
-                        C.byref(ctxs[s]), C.byref(before[s].abi(sp.maxnpart)),
-                        C.byref(after[s].abi(sp.maxnpart)), C.byref(fe), C.byref(metric), order,
-                        prec, src.data_ptr(), C.byref(pa_post_all[s]), C.byref(pa_pre_all[s]),
-                        lam_ptr, stream), "ntt_sr_particle_vjp")
+                        C.byref(ctxs[s]), C.byref(before[s].abi(sp.maxnpart, False)),
+                        C.byref(after[s].abi(sp.maxnpart, False)), C.byref(fe),
+                        C.byref(metric), order, prec, src.data_ptr(),
+                        C.byref(pa_post_all[s]), C.byref(pa_pre_all[s]), lam_ptr, stream),
+                        "ntt_sr_particle_vjp")
                 # first Faraday half
                 self._vjp_yee(_abi.VJP_FARADAY, *self.far, lamB, lamE)
                 post, pre = pre, post
                 pa_post_all, pa_pre_all = pa_pre_all, pa_post_all
         self._restore(final)
         torch.cuda.synchronize()
-        lam_u = [(d["lu1"], d["lu2"], d["lu3"]) for d in post]
+        # to the ID order (pld_i = 0..npart-1 at the start)
+        for s, d in enumerate(post):
+            if post_ids[s] is not None:
+                tmp[s].index_copy_(1, post_ids[s], d)
+                post[s] = tmp[s]
+        lam_u = [(d[2], d[3], d[4]) for d in post]
         return loss, lam, lam_u
 
 

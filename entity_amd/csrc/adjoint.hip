@@ -427,10 +427,38 @@ __device__ __forceinline__ void for_deposit_d(int i_init, R di_init, int i_fin, 
   }
 }
 
-template <int O, typename R>
-__device__ __forceinline__ void deposit_vjp_rev(const ParticleView& P, int64_t p,
-                                                const AdjField<R>& lj, R charge, R inv_dt,
-                                                R* acc) {
+// Current-adjoint accessors of the reverse deposit: slot (a, b) of component c
+// for a particle whose stencil starts at (i1_min, i2_min).
+template <typename R, int M>
+struct LamGlobal {  // the periodic field in HBM; columns and rows wrapped once
+  const R* d;
+  int64_t  cs;
+  int      col[M], row[M];
+  __device__ __forceinline__ LamGlobal(const AdjField<R>& lj, int i1_min, int i2_min)
+      : d(lj.d), cs(static_cast<int64_t>(lj.n1) * lj.n2) {
+    // offsets within one component stay below n1 * n2 < 2^31
+#pragma unroll
+    for (int a = 0; a < M; ++a) {
+      col[a] = AdjField<R>::wrap(i1_min + a, lj.n1);
+      row[a] = AdjField<R>::wrap(i2_min + a, lj.n2) * lj.n1;
+    }
+  }
+  __device__ __forceinline__ R operator()(int c, int a, int b) const {
+    return d[cs * c + (col[a] + row[b])];
+  }
+};
+template <typename R, int PITCH>
+struct LamWindow {  // an LDS copy: w at the particle's first slot, cstride per component
+  const R* w;
+  int      cstride;
+  __device__ __forceinline__ R operator()(int c, int a, int b) const {
+    return w[c * cstride + (b * PITCH + a)];
+  }
+};
+
+template <int O, typename R, class LamAt>
+__device__ __forceinline__ void deposit_vjp_rev(const ParticleView& P, int64_t p, LamAt&& lam_at,
+                                                R charge, R inv_dt, R* acc) {
   constexpr int M = O + 2;
   R   iS1[M], fS1[M], iS2[M], fS2[M], diS1[M], dfS1[M], diS2[M], dfS2[M];
   int i1_min, d1, i2_min, d2;
@@ -444,16 +472,9 @@ __device__ __forceinline__ void deposit_vjp_rev(const ParticleView& P, int64_t p
   const R coeff = static_cast<R>(P.w[p]) * charge;
   const R c0    = -(coeff * inv_dt) * R(0.5);
   const R third = static_cast<R>(0.333333f);
-  // the slots' periodic columns and row offsets, wrapped once per particle
-  // (offsets within one component stay below n1 * n2 < 2^31)
-  int col[M], row[M];
-#pragma unroll
-  for (int a = 0; a < M; ++a) {
-    col[a] = AdjField<R>::wrap(i1_min + a, lj.n1);
-    row[a] = AdjField<R>::wrap(i2_min + a, lj.n2) * lj.n1;
-  }
-  const int64_t cs = static_cast<int64_t>(lj.n1) * lj.n2;
-  auto lam = [&](int c, int a, int b) -> R { return lj.d[cs * c + (col[a] + row[b])]; };
+  // lam(c, a, b): the current adjoint at slot (i1_min + a, i2_min + b) of
+  // component c, from the accessor lam_at picks for this particle's slots
+  lam_at(i1_min, i2_min, [&](const auto& lam) {
   // adjoints of the shape values
   R gi1[M], gf1[M], gi2[M], gf2[M];
 #pragma unroll
@@ -527,6 +548,7 @@ __device__ __forceinline__ void deposit_vjp_rev(const ParticleView& P, int64_t p
   acc[4] = kz * (-u3 * u1 / g3);
   acc[5] = kz * (-u3 * u2 / g3);
   acc[6] = kz * (R(1) / gam - u3 * u3 / g3);
+  });
 }
 
 template <int O, typename R>
@@ -687,6 +709,8 @@ __global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const 
                                                           ParticleView Q, AdjField<R> lj,
                                                           R charge, R inv_dt) {
   __shared__ double win[6 * kVW * kVW];
+  // with DEP, an LDS copy of the current adjoint over the chunk's deposit reach
+  __shared__ R      ljw[DEP ? 3 * kVW * kVW : 1];
   __shared__ int    red[8][kVjpWaves];
   const int     tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int64_t beg = static_cast<int64_t>(blockIdx.x) * kVjpChunk;
@@ -727,8 +751,8 @@ __global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const 
     }
   }
   __syncthreads();
-  bool fits = true, rot[2];
-  int  wlo[2], ext[2];
+  bool fits = true, fitsJ = DEP, rot[2];
+  int  wlo[2], ext[2], wloJ[2], extJ[2];
   {
     const int n[2] = { n1, n2 };
 #pragma unroll
@@ -741,7 +765,7 @@ __global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const 
         rl = min(rl, red[4 * d + 2][w]);
         rh = max(rh, red[4 * d + 3][w]);
       }
-      if (l > h) fits = false;  // no valid particle
+      if (l > h) fits = fitsJ = false;  // no valid particle
       rot[d] = (rh - rl) < (h - l);
       // unwrapped cell coordinates: in the rotated frame, cells c >= n - n/2
       // are taken as c - n
@@ -749,12 +773,29 @@ __global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const 
       wlo[d] = ul - 1;
       ext[d] = uh - ul + 4;
       if (ext[d] > kVW) fits = false;
+      // the deposit of a particle that moved at most one cell reaches
+      // [cell - 2, cell + 3]
+      wloJ[d] = ul - 2;
+      extJ[d] = uh - ul + 6;
+      if (extJ[d] > kVW) fitsJ = false;
     }
   }
   if (fits) {
     for (int e = tid; e < 6 * ext[0] * ext[1]; e += kVjpThreads) {
       const int x = e % ext[0], r = e / ext[0];
       win[r * kVW + x] = 0.0;  // r = c * ext[1] + y -> row (c, y) of the window
+    }
+  }
+  if constexpr (DEP) {
+    if (fitsJ) {
+      const int64_t cs = static_cast<int64_t>(lj.n1) * lj.n2;
+      for (int e = tid; e < 3 * extJ[0] * extJ[1]; e += kVjpThreads) {
+        const int x = e % extJ[0], r = e / extJ[0];  // r = c * extJ[1] + y
+        const int c = r / extJ[1], y = r - c * extJ[1];
+        const int gx = AdjField<R>::wrap(wloJ[0] + x, lj.n1);
+        const int gy = AdjField<R>::wrap(wloJ[1] + y, lj.n2);
+        ljw[r * kVW + x] = lj.d[cs * c + (gx + static_cast<int64_t>(lj.n1) * gy)];
+      }
     }
   }
   __syncthreads();
@@ -767,7 +808,22 @@ __global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const 
     if (q < 0) continue;
     R a7[7] = { R(0), R(0), R(0), R(0), R(0), R(0), R(0) };
     if constexpr (DEP) {
-      if (Q.tag[p] == NTT_TAG_ALIVE) deposit_vjp_rev<O, R>(Q, p, lj, charge, inv_dt, a7);
+      // the current adjoint from the LDS copy when all of the particle's slots
+      // fall inside it, else from HBM.  Q's cells are in its own periodic
+      // frame, P's previous cell plus (Q.i_prev - P.i); the window is in P's
+      // unwrapped frame.
+      auto lam_at = [&](int i1m, int i2m, auto&& body) {
+        constexpr int M = O + 2;
+        const int sh1 = (rot[0] && P.i1[p] >= n1 - (n1 >> 1)) ? -n1 : 0;
+        const int sh2 = (rot[1] && P.i2[p] >= n2 - (n2 >> 1)) ? -n2 : 0;
+        const int x0 = i1m - (Q.i1p[p] - P.i1[p]) + sh1 - wloJ[0];
+        const int y0 = i2m - (Q.i2p[p] - P.i2[p]) + sh2 - wloJ[1];
+        if (fitsJ && x0 >= 0 && x0 + M <= extJ[0] && y0 >= 0 && y0 + M <= extJ[1])
+          body(LamWindow<R, kVW>{ ljw + (y0 * kVW + x0), extJ[1] * kVW });
+        else
+          body(LamGlobal<R, M>(lj, i1m, i2m));
+      };
+      if (Q.tag[p] == NTT_TAG_ALIVE) deposit_vjp_rev<O, R>(Q, p, lam_at, charge, inv_dt, a7);
     }
     if (!valid(p)) {
       pre.lu1[q] = pre.lu2[q] = pre.lu3[q] = R(0);

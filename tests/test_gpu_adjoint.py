@@ -414,7 +414,7 @@ def _loss(ud, b, nsteps, **kw):
     return SRAdjoint(eng, nsteps).forward()
 
 
-def _grads(ud, b, nsteps, checkpoint, precision, **kw):
+def _grads(ud, b, nsteps, checkpoint, precision, per_particle=False, **kw):
     from entity_amd.adjoint import SRAdjoint
     eng = _weibel(ud, b, **kw)
     loss, lam, lam_u = SRAdjoint(eng, nsteps, checkpoint, precision).gradient()
@@ -422,6 +422,8 @@ def _grads(ud, b, nsteps, checkpoint, precision, **kw):
     seed = torch.cos(2 * math.pi * j / N_BOX[1])[:, None].expand(N_BOX[1], N_BOX[0])
     g_b = float((lam[5].double().cpu() * seed).sum())
     g_ud = sum(s * float(lu[0].double().sum()) for s, lu in zip(SIGN, lam_u))
+    if per_particle:
+        return loss, g_ud, g_b, [torch.stack(lu).cpu().numpy() for lu in lam_u]
     return loss, g_ud, g_b
 
 
@@ -460,12 +462,15 @@ def test_gradient_through_resorts(hip):
     round-off with the run that never re-sorts."""
     from entity_amd.kernels import device_errors
     ud, b, nsteps = 0.4, 0.05, 10
-    ref = _grads(ud, b, nsteps, 4, 64)
-    srt = _grads(ud, b, nsteps, 4, 64, sort_interval=2)
+    ref = _grads(ud, b, nsteps, 4, 64, per_particle=True)
+    srt = _grads(ud, b, nsteps, 4, 64, per_particle=True, sort_interval=2)
     assert device_errors() == 0
     assert srt[0] == ref[0]
     assert srt[1] == pytest.approx(ref[1], rel=1e-9)
     assert srt[2] == pytest.approx(ref[2], rel=1e-9)
+    # every particle's initial-momentum adjoint, indexed by ID in both runs
+    for a, r in zip(srt[3], ref[3]):
+        np.testing.assert_allclose(a, r, rtol=1e-9, atol=1e-12 * float(np.abs(r).max()))
 
 
 def test_autograd_function(hip):
