@@ -97,7 +97,8 @@ class NttSrStepParams(C.Structure):
                 ("shape_order", C.c_int32), ("n_filters", C.c_int32),
                 ("fieldsolver_enable", C.c_int32), ("deposit_enable", C.c_int32),
                 ("step", C.c_int64), ("fd", NttFdCoeffs), ("prtl_bc", NttPrtlBC),
-                ("particles_per_thread", C.c_int32), ("weight_max", C.c_float)]
+                ("particles_per_thread", C.c_int32), ("weight_max", C.c_float),
+                ("em_half_out", C.c_void_p)]
 
 
 class NttBox(C.Structure):

@@ -181,7 +181,6 @@ class SRAdjoint:
         self.kJ = float(-dt * f(p.q0) / (f(p.B0) * f(p.V0)))  # fieldsolvers.h:157
         self.lib = engine.lib
         self._views = {}
-        self._fd0 = _abi.NttFdCoeffs()   # the plain Yee stencil
 
     # -- helpers -------------------------------------------------------------------
     def _active_B(self) -> torch.Tensor:
@@ -191,18 +190,6 @@ class SRAdjoint:
 
     def loss(self) -> float:
         return 0.5 * float((self._active_B().double() ** 2).sum())
-
-    def _em_half(self) -> torch.Tensor:
-        """E and B after the step's first Faraday half (what the push gathers)."""
-        e = self.eng
-        em = e.fields.em.clone()
-        f = field_abi(em, e.shape)
-        st = _stream()
-        _abi.check(self.lib.ntt_communicate_fields(C.byref(f), 0, 6, st), "ntt_communicate_fields")
-        _abi.check(self.lib.ntt_faraday_mink(C.byref(f), self.far[0], self.far[1],
-                                             C.byref(self._fd0), st), "ntt_faraday_mink")
-        _abi.check(self.lib.ntt_communicate_fields(C.byref(f), 3, 6, st), "ntt_communicate_fields")
-        return em
 
     def _checkpoint(self) -> dict:
         e = self.eng
@@ -242,8 +229,8 @@ class SRAdjoint:
         for _ in range(n0, n1):
             if e.maybe_sort():
                 before = sorted_ = [_Snap(sp, True, self._views) for sp in e.species]
-            emh = self._em_half()
-            e.advance()
+            emh = torch.empty_like(e.fields.em)   # E, B as the push reads them
+            e.advance(em_half_out=emh)
             after = [_Snap(sp, False, self._views, b, a)
                      for sp, b, a in zip(e.species, sorted_, before)]
             recs.append((before, emh, after))

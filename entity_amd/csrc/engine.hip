@@ -189,6 +189,8 @@ extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t
     s = faraday_half();
     if (s != NTT_OK) return s;
   }
+  // the fields the push gathers, for a caller that records them
+  if (prm->em_half_out != nullptr) launch_copy(prm->em_half_out, E.d, E.cstride * 6, st);
   // currents.h:68: J = 0 -- here the fixed-point accumulator (a fill kernel
   // rather than a memset, so that the step can be captured in a graph)
   if (prm->deposit_enable) launch_zero_u64(P.acc, J.cstride * 3, st);

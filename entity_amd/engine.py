@@ -207,10 +207,16 @@ class SRPICEngine:
             self._args_key = key
         return self._args
 
-    def advance(self, stream=None) -> None:
-        """One ntt_sr_step (the step without its opening re-sort)."""
+    def advance(self, stream=None, em_half_out: torch.Tensor | None = None) -> None:
+        """One ntt_sr_step (the step without its opening re-sort).  em_half_out (a
+        tensor shaped like fields.em) receives E and B as the push reads them."""
         (em, cur, buff), sp, prm, metric = self._step_args()
         prm.step = self.step
+        if em_half_out is not None and (em_half_out.shape != self.fields.em.shape or
+                                        em_half_out.dtype != self.fields.em.dtype or
+                                        not em_half_out.is_contiguous()):
+            raise ValueError("em_half_out must be a contiguous tensor shaped like fields.em")
+        prm.em_half_out = em_half_out.data_ptr() if em_half_out is not None else None
         _abi.check(self.lib.ntt_sr_step(C.byref(em), C.byref(cur), C.byref(buff), sp,
                                         len(self.species), C.byref(prm), C.byref(metric),
                                         self._step_ws.data_ptr(), self._step_ws.numel(),

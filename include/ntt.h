@@ -320,6 +320,11 @@ typedef struct ntt_sr_step_params {
   float           weight_max;         /* bound on |weight| of every particle (0: 1);
                                            sets the fixed-point scale of the current;
                                            heavier particles stay exact, only slower  */
+  float*          em_half_out;        /* device, or NULL: receives em (6 components,
+                                           ghosts included) as the push reads it,
+                                           after the first Faraday half.  No reference
+                                           counterpart: for adjoint recording (read by
+                                           ntt_sr_step only)                          */
 } ntt_sr_step_params_t;
 
 /* One full SRPIC step of a periodic box, in the reference's phase order:
