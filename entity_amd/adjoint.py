@@ -285,6 +285,7 @@ class SRAdjoint:
         def reorder(s, d, ids_from, ids_to):
             tmp[s].index_copy_(1, ids_from, d)                 # by ID
             torch.index_select(tmp[s], 1, ids_to, out=d)       # in the new order
+
         metric = _abi.NttMinkowski(2, e.params.dx)
         ctxs = [_abi.NttSrPusherCtx(sp.mass, sp.charge, e.params.dt, e.params.omegaB0, n1, n2, 1,
                                     sp.pusher) for sp in e.species]
@@ -325,12 +326,12 @@ class SRAdjoint:
                 post, pre = pre, post
                 pa_post_all, pa_pre_all = pa_pre_all, pa_post_all
         self._restore(final)
-        torch.cuda.synchronize()
         # to the ID order (pld_i = 0..npart-1 at the start)
         for s, d in enumerate(post):
             if post_ids[s] is not None:
                 tmp[s].index_copy_(1, post_ids[s], d)
                 post[s] = tmp[s]
+        torch.cuda.synchronize()
         lam_u = [(d[2], d[3], d[4]) for d in post]
         return loss, lam, lam_u
 
