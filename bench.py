@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--pusher", choices=["vay", "boris"], default="vay")
     ap.add_argument("--filters", type=int, default=2)
     ap.add_argument("--sort-interval", type=int, default=4)
-    ap.add_argument("--tile", type=str, default="4,4,4")
+    ap.add_argument("--tile", type=str, default="8,4,4")
     ap.add_argument("--double-buffer", type=int, default=1,
                     help="re-sort out of place into a second set of particle arrays")
     ap.add_argument("--cpu-baseline", type=int, default=1)

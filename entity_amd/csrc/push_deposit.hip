@@ -39,10 +39,14 @@
 
 namespace ntt {
 
-// LDS window extents (cells).  3D: a 4^3 tile plus the [-1, +2] reach and
-// one cell of drift on each side needs 10 per dimension.
+// LDS window extents (cells).  3D: an 8 x 4 x 4 tile plus the [-1, +2] reach
+// and one cell of drift on each side needs 14 x 10 x 10 (67.8 KB with the
+// current window, two workgroups per CU).  Against 4^3 tiles in a 10^3 window
+// it halves the workgroups, and with them the per-workgroup latency chain
+// (box, staging, flush) and the partial last pass of tiles just over the
+// mean count: 38.6 against 45.6 ms per launch at 512 x 256 x 256, order 2.
 #ifndef NTT_PD_W3D
-#define NTT_PD_W3D 10, 10, 10
+#define NTT_PD_W3D 14, 10, 10
 #endif
 template <int D> struct Win;
 template <> struct Win<3> {
@@ -54,7 +58,7 @@ template <> struct Win<2> { static constexpr int W1 = 32, W2 = 16, W3 = 1; };
 namespace {
 
 // Workgroup size and occupancy target of the fused kernel.  The LDS windows
-// (~48 KB in 3D) admit three workgroups per CU; with 512 threads and at most
+// (67.8 KB in 3D) admit two workgroups per CU; with 512 threads and at most
 // 128 VGPRs two workgroups give 4 waves per SIMD.
 #ifndef NTT_PD_THREADS
 #define NTT_PD_THREADS 512

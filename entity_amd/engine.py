@@ -55,7 +55,7 @@ class SRParams:
         if self.extent is None:
             self.extent = tuple((0.0, float(x)) for x in self.n)
         if self.tile is None:
-            self.tile = (8, 8) if self.dim == 2 else (4, 4, 4)
+            self.tile = (8, 8) if self.dim == 2 else (8, 4, 4)
 
     # derived scalars (single precision, as the FP32 reference build holds them)
     @property

@@ -25,11 +25,13 @@ BORIS, VAY = 2, 4
 
 
 def make_engine(dim, n, order, alg, nf, strict=False, ppc=4, seed=3, theta=0.1, drift=0.5,
-                field_amp=0.05):
+                field_amp=0.05, tile=None, sort_interval=None):
     from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
-    tile = (4, 4) if dim == 2 else (4, 4, 4)
+    if tile is None:
+        tile = (4, 4) if dim == 2 else (4, 4, 4)
+    kw = {} if sort_interval is None else {"sort_interval": sort_interval}
     prm = SRParams(dim=dim, n=n, shape_order=order, n_filters=nf, ppc0=2.0 * ppc, tile=tile,
-                   larmor0=0.5, skindepth0=2.0)
+                   larmor0=0.5, skindepth0=2.0, **kw)
     sp = [SpeciesSpec("e-", 1.0, -1.0, alg), SpeciesSpec("e+", 1.0, 1.0, alg)]
     ncell = int(np.prod(n))
     eng = SRPICEngine(prm, sp, ncell * ppc, strict=strict)
@@ -115,6 +117,34 @@ def test_one_step_parity(hip, oracle, case, strict):
         np.testing.assert_allclose(em[3:], em_ref[3:], rtol=1e-5, atol=1e-6 * escale)
     np.testing.assert_allclose(cur, cur_ref, rtol=0, atol=4e-6 * jscale)
     np.testing.assert_allclose(em[:3], em_ref[:3], rtol=0, atol=4e-6 * escale)
+
+
+@pytest.mark.parametrize("order,alg", [(1, BORIS), (2, VAY)])
+def test_tile_shape_does_not_change_results(hip, order, alg):
+    """The current is summed in integers, so the chunking does not enter the
+    result: 4x4x4 tiles and the default 8x4x4 tiles (the 14x10x10 LDS window),
+    re-sorted every 2 steps, give bit-identical fields and the same particles."""
+    from entity_amd.kernels import device_errors
+    n = (32, 16, 16)
+    # the generator's positions follow its tile layout: the same plasma for
+    # both, then b re-sorted into 8x4x4 tiles
+    a = make_engine(3, n, order, alg, 1, tile=(4, 4, 4), sort_interval=2, drift=0.6)
+    b = make_engine(3, n, order, alg, 1, tile=(4, 4, 4), sort_interval=2, drift=0.6)
+    b.params.tile = (8, 4, 4)
+    b.sort()
+    for _ in range(6):
+        a.step_forward()
+        b.step_forward()
+    assert device_errors() == 0
+    pa, fa = snapshot(a)
+    pb, fb = snapshot(b)
+    for k in ("em", "cur"):
+        np.testing.assert_array_equal(fa[k], fb[k], err_msg=k)
+    keys = ["i1", "i2", "i3", "dx1", "dx2", "dx3", "ux1", "ux2", "ux3"]
+    for ha, hb in zip(pa, pb):
+        ra = np.stack([ha[k].astype(np.float64) for k in keys], 1)
+        rb = np.stack([hb[k].astype(np.float64) for k in keys], 1)
+        np.testing.assert_array_equal(ra[np.lexsort(ra.T[::-1])], rb[np.lexsort(rb.T[::-1])])
 
 
 @pytest.mark.parametrize("case", [CASES[1], CASES[4]], ids=["2d-o1", "3d-o2-vay"])
