@@ -109,11 +109,12 @@ def init_fields(eng, seed=12345, nmodes=16, kmax=4, dB=0.5, B0=1.0):
     torch.cuda.synchronize()
 
 
-def measured_traffic(cells, ppc):
+def measured_traffic(cells, ppc, tile):
     """HBM bytes per fused launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (None for another workload)."""
+    WRITE_SIZE summary of this workload (None for another workload or tile)."""
     path = ROOT / "profiles" / "k_push_deposit_3d_o2_vay_512x256x256_traffic.md"
-    want = f"workload: 3D periodic {'x'.join(map(str, cells))} cells, e-/e+ {ppc} ppc per species"
+    want = (f"workload: 3D periodic {'x'.join(map(str, cells))} cells, e-/e+ {ppc} ppc per "
+            f"species, {'x'.join(map(str, tile))} tiles"
     try:
         text = path.read_text()
     except OSError:
@@ -657,7 +658,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(cells, args.ppc) if args.order == 2 and
+            "traffic": measured_traffic(cells, args.ppc, tile) if args.order == 2 and
                        args.pusher == "vay" else None,
             "avg_launch_ms": avg_ms,
             "launches": int(nl.value),
