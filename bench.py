@@ -28,6 +28,9 @@ import time
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
+# 2D tiles of the 2D legs: 16 x 8 cells (the engine's 2D default) hold 2048
+# particles at 16 ppc, which halves the fused kernel's workgroups against 8 x 8
+TILE_2D = (16, 8)
 sys.path.insert(0, str(ROOT))
 
 METRIC = "particle-steps/s (push+deposit+fields) 3D SR; batched & adjoint vs plain"
@@ -114,7 +117,7 @@ def measured_traffic(cells, ppc, tile):
     WRITE_SIZE summary of this workload (None for another workload or tile)."""
     path = ROOT / "profiles" / "k_push_deposit_3d_o2_vay_512x256x256_traffic.md"
     want = (f"workload: 3D periodic {'x'.join(map(str, cells))} cells, e-/e+ {ppc} ppc per "
-            f"species, {'x'.join(map(str, tile))} tiles"
+            f"species, {'x'.join(map(str, tile))} tiles")
     try:
         text = path.read_text()
     except OSError:
@@ -197,7 +200,7 @@ def batched_leg(args, sizes=(64, 128, 256, 512), label=None):
         ud = beta / np.sqrt(1 - beta * beta)
         order = int(rng.choice([1, 2]))
         prm = SRParams(dim=2, n=(n, n), shape_order=order, n_filters=1, ppc0=2.0 * ppc,
-                       tile=(8, 8), larmor0=4.0, skindepth0=4.0, sort_interval=4)
+                       tile=TILE_2D, larmor0=4.0, skindepth0=4.0, sort_interval=4)
         eng = SRPICEngine(prm, [SpeciesSpec("e-", 1.0, -1.0, _abi.PUSHER_BORIS),
                                 SpeciesSpec("e+", 1.0, 1.0, _abi.PUSHER_BORIS)], n * n * ppc)
         eng.inject_uniform(ppc, temperature=theta, drifts=[(ud, 0, 0), (-ud, 0, 0)],
@@ -390,7 +393,7 @@ def adjoint_leg(args):
 
     def setup(ud, b):
         prm = SRParams(dim=2, n=n, shape_order=2, n_filters=1, ppc0=4.0 * ppc, larmor0=1.0,
-                       skindepth0=4.0, sort_interval=4, tile=(8, 8))
+                       skindepth0=4.0, sort_interval=4, tile=TILE_2D)
         sp = [SpeciesSpec(f"s{k}", 1.0, q) for k, q in enumerate((-1.0, 1.0, -1.0, 1.0))]
         eng = SRPICEngine(prm, sp, n[0] * n[1] * ppc, with_payload=True)
         eng.inject_uniform(ppc, temperature=0.1, seed=31)
@@ -450,8 +453,8 @@ def autotune_leg(args):
     (2,097,152 particles), theta 0.1, counter-drift u = 0.57735, order 1, Boris,
     1 filter pass: every candidate (tile shape x re-sort cadence, and the
     unsorted layout x chunk size) runs 16 timed steps after 32 steps that age
-    the layout; reports the fastest and its gain over the default (8x8 tiles,
-    re-sort every 4)."""
+    the layout; reports the fastest and its gain over 8x8 tiles re-sorted
+    every 4 steps (the layout before tuning)."""
     import torch
     from entity_amd.autotune import Candidate, candidates_for, engine_runner, tune
     from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
@@ -504,7 +507,7 @@ def c0_leg(args):
     n, ppc, nsteps = (256, 256), 16, 100
     drifts = [(0.57735, 0, 0), (0.57735, 0, 0), (-0.57735, 0, 0), (-0.57735, 0, 0)]
     prm = SRParams(dim=2, n=n, cfl=0.5, shape_order=1, n_filters=1, ppc0=4.0 * ppc, larmor0=8.0,
-                   skindepth0=8.0, sort_interval=4, tile=(8, 8))
+                   skindepth0=8.0, sort_interval=4, tile=TILE_2D)
     sp = [SpeciesSpec(f"s{k}", 1.0, q) for k, q in enumerate((-1.0, 1.0, -1.0, 1.0))]
     eng = SRPICEngine(prm, sp, n[0] * n[1] * ppc)
     eng.inject_uniform(ppc, temperature=0.1, drifts=drifts, seed=101)
