@@ -28,9 +28,6 @@ import time
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
-# 2D tiles of the 2D legs: 16 x 8 cells (the engine's 2D default) hold 2048
-# particles at 16 ppc, which halves the fused kernel's workgroups against 8 x 8
-TILE_2D = (16, 8)
 sys.path.insert(0, str(ROOT))
 
 METRIC = "particle-steps/s (push+deposit+fields) 3D SR; batched & adjoint vs plain"
@@ -199,8 +196,9 @@ def batched_leg(args, sizes=(64, 128, 256, 512), label=None):
         beta = float(rng.uniform(0, 0.9))
         ud = beta / np.sqrt(1 - beta * beta)
         order = int(rng.choice([1, 2]))
+        # the engine's default tile for the box size: 16 x 8 from 256^2 up, 8 x 8 below
         prm = SRParams(dim=2, n=(n, n), shape_order=order, n_filters=1, ppc0=2.0 * ppc,
-                       tile=TILE_2D, larmor0=4.0, skindepth0=4.0, sort_interval=4)
+                       larmor0=4.0, skindepth0=4.0, sort_interval=4)
         eng = SRPICEngine(prm, [SpeciesSpec("e-", 1.0, -1.0, _abi.PUSHER_BORIS),
                                 SpeciesSpec("e+", 1.0, 1.0, _abi.PUSHER_BORIS)], n * n * ppc)
         eng.inject_uniform(ppc, temperature=theta, drifts=[(ud, 0, 0), (-ud, 0, 0)],
@@ -393,7 +391,7 @@ def adjoint_leg(args):
 
     def setup(ud, b):
         prm = SRParams(dim=2, n=n, shape_order=2, n_filters=1, ppc0=4.0 * ppc, larmor0=1.0,
-                       skindepth0=4.0, sort_interval=4, tile=TILE_2D)
+                       skindepth0=4.0, sort_interval=4, tile=(8, 8))   # 256 tiles per species
         sp = [SpeciesSpec(f"s{k}", 1.0, q) for k, q in enumerate((-1.0, 1.0, -1.0, 1.0))]
         eng = SRPICEngine(prm, sp, n[0] * n[1] * ppc, with_payload=True)
         eng.inject_uniform(ppc, temperature=0.1, seed=31)
@@ -507,7 +505,7 @@ def c0_leg(args):
     n, ppc, nsteps = (256, 256), 16, 100
     drifts = [(0.57735, 0, 0), (0.57735, 0, 0), (-0.57735, 0, 0), (-0.57735, 0, 0)]
     prm = SRParams(dim=2, n=n, cfl=0.5, shape_order=1, n_filters=1, ppc0=4.0 * ppc, larmor0=8.0,
-                   skindepth0=8.0, sort_interval=4, tile=TILE_2D)
+                   skindepth0=8.0, sort_interval=4)   # default tile: 16 x 8 at 256^2
     sp = [SpeciesSpec(f"s{k}", 1.0, q) for k, q in enumerate((-1.0, 1.0, -1.0, 1.0))]
     eng = SRPICEngine(prm, sp, n[0] * n[1] * ppc)
     eng.inject_uniform(ppc, temperature=0.1, drifts=drifts, seed=101)

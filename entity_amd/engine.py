@@ -55,7 +55,16 @@ class SRParams:
         if self.extent is None:
             self.extent = tuple((0.0, float(x)) for x in self.n)
         if self.tile is None:
-            self.tile = (8, 8) if self.dim == 2 else (8, 4, 4)
+            # 128-cell tiles (2048 particles at 16 ppc, inside the fused kernel's
+            # LDS windows of 32 x 16 and 14 x 10 x 10) halve the workgroups of
+            # 64-cell tiles and each pays its latency chain once per more
+            # particles; a box with fewer than 512 of them (two workgroups per
+            # CU of the 256) keeps 64-cell tiles, or the launches leave CUs idle
+            big = (16, 8) if self.dim == 2 else (8, 4, 4)
+            small = (8, 8) if self.dim == 2 else (4, 4, 4)
+            ncell = math.prod(self.n[: self.dim])
+            fits = all(x % t == 0 for x, t in zip(self.n, big))
+            self.tile = big if fits and ncell // math.prod(big) >= 512 else small
 
     # derived scalars (single precision, as the FP32 reference build holds them)
     @property
