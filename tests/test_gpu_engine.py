@@ -512,3 +512,28 @@ def test_double_buffered_sort_matches_in_place(hip):
     torch.cuda.synchronize()
     np.testing.assert_array_equal(a.fields.em.cpu().numpy(), b.fields.em.cpu().numpy())
     assert device_errors() == 0
+
+
+@pytest.mark.parametrize("dim,n", [(2, (32, 24)), (3, (16, 12, 8))])
+def test_em_half_out_is_the_field_the_push_reads(hip, dim, n):
+    """advance(em_half_out=x) hands out E and B after the step's first Faraday
+    half, ghosts included: the same as Faraday(1/2) applied to a copy of the
+    fields before the step (fieldsolvers.h:44-57), to float round-off."""
+    import numpy as np
+    from entity_amd.kernels import communicate_fields, mink
+    eng = make_engine(dim, n, 1, BORIS, 1)
+    eng.step_forward()                       # a step with ghosts already filled
+    p = eng.params
+    f = np.float32
+    dT = f(0.5) * f(p.correction) * f(p.dt)
+    ref = eng.fields.em.clone()
+    communicate_fields(ref, eng.shape, 0, 6)
+    mink.Faraday_kernel(ref, eng.shape, float(dT / (f(p.dx) * f(p.dx))), float(dT))()
+    communicate_fields(ref, eng.shape, 3, 6)
+    out = torch.full_like(eng.fields.em, float("nan"))
+    eng.advance(em_half_out=out)
+    torch.cuda.synchronize()
+    scale = float(ref.abs().max())
+    np.testing.assert_allclose(out.cpu().numpy(), ref.cpu().numpy(), rtol=0, atol=1e-6 * scale)
+    with pytest.raises(ValueError):
+        eng.advance(em_half_out=torch.zeros(3, device="cuda"))
