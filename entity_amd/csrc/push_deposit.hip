@@ -53,7 +53,13 @@ template <> struct Win<3> {
   static constexpr int W[3] = { NTT_PD_W3D };
   static constexpr int W1 = W[0], W2 = W[1], W3 = W[2];
 };
-template <> struct Win<2> { static constexpr int W1 = 32, W2 = 16, W3 = 1; };
+#ifndef NTT_PD_W2D
+#define NTT_PD_W2D 32, 16
+#endif
+template <> struct Win<2> {
+  static constexpr int W[2] = { NTT_PD_W2D };
+  static constexpr int W1 = W[0], W2 = W[1], W3 = 1;
+};
 
 namespace {
 
