@@ -571,7 +571,7 @@ def c0_cpu_baseline(eng, n, ppc, drifts):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+        print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1 "
                           "(multi-GPU decomposition is a design note, not a measured path)"}))
         sys.exit(2)
     import numpy as np

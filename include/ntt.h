@@ -55,7 +55,7 @@ typedef enum ntt_status {
   NTT_ERR_ARG         = 1, /* invalid argument (null pointer, bad size, ...)  */
   NTT_ERR_HIP         = 2, /* a HIP runtime call failed                       */
   NTT_ERR_DEVICE      = 3, /* device error word is non-zero                   */
-  NTT_ERR_UNSUPPORTED = 4, /* valid request outside what this build supports  */
+  NTT_ERR_UNSUPPORTED = 4, /* valid request outside what the library supports */
   NTT_ERR_CAPACITY    = 5  /* workspace or capacity too small                 */
 } ntt_status;
 
@@ -135,7 +135,7 @@ typedef struct ntt_prtl_bc {
 } ntt_prtl_bc_t;
 
 typedef struct ntt_flds_bc {
-  uint8_t bc[3][2]; /* NTT_FLDS_BC_*; this build filters the periodic interior */
+  uint8_t bc[3][2]; /* NTT_FLDS_BC_*; the library filters the periodic interior */
 } ntt_flds_bc_t;
 
 typedef struct ntt_minkowski {
