@@ -677,7 +677,7 @@ __device__ __forceinline__ void boris_vjp_rev(R coeff, R dt, R dxm, const R* u, 
 constexpr int kVjpThreads = 256, kVjpWaves = kVjpThreads / 64, kVjpPerThread = NTT_VJP_PER_THREAD;
 constexpr int kVjpChunk = kVjpThreads * kVjpPerThread;
 #ifndef NTT_VJP_WINDOW
-#define NTT_VJP_WINDOW 32
+#define NTT_VJP_WINDOW 28
 #endif
 constexpr int kVW = NTT_VJP_WINDOW;
 
@@ -709,8 +709,10 @@ __global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const 
                                                           ParticleView Q, AdjField<R> lj,
                                                           R charge, R inv_dt) {
   __shared__ double win[6 * kVW * kVW];
-  // with DEP, an LDS copy of the current adjoint over the chunk's deposit reach
+  // with DEP, an LDS copy of the current adjoint over the chunk's deposit reach,
+  // and of E/B over the gather's reach (the cells the scatter writes)
   __shared__ R      ljw[DEP ? 3 * kVW * kVW : 1];
+  __shared__ float  emw[DEP ? 6 * kVW * kVW : 1];
   __shared__ int    red[8][kVjpWaves];
   const int     tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int64_t beg = static_cast<int64_t>(blockIdx.x) * kVjpChunk;
@@ -797,6 +799,16 @@ __global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const 
         ljw[r * kVW + x] = lj.d[cs * c + (gx + static_cast<int64_t>(lj.n1) * gy)];
       }
     }
+    if (fits) {
+      for (int e = tid; e < 6 * ext[0] * ext[1]; e += kVjpThreads) {
+        const int x = e % ext[0], r = e / ext[0];  // r = c * ext[1] + y
+        const int c = r / ext[1], y = r - c * ext[1];
+        // ghost-extended index of the periodic image of the window cell
+        const int gx = AdjField<R>::wrap(wlo[0] + x, n1) + ngh;
+        const int gy = AdjField<R>::wrap(wlo[1] + y, n2) + ngh;
+        emw[r * kVW + x] = F.d[F.cs * c + (gx + static_cast<int64_t>(F.N1) * gy)];
+      }
+    }
   }
   __syncthreads();
 
@@ -834,6 +846,9 @@ __global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const 
       continue;
     }
     const int i = P.i1[p] + ngh, j = P.i2[p] + ngh;
+    // cell offset of this particle's unwrapped frame (0 or -n)
+    const int sh1 = (rot[0] && P.i1[p] >= n1 - (n1 >> 1)) ? -n1 : 0;
+    const int sh2 = (rot[1] && P.i2[p] >= n2 - (n2 >> 1)) ? -n2 : 0;
     // gathered E, B (gv) and their derivatives in dx1 (g1) and dx2 (g2); the
     // shapes (values) are kept for the scatter
     R   gv[6], g1[6], g2[6];
@@ -847,16 +862,17 @@ __global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const 
       shape_d<O, R>(j, static_cast<R>(P.dx2[p]), jp, S2p, T2p);
       shape_sd<O, R>(i, static_cast<R>(P.dx1[p]), id, S1d, T1d);
       shape_sd<O, R>(j, static_cast<R>(P.dx2[p]), jd, S2d, T2d);
-      auto gather = [&](int c, int imin, int jmin, const R* A, const R* dA, const R* B,
-                        const R* dB) {
-        const float* f = F.d + F.cs * c + (imin + static_cast<int64_t>(F.N1) * jmin);
+      // f(c, imin, jmin) -> row access f_row(a, b) at (imin + a, jmin + b)
+      auto gather = [&](const auto& rows, int c, int imin, int jmin, const R* A, const R* dA,
+                        const R* B, const R* dB) {
+        const auto f = rows(c, imin, jmin);
         R e = R(0), e1 = R(0), e2 = R(0);
 #pragma unroll
         for (int b = 0; b < O + 1; b++) {
           R r = R(0), r1 = R(0);
 #pragma unroll
           for (int a = 0; a < O + 1; a++) {
-            const R v = static_cast<R>(f[a + F.N1 * b]);
+            const R v = static_cast<R>(f(a, b));
             r += A[a] * v;
             r1 += dA[a] * v;
           }
@@ -868,12 +884,27 @@ __global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const 
         g1[c] = e1;
         g2[c] = e2;
       };
-      gather(0, id, jp, S1d, T1d, S2p, T2p);
-      gather(1, ip, jd, S1p, T1p, S2d, T2d);
-      gather(2, ip, jp, S1p, T1p, S2p, T2p);
-      gather(3, ip, jd, S1p, T1p, S2d, T2d);
-      gather(4, id, jp, S1d, T1d, S2p, T2p);
-      gather(5, id, jd, S1d, T1d, S2d, T2d);
+      auto gather6 = [&](const auto& rows) {
+        gather(rows, 0, id, jp, S1d, T1d, S2p, T2p);
+        gather(rows, 1, ip, jd, S1p, T1p, S2d, T2d);
+        gather(rows, 2, ip, jp, S1p, T1p, S2p, T2p);
+        gather(rows, 3, ip, jd, S1p, T1p, S2d, T2d);
+        gather(rows, 4, id, jp, S1d, T1d, S2p, T2p);
+        gather(rows, 5, id, jd, S1d, T1d, S2d, T2d);
+      };
+      if (fits) {  // workgroup-uniform: the LDS copy of the window
+        gather6([&](int c, int imin, int jmin) {
+          const float* w = emw + ((c * ext[1] + (jmin - ngh + sh2 - wlo[1])) * kVW +
+                                  (imin - ngh + sh1 - wlo[0]));
+          return [w](int a, int b) { return w[b * kVW + a]; };
+        });
+      } else {
+        gather6([&](int c, int imin, int jmin) {
+          const float* g = F.d + F.cs * c + (imin + static_cast<int64_t>(F.N1) * jmin);
+          const int    N1 = F.N1;
+          return [g, N1](int a, int b) { return g[a + N1 * b]; };
+        });
+      }
     } else {
       // forward-mode reference: dual numbers over (dx1, dx2)
       const D2 x1 = D2::var(static_cast<R>(P.dx1[p]), 0), x2 = D2::var(static_cast<R>(P.dx2[p]), 1);
@@ -947,9 +978,6 @@ __global__ __launch_bounds__(kVjpThreads) void k_push_vjp(ParticleView P, const 
       lx1 += l * g1[c];
       lx2 += l * g2[c];
     }
-    // cell offset of this particle's unwrapped frame (0 or -n)
-    const int sh1 = (rot[0] && P.i1[p] >= n1 - (n1 >> 1)) ? -n1 : 0;
-    const int sh2 = (rot[1] && P.i2[p] >= n2 - (n2 >> 1)) ? -n2 : 0;
     if (fits) {  // workgroup-uniform
       auto scatter = [&](int c, int imin, int jmin, const R* A, const R* B, R l) {
         if (l == R(0)) return;
