@@ -668,9 +668,11 @@ __device__ __forceinline__ void boris_vjp_rev(R coeff, R dt, R dxm, const R* u, 
 // 20x the throughput of the 32-bit float LDS atomic on this chip), which is
 // then added to the field adjoint with one global atomic per non-zero entry.
 // Chunks that do not fit scatter with global atomics.
-// Particles per thread and window side: two particles per thread and a 32 x 32
-// window measured no slower than one per thread with a 24 x 24 window at
-// three or four waves per SIMD (2D order 2, 128 x 128, 262,144 particles).
+// Particles per thread and window side: two particles per thread with 28 x 28
+// windows (the field adjoint in doubles plus copies of the current adjoint and
+// of E/B: 66 KB of LDS in float, two workgroups per CU) measured faster than
+// one particle per thread, and than 24 x 24 windows at three workgroups per CU
+// (2D order 2, 128 x 128, 262,144 particles per species).
 #ifndef NTT_VJP_PER_THREAD
 #define NTT_VJP_PER_THREAD 2
 #endif
