@@ -253,6 +253,7 @@ struct ntt_ensemble {
   int64_t*               sort_kbase = nullptr;
   int64_t*               sort_bbase = nullptr;
   int64_t                sort_ptot = 0, sort_ktot = 0, sort_blocks = 0;
+  uint32_t               sort_live = 0;  // bit a: SortSeg::arr[a] non-null in some segment
   void*                  sort_ws = nullptr;
   std::vector<Seg>       filter_pass;
   std::vector<Group>     groups;
@@ -447,6 +448,9 @@ extern "C" ntt_status ntt_ensemble_create(const ntt_box_t* boxes, int32_t nboxes
     e->sort_ptot   = pbase.back();
     e->sort_ktot   = kbase.back();
     e->sort_blocks = bbase.back();
+    for (const SortSeg& g : sorts)
+      for (int a = 0; a < kSortArrays; ++a)
+        if (g.arr[a] != nullptr) e->sort_live |= 1u << a;
     if (s == NTT_OK) {
       const int64_t bytes = sort_batched_bytes(e->sort_ptot, e->sort_ktot);
       if (hipMalloc(&e->sort_ws, bytes) != hipSuccess) s = NTT_ERR_HIP;
@@ -505,7 +509,8 @@ extern "C" ntt_status ntt_ensemble_sort(ntt_ensemble_t e, void* stream) {
   if (e->sort.n == 0) return NTT_OK;
   ntt_status s = sort_batched(static_cast<const SortSeg*>(e->sort.dev), e->sort_pbase,
                               e->sort_kbase, e->sort_bbase, e->sort_blocks, e->sort.n,
-                              e->sort_ptot, e->sort_ktot, e->sort_ws, as_stream(stream));
+                              e->sort_ptot, e->sort_ktot, e->sort_live, e->sort_ws,
+                              as_stream(stream));
   if (s != NTT_OK) return s;
   return check_launch();
 }

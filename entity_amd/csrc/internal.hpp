@@ -88,10 +88,11 @@ int64_t    sort_batched_bytes(int64_t ptot, int64_t ktot);
 // workgroups a segment of n particles takes
 int64_t    sort_blocks(int64_t n);
 // segs, pbase, kbase, bbase: device arrays (nseg, then nseg + 1 prefix sums
-// of particles, keys and sort_blocks); nblocks = bbase[nseg]
+// of particles, keys and sort_blocks); nblocks = bbase[nseg].  live: bit a set
+// when arr[a] is non-null in some segment (only those arrays are moved).
 ntt_status sort_batched(const SortSeg* segs, const int64_t* pbase, const int64_t* kbase,
                         const int64_t* bbase, int64_t nblocks, int nseg, int64_t ptot,
-                        int64_t ktot, void* ws, hipStream_t st);
+                        int64_t ktot, uint32_t live, void* ws, hipStream_t st);
 
 struct FlatSeg {
   void*       dst;

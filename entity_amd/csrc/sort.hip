@@ -475,13 +475,20 @@ __device__ __forceinline__ void move_elem(void* dst, int64_t to, const void* src
   else static_cast<int64_t*>(dst)[to] = static_cast<const int64_t*>(src)[from];
 }
 
-// arrays a0 .. a0 + kBatch - 1 of every segment: tmp slot k <- src[perm]
+// the arrays (indices into SortSeg::arr) one pass of the batched sort moves:
+// up to kBatch of those that are non-null in some segment
+struct ArrList {
+  int32_t a[kBatch];
+  int32_t count;
+};
+
+// arrays L.a[0 .. L.count) of every segment: tmp slot k <- src[perm]
 // by_dest: perm holds each particle's global destination, and the slot is
 // written there (coalesced reads); otherwise the slot gathers from perm
 __global__ __launch_bounds__(kB) void k_gather_batched(const SortSeg* __restrict__ segs,
                                                        const int64_t* __restrict__ pbase,
                                                        const int64_t* __restrict__ bbase, int nseg,
-                                                       int64_t ptot, int a0,
+                                                       int64_t ptot, ArrList L,
                                                        const uint32_t* __restrict__ perm,
                                                        char* __restrict__ tmp, int by_dest) {
   const SegBlock b  = seg_block(bbase, nseg);
@@ -494,8 +501,8 @@ __global__ __launch_bounds__(kB) void k_gather_batched(const SortSeg* __restrict
   int64_t off = 0;  // slot offset in bytes per particle (the element sizes before)
 #pragma unroll
   for (int k = 0; k < kBatch; ++k) {
-    const int a = a0 + k;
-    if (a >= kSortArrays) break;
+    if (k >= L.count) break;
+    const int a  = L.a[k];
     const int sz = arr_bytes(a);
     off = (off + sz - 1) / sz * sz;  // aligned slot
     if (sg.arr[a] != nullptr) move_elem(tmp + ptot * off, g, sg.arr[a], q, sz);
@@ -506,7 +513,7 @@ __global__ __launch_bounds__(kB) void k_gather_batched(const SortSeg* __restrict
 __global__ __launch_bounds__(kB) void k_copy_batched(const SortSeg* __restrict__ segs,
                                                      const int64_t* __restrict__ pbase,
                                                      const int64_t* __restrict__ bbase, int nseg,
-                                                     int64_t ptot, int a0,
+                                                     int64_t ptot, ArrList L,
                                                      const char* __restrict__ tmp) {
   const SegBlock b  = seg_block(bbase, nseg);
   const SortSeg& sg = segs[b.s];
@@ -516,8 +523,8 @@ __global__ __launch_bounds__(kB) void k_copy_batched(const SortSeg* __restrict__
   int64_t off = 0;
 #pragma unroll
   for (int k = 0; k < kBatch; ++k) {
-    const int a = a0 + k;
-    if (a >= kSortArrays) break;
+    if (k >= L.count) break;
+    const int a  = L.a[k];
     const int sz = arr_bytes(a);
     off = (off + sz - 1) / sz * sz;
     if (sg.arr[a] != nullptr) move_elem(sg.arr[a], p, tmp + ptot * off, g, sz);
@@ -573,7 +580,7 @@ int64_t sort_batched_bytes(int64_t ptot, int64_t ktot) {
 
 ntt_status sort_batched(const SortSeg* segs, const int64_t* pbase, const int64_t* kbase,
                         const int64_t* bbase, int64_t nblocks, int nseg, int64_t ptot,
-                        int64_t ktot, void* ws, hipStream_t st) {
+                        int64_t ktot, uint32_t live, void* ws, hipStream_t st) {
   if (ptot == 0 || nseg == 0 || nblocks == 0) return NTT_OK;
   if (nblocks > 0x7FFFFFFFll) return NTT_ERR_ARG;
   const unsigned nb = static_cast<unsigned>(nblocks);
@@ -593,10 +600,17 @@ ntt_status sort_batched(const SortSeg* segs, const int64_t* pbase, const int64_t
   if (s != NTT_OK) return s;
   k_scatter<<<blocks_for(ptot), kB, 0, st>>>(keys, ptot, cnt, ktot, rank, perm, NTT_SORT_SCATTER);
   k_offsets_batched<<<blocks_for(ktot), kB, 0, st>>>(segs, pbase, kbase, nseg, ktot, cnt);
-  for (int a0 = 0; a0 < kSortArrays; a0 += kBatch) {
-    k_gather_batched<<<nb, kB, 0, st>>>(segs, pbase, bbase, nseg, ptot, a0,
-                                        NTT_SORT_SCATTER ? keys : perm, tmp, NTT_SORT_SCATTER);
-    k_copy_batched<<<nb, kB, 0, st>>>(segs, pbase, bbase, nseg, ptot, a0, tmp);
+  // the live arrays, kBatch per pass (a 2D species without its prev arrays
+  // has 9 of the 19: three passes)
+  ArrList L{};
+  for (int a = 0; a < kSortArrays; ++a) {
+    if ((live >> a) & 1u) L.a[L.count++] = a;
+    if (L.count == kBatch || (a == kSortArrays - 1 && L.count > 0)) {
+      k_gather_batched<<<nb, kB, 0, st>>>(segs, pbase, bbase, nseg, ptot, L,
+                                          NTT_SORT_SCATTER ? keys : perm, tmp, NTT_SORT_SCATTER);
+      k_copy_batched<<<nb, kB, 0, st>>>(segs, pbase, bbase, nseg, ptot, L, tmp);
+      L.count = 0;
+    }
   }
   return check_launch();
 }
