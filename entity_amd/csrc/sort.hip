@@ -425,14 +425,19 @@ __device__ __forceinline__ uint32_t seg_key(const SortSeg& s, int64_t p) {
 }
 
 // Every workgroup works inside one segment (bbase: first block of each
-// segment), so the segment index and its table entry are wave-uniform.
+// segment), so the segment index and its table entry are wave-uniform.  A
+// workgroup takes kB * kSegItems consecutive particles of its segment, thread
+// t the particles p + r * kB (r < kSegItems): the segment lookup, a chain of
+// dependent loads, is paid once per 1024 particles rather than per 256.
+constexpr int kSegItems = 4;
 struct SegBlock {
   int     s;
-  int64_t p;  // particle index inside the segment
+  int64_t p;  // the thread's first particle index inside the segment
 };
 __device__ __forceinline__ SegBlock seg_block(const int64_t* __restrict__ bbase, int nseg) {
   const int s = __builtin_amdgcn_readfirstlane(find_seg(bbase, nseg, blockIdx.x));
-  return SegBlock{ s, (static_cast<int64_t>(blockIdx.x) - bbase[s]) * kB + threadIdx.x };
+  return SegBlock{ s,
+                   (static_cast<int64_t>(blockIdx.x) - bbase[s]) * (kB * kSegItems) + threadIdx.x };
 }
 
 __global__ __launch_bounds__(kB) void k_keys_batched(const SortSeg* __restrict__ segs,
@@ -448,8 +453,9 @@ __global__ __launch_bounds__(kB) void k_keys_batched(const SortSeg* __restrict__
   const SortSeg& sg   = segs[b.s];
   const uint32_t koff = static_cast<uint32_t>(kbase[b.s]);
   auto key_of = [&](int64_t p) -> uint32_t { return koff + seg_key(sg, p); };
-  // one particle per thread: the workgroups follow the segments' block table
-  ranked_keys<1>(b.p - threadIdx.x, sg.n, pbase[b.s], key_of, keys, rank, counts, hist, &kref);
+  // the workgroups follow the segments' block table, kSegItems particles per thread
+  ranked_keys<kSegItems>(b.p - threadIdx.x, sg.n, pbase[b.s], key_of, keys, rank, counts, hist,
+                         &kref);
 }
 
 // tile offsets of every segment: the global exclusive prefix minus pbase
@@ -493,11 +499,18 @@ __global__ __launch_bounds__(kB) void k_gather_batched(const SortSeg* __restrict
                                                        char* __restrict__ tmp, int by_dest) {
   const SegBlock b  = seg_block(bbase, nseg);
   const SortSeg& sg = segs[b.s];
-  if (b.p >= sg.n) return;
-  const int64_t g0 = pbase[b.s] + b.p;
+  const int64_t  pb = pbase[b.s];
   // g: slot index (global), q: source index (in the segment)
-  const int64_t g = by_dest ? static_cast<int64_t>(perm[g0]) : g0;
-  const int64_t q = by_dest ? b.p : static_cast<int64_t>(perm[g0]) - pbase[b.s];
+  int64_t g[kSegItems], q[kSegItems];
+  bool    act[kSegItems];
+#pragma unroll
+  for (int r = 0; r < kSegItems; ++r) {
+    const int64_t p = b.p + r * kB;
+    act[r]          = p < sg.n;
+    const int64_t d = act[r] ? static_cast<int64_t>(perm[pb + p]) : 0;
+    g[r]            = by_dest ? d : pb + p;
+    q[r]            = by_dest ? p : d - pb;
+  }
   int64_t off = 0;  // slot offset in bytes per particle (the element sizes before)
 #pragma unroll
   for (int k = 0; k < kBatch; ++k) {
@@ -505,7 +518,12 @@ __global__ __launch_bounds__(kB) void k_gather_batched(const SortSeg* __restrict
     const int a  = L.a[k];
     const int sz = arr_bytes(a);
     off = (off + sz - 1) / sz * sz;  // aligned slot
-    if (sg.arr[a] != nullptr) move_elem(tmp + ptot * off, g, sg.arr[a], q, sz);
+    const void* src = sg.arr[a];
+    if (src != nullptr) {
+#pragma unroll
+      for (int r = 0; r < kSegItems; ++r)
+        if (act[r]) move_elem(tmp + ptot * off, g[r], src, q[r], sz);
+    }
     off += sz;
   }
 }
@@ -517,9 +535,7 @@ __global__ __launch_bounds__(kB) void k_copy_batched(const SortSeg* __restrict__
                                                      const char* __restrict__ tmp) {
   const SegBlock b  = seg_block(bbase, nseg);
   const SortSeg& sg = segs[b.s];
-  if (b.p >= sg.n) return;
-  const int64_t p = b.p;
-  const int64_t g = pbase[b.s] + p;
+  const int64_t  pb = pbase[b.s];
   int64_t off = 0;
 #pragma unroll
   for (int k = 0; k < kBatch; ++k) {
@@ -527,7 +543,14 @@ __global__ __launch_bounds__(kB) void k_copy_batched(const SortSeg* __restrict__
     const int a  = L.a[k];
     const int sz = arr_bytes(a);
     off = (off + sz - 1) / sz * sz;
-    if (sg.arr[a] != nullptr) move_elem(sg.arr[a], p, tmp + ptot * off, g, sz);
+    void* dst = sg.arr[a];
+    if (dst != nullptr) {
+#pragma unroll
+      for (int r = 0; r < kSegItems; ++r) {
+        const int64_t p = b.p + r * kB;
+        if (p < sg.n) move_elem(dst, p, tmp + ptot * off, pb + p, sz);
+      }
+    }
     off += sz;
   }
 }
@@ -565,7 +588,7 @@ ntt_status make_sort_seg(const ntt_particles_t* p, int32_t dim, const int32_t n[
   return NTT_OK;
 }
 
-int64_t sort_blocks(int64_t n) { return (n + kB - 1) / kB; }
+int64_t sort_blocks(int64_t n) { return (n + kB * kSegItems - 1) / (kB * kSegItems); }
 
 int64_t sort_batched_bytes(int64_t ptot, int64_t ktot) {
   int64_t off = 0;
