@@ -145,6 +145,73 @@ def test_ensemble_sort_orders_every_box(hip):
             np.testing.assert_array_equal(off, np.concatenate([[0], np.cumsum(counts)]))
 
 
+def test_ensemble_sort_with_arrays_held_by_some_boxes(hip):
+    """One batched re-sort over boxes that hold different arrays: a 2D box
+    whose species carry an ID payload (8-byte pld_i), a 2D box without one and
+    a 3D box (i3, dx3).  The sort moves an array in the boxes that hold it and
+    leaves the others alone: every species ends grouped by tile with dead
+    particles last, carrying each particle's arrays together (exactly, through
+    the IDs, where there are IDs; as a multiset of rows elsewhere), with its
+    prev arrays untouched and tile offsets equal to the tile counts."""
+    from entity_amd.ensemble import SREnsemble
+    specs = [BOXES[1], BOXES[0], BOXES_3D[0]]
+    boxes = [make_box(s, sort_interval=4) for s in specs]
+    rng = np.random.default_rng(17)
+    befores = []
+    for b, eng in enumerate(boxes):
+        for sp in eng.species:
+            n = sp.npart
+            if b == 0:
+                sp.pld_i = torch.arange(sp.maxnpart, device="cuda") * 3 + 1
+            perm = torch.from_numpy(rng.permutation(n)).cuda()
+            for v in sp.arrays().values():
+                v.copy_(v[perm].clone())
+            dead = torch.from_numpy(rng.random(n) < 0.05).cuda()
+            sp.tag[:n][dead] = 0
+            befores.append(sp.to_host())
+    ens = SREnsemble(boxes)
+    _abi_check = __import__("entity_amd._abi", fromlist=["check"]).check
+    _abi_check(ens.lib.ntt_ensemble_sort(ens._plan, torch.cuda.current_stream().cuda_stream),
+               "ntt_ensemble_sort")
+    torch.cuda.synchronize()
+
+    def moved_rows(h):
+        names = sorted(k for k in h if not k.endswith("_prev"))
+        m = np.stack([np.asarray(h[k]).astype(np.float64) for k in names], 1)
+        return m[np.lexsort(m.T[::-1])]
+
+    k = 0
+    for b, eng in enumerate(boxes):
+        t = eng.params.tile
+        nt = [(eng.params.n[d] + t[d] - 1) // t[d] for d in range(len(t))]
+        for s, sp in enumerate(eng.species):
+            before, after = befores[k], sp.to_host()
+            k += 1
+            assert ("pld_i" in after) == (b == 0)
+            for name, arr in before.items():
+                if name.endswith("_prev"):
+                    np.testing.assert_array_equal(after[name], arr, err_msg=name)
+            if b == 0:
+                inv = np.full(int(before["pld_i"].max()) + 1, -1, dtype=np.int64)
+                inv[before["pld_i"]] = np.arange(len(before["pld_i"]))
+                src = inv[after["pld_i"]]
+                assert (src >= 0).all()
+                for name, arr in before.items():
+                    if not name.endswith("_prev"):
+                        np.testing.assert_array_equal(after[name], arr[src], err_msg=name)
+            else:
+                np.testing.assert_array_equal(moved_rows(after), moved_rows(before))
+            alive = int((after["tag"] == 1).sum())
+            assert (after["tag"][:alive] == 1).all() and (after["tag"][alive:] == 0).all()
+            key = np.zeros(alive, dtype=np.int64)
+            for d in reversed(range(len(t))):
+                key = key * nt[d] + after["i%d" % (d + 1)][:alive] // t[d]
+            assert (np.diff(key) >= 0).all()
+            off = eng.tile_offsets[s].cpu().numpy()
+            counts = np.bincount(key, minlength=len(off) - 1)
+            np.testing.assert_array_equal(off, np.concatenate([[0], np.cumsum(counts)]))
+
+
 def test_ensemble_graph_replays_with_resorts(hip):
     """Re-sorting every 2 steps inside the replayed graphs: fields bit-identical
     to the boxes stepped alone, particles equal as sets."""
