@@ -234,6 +234,38 @@ def test_ensemble_graph_replays_with_resorts(hip):
     assert device_errors() == 0
 
 
+def test_ensemble_with_an_empty_species(hip):
+    """A box whose second species holds no particles: the batched step skips
+    it, the batched re-sort writes its tile offsets as zeros, and every box
+    stays bit-identical to itself stepped alone, across graph replays with
+    re-sorts every 2 steps."""
+    from entity_amd.ensemble import SREnsemble
+    from entity_amd.kernels import device_errors
+
+    def boxes():
+        out = [make_box(s, sort_interval=2) for s in BOXES[:3]]
+        out[1].species[1].npart = 0
+        out[1].sort()
+        return out
+
+    ens_boxes, ref_boxes = boxes(), boxes()
+    ens = SREnsemble(ens_boxes)
+    ens.step_forward()
+    for b in ref_boxes:
+        b.step_forward()
+    ens.capture()
+    for r in range(5):
+        ens.step_graph()
+        for b in ref_boxes:
+            b.step_forward()
+        _ = (torch.ones(1000, device="cuda") * r).sum().item()
+        for a, b in zip(ens_boxes, ref_boxes):
+            assert_same(a, b)
+    assert ens_boxes[1].species[1].npart == 0
+    assert int(ens_boxes[1].tile_offsets[1].abs().sum().item()) == 0
+    assert device_errors() == 0
+
+
 def test_ensemble_mixes_2d_and_3d_boxes(hip):
     """2D and 3D boxes (different dimension groups of the batched fused
     kernel, and 3D tiles in the batched re-sort) in one ensemble, replayed
