@@ -38,9 +38,30 @@ class TuneResult:
                        for c, t in self.ms_per_step.items()), key=lambda r: r["ms_per_step"])
 
 
-def candidates_for(dim: int) -> list:
+def fused_window(dim: int) -> tuple:
+    """Cells per dimension of the fused kernel's LDS window (ntt_fused_window)."""
+    import ctypes as C
+
+    from . import _abi
+    w = (C.c_int32 * 3)()
+    _abi.check(_abi.load(False).ntt_fused_window(dim, w), "ntt_fused_window")
+    return tuple(int(x) for x in w[:dim])
+
+
+def tile_fits_window(tile, window) -> bool:
+    """A freshly sorted tile of t cells spans t - 1 cells; the chunk keeps the
+    LDS path while that span + 4 (the stencil's [-1, +2] reach and one cell of
+    drift) fits the window, i.e. while t + 3 <= w in every dimension, with
+    at least one more cell of room for the drift of a few steps."""
+    return all(t + 4 <= w for t, w in zip(tile, window))
+
+
+def candidates_for(dim: int, window=None) -> list:
     """The default search space: sorted layouts over a few tile shapes and
-    cadences, and the unsorted layout over a few chunk sizes."""
+    cadences, and the unsorted layout over a few chunk sizes.  Tiles that do
+    not fit the fused kernel's LDS window (`window`, cells per dimension; by
+    default the built library's) are left out: every chunk of theirs would
+    take the HBM path."""
     if dim == 2:
         tiles = [(8, 8), (16, 8), (8, 16), (16, 16)]
         cadences = [2, 4, 8]
@@ -51,6 +72,9 @@ def candidates_for(dim: int) -> list:
         unsorted = (4, 4, 4)
     else:
         raise ValueError("dim must be 2 or 3")
+    if window is None:
+        window = fused_window(dim)
+    tiles = [t for t in tiles if tile_fits_window(t, window)]
     out = [Candidate(t, c) for t in tiles for c in cadences]
     out += [Candidate(unsorted, 0, ppt) for ppt in (1, 2, 4, 8)]
     return out

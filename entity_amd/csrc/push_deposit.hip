@@ -187,7 +187,7 @@ struct FusedArgs {
   float           big_coeff;   // |coeff * fscale| up to this: general conversion
   int64_t         nchunks;
   int32_t         ppt;   // particles per thread of a fixed chunk
-  const uint32_t* toff;  // tile offsets (nchunks + 1 entries) or null
+  const uint32_t* toff;  // tile offsets (nchunks entries: ntiles + 1) or null
   uint32_t*       err;
 };
 
@@ -284,8 +284,10 @@ __device__ __forceinline__ void push_deposit_chunk(const FusedArgs& a, int64_t c
   const bool do_dep = a.charge != ZERO;
   int64_t    pbeg, pend;
   if (a.toff != nullptr) {
+    // chunk ntiles is the tail [toff[ntiles], n): the dead particles a sort
+    // put last, and any particles appended after that sort
     pbeg = a.toff[chunk];
-    pend = a.toff[chunk + 1];
+    pend = (chunk + 1 < a.nchunks) ? static_cast<int64_t>(a.toff[chunk + 1]) : a.P.n;
   } else {
     pbeg = chunk * (static_cast<int64_t>(kThreads) * a.ppt);
     pend = pbeg + static_cast<int64_t>(kThreads) * a.ppt;
@@ -589,7 +591,7 @@ ntt_status build_fused_args(void* out, int64_t* nchunks, const ParticleView& pv,
   if (pv.n == 0) {
     a.nchunks = 0;
   } else if (tile_offsets != nullptr) {
-    a.nchunks = ntiles;
+    a.nchunks = ntiles + 1;  // the tiles, then the tail past the last tile
   } else {
     const int64_t per = static_cast<int64_t>(kThreads) * a.ppt;
     a.nchunks = (pv.n + per - 1) / per;
@@ -665,3 +667,20 @@ ntt_status launch_push_deposit_batched(int D, int order, int alg, const void* ta
 }
 
 }  // namespace ntt
+
+// LDS window extents (cells per dimension) of the fused push + deposit kernel.
+extern "C" ntt_status ntt_fused_window(int32_t dim, int32_t w[3]) {
+  if (w == nullptr) return NTT_ERR_ARG;
+  if (dim == 2) {
+    w[0] = ntt::Win<2>::W1;
+    w[1] = ntt::Win<2>::W2;
+    w[2] = 1;
+  } else if (dim == 3) {
+    w[0] = ntt::Win<3>::W1;
+    w[1] = ntt::Win<3>::W2;
+    w[2] = ntt::Win<3>::W3;
+  } else {
+    return NTT_ERR_ARG;
+  }
+  return NTT_OK;
+}

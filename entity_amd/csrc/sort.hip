@@ -725,6 +725,7 @@ ntt_status sort_species(ntt_particles_t* src, ntt_particles_t* dst, int32_t dim,
       if (s != NTT_OK) return s;
     }
     if (npart_alive) *npart_alive = 0;
+    if (dst != nullptr) dst->npart = 0;  // the sorted (empty) species
     return NTT_OK;
   }
   if (workspace == nullptr || static_cast<int64_t>(workspace_bytes) < L.total)

@@ -355,6 +355,14 @@ NTT_API ntt_status ntt_sr_step(ntt_field_t*                em,
 /* bytes of the workspace ntt_sr_step needs for a current field of this shape */
 NTT_API ntt_status ntt_sr_step_workspace_size(const ntt_field_t* cur, uint64_t* bytes);
 
+/* Cells per dimension (w[2] = 1 in 2D) of the LDS window of ntt_sr_step's
+ * fused push + deposit: a chunk whose live particles span a box of b cells
+ * runs from LDS when b + 3 <= w in every dimension (the [-1, +2] reach of
+ * the order-2 stencil), else from HBM.  A sort tile of t cells therefore
+ * keeps the fast path while t + 3 (+ the drift since the last sort) <= w.
+ * No reference counterpart (a launch-shape query for tuners). */
+NTT_API ntt_status ntt_fused_window(int32_t dim, int32_t w[3]);
+
 /* ---------------------------------------------------------------- adjoint */
 
 /* Per-particle adjoints (d loss / d state) of a 2D species: offsets dx1,

@@ -63,3 +63,22 @@ def test_container_versions_track_assignments():
     w = f.version
     f.em = f.em.clone()
     assert f.version > w
+
+
+def test_empty_out_of_place_sort_sets_dst_count():
+    """ntt_sort_particles_to of an empty species (no tile offsets asked for,
+    so no device work) leaves dst holding the sorted, empty species."""
+    import ctypes as C
+
+    from entity_amd import _abi
+    lib = _abi.load(False)
+    src, dst = _abi.NttParticles(), _abi.NttParticles()
+    src.npart, src.maxnpart = 0, 16
+    dst.npart, dst.maxnpart = 7, 16      # stale count from an earlier use
+    n = (C.c_int32 * 3)(8, 8, 1)
+    tile = (C.c_int32 * 3)(4, 4, 1)
+    alive = C.c_int64(-1)
+    st = lib.ntt_sort_particles_to(C.byref(src), C.byref(dst), 2, n, tile, None, 0, None,
+                                   C.byref(alive), None)
+    assert st == _abi.NTT_OK
+    assert dst.npart == 0 and alive.value == 0

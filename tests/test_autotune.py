@@ -27,6 +27,19 @@ def test_search_space(dim):
         candidates_for(1)
 
 
+def test_candidates_fit_the_fused_window():
+    # the built library's window (32 x 16 in 2D, 14 x 10 x 10 in 3D) keeps
+    # 8x8 and 16x8, drops 8x16 and 16x16 (16 + 4 rows > 16)
+    from entity_amd.autotune import fused_window
+    assert fused_window(2) == (32, 16) and fused_window(3) == (14, 10, 10)
+    tiles2 = {c.tile for c in candidates_for(2) if c.sort_interval}
+    assert tiles2 == {(8, 8), (16, 8)}
+    tiles3 = {c.tile for c in candidates_for(3) if c.sort_interval}
+    assert tiles3 == {(4, 4, 4), (8, 4, 4), (4, 4, 2)}
+    # a larger window admits the tall tiles
+    assert (16, 16) in {c.tile for c in candidates_for(2, window=(32, 32))}
+
+
 def test_cache_round_trip(tmp_path):
     from entity_amd.engine import SRParams
     key = signature(SRParams(dim=2, n=(64, 64)), 16, [2])
