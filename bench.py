@@ -174,6 +174,41 @@ def cpu_baseline(args, cells, order, pusher_flag, nfilt):
                       f"oracle C++ restatement in FP32, one core, {el:.1f} s"}
 
 
+def ensemble_specs(sizes=(64, 128, 256, 512), nboxes=64, seed=777):
+    """The seeded parameter scan of the batched workload: per box a size n
+    (n x n cells) from `sizes`, ppc per species in [8, 32], theta log-uniform
+    in [0.01, 1], counter-drift beta uniform in [0, 0.9] (drift 4-velocity
+    u_d = beta gamma), shape order 1 or 2, and the position/velocity seed."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(nboxes):
+        n = int(rng.choice(list(sizes)))
+        ppc = int(rng.integers(8, 33))
+        theta = float(10 ** rng.uniform(-2, 0))
+        beta = float(rng.uniform(0, 0.9))
+        out.append({"n": n, "ppc": ppc, "theta": theta, "ud": beta / np.sqrt(1 - beta * beta),
+                    "order": int(rng.choice([1, 2])), "seed": 1000 + b})
+    return out
+
+
+def ensemble_box(spec, **engine_kw):
+    """One box of the scan as an SRPICEngine at its initial state: e-/e+ at
+    +-u_d along x, Boris, 1 filter pass, re-sort every 4 steps, the engine's
+    default tile for the box size (16 x 8 from 256^2 up, 8 x 8 below)."""
+    from entity_amd import _abi
+    from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
+    n, ppc, ud = spec["n"], spec["ppc"], spec["ud"]
+    prm = SRParams(dim=2, n=(n, n), shape_order=spec["order"], n_filters=1, ppc0=2.0 * ppc,
+                   larmor0=4.0, skindepth0=4.0, sort_interval=4)
+    eng = SRPICEngine(prm, [SpeciesSpec("e-", 1.0, -1.0, _abi.PUSHER_BORIS),
+                            SpeciesSpec("e+", 1.0, 1.0, _abi.PUSHER_BORIS)], n * n * ppc,
+                      **engine_kw)
+    eng.inject_uniform(ppc, temperature=spec["theta"], drifts=[(ud, 0, 0), (-ud, 0, 0)],
+                       seed=spec["seed"])
+    return eng
+
+
 def batched_leg(args, sizes=(64, 128, 256, 512), label=None):
     """Ensemble of 64 independent 2D periodic boxes (parameter scan): sizes
     from {64^2, 128^2, 256^2, 512^2}, 8-32 ppc per species, theta log-uniform
@@ -181,30 +216,11 @@ def batched_leg(args, sizes=(64, 128, 256, 512), label=None):
     1 filter pass, re-sort every 4 steps.  Times the plain per-box loop
     (SRPICEngine.step_forward box after box) and the batched step replayed
     from a HIP graph, on the same boxes."""
-    import numpy as np
     import torch
-    from entity_amd import _abi
-    from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
     from entity_amd.ensemble import SREnsemble
-    rng = np.random.default_rng(777)
-    engines = []
-    total = 0
-    for b in range(64):
-        n = int(rng.choice(list(sizes)))
-        ppc = int(rng.integers(8, 33))
-        theta = float(10 ** rng.uniform(-2, 0))
-        beta = float(rng.uniform(0, 0.9))
-        ud = beta / np.sqrt(1 - beta * beta)
-        order = int(rng.choice([1, 2]))
-        # the engine's default tile for the box size: 16 x 8 from 256^2 up, 8 x 8 below
-        prm = SRParams(dim=2, n=(n, n), shape_order=order, n_filters=1, ppc0=2.0 * ppc,
-                       larmor0=4.0, skindepth0=4.0, sort_interval=4)
-        eng = SRPICEngine(prm, [SpeciesSpec("e-", 1.0, -1.0, _abi.PUSHER_BORIS),
-                                SpeciesSpec("e+", 1.0, 1.0, _abi.PUSHER_BORIS)], n * n * ppc)
-        eng.inject_uniform(ppc, temperature=theta, drifts=[(ud, 0, 0), (-ud, 0, 0)],
-                           seed=1000 + b)
-        engines.append(eng)
-        total += 2 * n * n * ppc
+    specs = ensemble_specs(sizes)
+    engines = [ensemble_box(s) for s in specs]
+    total = sum(2 * s["n"] ** 2 * s["ppc"] for s in specs)
     for eng in engines:
         eng.step_forward()
     torch.cuda.synchronize()
@@ -371,6 +387,32 @@ def gr_leg(args):
             "hbm_gbs": total * 78 / (ms * 1e-3) / 1e9}
 
 
+# the adjoint workload: 128 x 128 cells, 4 species x 16 ppc, 40 steps,
+# checkpoint every 8, drift +-u_d on the species with ADJ_SIGN, uniform B_z = b
+ADJ_N, ADJ_PPC, ADJ_STEPS, ADJ_CHECKPOINT = (128, 128), 16, 40, 8
+ADJ_SIGN = (1.0, 1.0, -1.0, -1.0)
+ADJ_UD, ADJ_B = 0.57735, 0.1
+
+
+def adjoint_box(ud, b):
+    """Initial state of the adjoint workload with drift u_d and B_z amplitude b:
+    e-/e+ beams (species charges -1, +1, -1, +1; u_x shifted by ADJ_SIGN * u_d)
+    over theta = 0.1, shape order 2, 1 filter pass, re-sort every 4 steps
+    into 8 x 8 tiles, particle IDs carried for the adjoint."""
+    from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
+    n, ppc = ADJ_N, ADJ_PPC
+    prm = SRParams(dim=2, n=n, shape_order=2, n_filters=1, ppc0=4.0 * ppc, larmor0=1.0,
+                   skindepth0=4.0, sort_interval=4, tile=(8, 8))   # 256 tiles per species
+    sp = [SpeciesSpec(f"s{k}", 1.0, q) for k, q in enumerate((-1.0, 1.0, -1.0, 1.0))]
+    eng = SRPICEngine(prm, sp, n[0] * n[1] * ppc, with_payload=True)
+    eng.inject_uniform(ppc, temperature=0.1, seed=31)
+    for sg, s in zip(ADJ_SIGN, eng.species):
+        s.ux1[: s.npart] += sg * ud
+    G = eng.shape.ngh
+    eng.fields.em[5, G:G + n[1], G:G + n[0]] = b
+    return eng
+
+
 def adjoint_leg(args):
     """Adjoint of a 2D counter-streaming run: periodic 128 x 128 cells, e-/e+
     beams at u_x +- u_d (u_d = 0.57735, beta = 0.5) over theta = 0.1 thermal
@@ -384,22 +426,10 @@ def adjoint_leg(args):
     finite differences (h = 1e-3) of the forward."""
     import torch
     from entity_amd.adjoint import SRAdjoint
-    from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
-    n, ppc, nsteps, K = (128, 128), 16, 40, 8
-    sign = (1.0, 1.0, -1.0, -1.0)
-    ud0, b0, h = 0.57735, 0.1, 1e-3
-
-    def setup(ud, b):
-        prm = SRParams(dim=2, n=n, shape_order=2, n_filters=1, ppc0=4.0 * ppc, larmor0=1.0,
-                       skindepth0=4.0, sort_interval=4, tile=(8, 8))   # 256 tiles per species
-        sp = [SpeciesSpec(f"s{k}", 1.0, q) for k, q in enumerate((-1.0, 1.0, -1.0, 1.0))]
-        eng = SRPICEngine(prm, sp, n[0] * n[1] * ppc, with_payload=True)
-        eng.inject_uniform(ppc, temperature=0.1, seed=31)
-        for sg, s in zip(sign, eng.species):
-            s.ux1[: s.npart] += sg * ud
-        G = eng.shape.ngh
-        eng.fields.em[5, G:G + n[1], G:G + n[0]] = b
-        return eng
+    n, ppc, nsteps, K = ADJ_N, ADJ_PPC, ADJ_STEPS, ADJ_CHECKPOINT
+    sign = ADJ_SIGN
+    ud0, b0, h = ADJ_UD, ADJ_B, 1e-3
+    setup = adjoint_box
 
     def grads(prec):
         eng = setup(ud0, b0)

@@ -1,0 +1,150 @@
+"""The batched and adjoint workloads checked at their full sizes.
+
+* Batched: the seeded scan of 64 independent 2D boxes of 64^2..512^2 cells
+  (2.9e8 particles; 8-32 ppc, theta 0.01-1, drift beta 0-0.9, order 1/2).
+  One eager ensemble step, then 9 graph replays (re-sorts at steps 4 and 8).
+  Every box ends bit-identical to the same box stepped alone with
+  SRPICEngine.step_forward: fields exactly, particles as a multiset (compared
+  on the GPU, sorted by a hash of all their arrays).  The first ensemble step
+  of one 64^2 box is checked against the oracle's step, and a final batched
+  re-sort leaves every species of every box in tile order with offsets equal
+  to a recount of its tile keys.
+* Adjoint: 128 x 128 cells, 4 species x 16 ppc, order 2, 40 steps, re-sort
+  every 4, checkpoint every 8: dL/du_d and dL/db of the FP64 adjoint within
+  1e-3 of central differences (h = 1e-3) of the forward, the FP32 adjoint
+  within 1e-3 of the FP64 one.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+PRT_KEYS = ("i1", "i2", "i1_prev", "i2_prev", "dx1", "dx2", "dx1_prev", "dx2_prev",
+            "ux1", "ux2", "ux3", "weight", "tag")
+
+
+def _bits(t):
+    return t.view(torch.int32).to(torch.int64) if t.dtype == torch.float32 else t.to(torch.int64)
+
+
+def _sorted_rows(sp):
+    """Every array of the species permuted into the order of a 64-bit hash of
+    the particle's bits (equal multisets give equal results)."""
+    arrs = sp.arrays()
+    h = torch.zeros(sp.npart, dtype=torch.int64, device="cuda")
+    for k in PRT_KEYS:
+        h = (h ^ _bits(arrs[k])) * 1099511628211 + 7
+        h = h ^ (h >> 29)
+    order = torch.argsort(h, stable=True)
+    return [arrs[k][order] for k in PRT_KEYS], h[order]
+
+
+def _assert_box_equal(a, b, label):
+    for k in ("em", "cur"):
+        assert torch.equal(getattr(a.fields, k), getattr(b.fields, k)), f"{label}: {k}"
+    for sa, sb in zip(a.species, b.species):
+        assert sa.npart == sb.npart, label
+        ra, ha = _sorted_rows(sa)
+        rb, hb = _sorted_rows(sb)
+        assert torch.equal(ha, hb), f"{label}: particle multisets differ"
+        for k, xa, xb in zip(PRT_KEYS, ra, rb):
+            assert torch.equal(xa, xb), f"{label}: {k}"
+
+
+def test_batched_scan_of_64_boxes(hip, oracle):
+    from bench import ensemble_box, ensemble_specs
+    from entity_amd.ensemble import SREnsemble
+    from entity_amd.kernels import device_errors
+    from test_gpu_engine import near_face, oracle_step, snapshot
+    specs = ensemble_specs()
+    boxes = [ensemble_box(s) for s in specs]
+    assert sum(s.npart for b in boxes for s in b.species) > 2.5e8
+    small = next(k for k, s in enumerate(specs) if s["n"] == 64)
+    parts0, flds0 = snapshot(boxes[small])
+    ens = SREnsemble(boxes)
+    ens.step_forward()
+    torch.cuda.synchronize()
+    assert device_errors() == 0
+    # one box's first step against the oracle (product build: near-face rule)
+    parts1, flds1 = snapshot(boxes[small])
+    sp_ref, em_ref, cur_ref = oracle_step(oracle, boxes[small], parts0, flds0)
+    for got, ref in zip(parts1, sp_ref):
+        q = ref["p"]
+        flagged = np.zeros(len(q["tag"]), bool)
+        for d in range(2):
+            flagged |= near_face(q[f"dx{d+1}"]) | near_face(got[f"dx{d+1}"])
+        mism = np.zeros_like(flagged)
+        for k in ("i1", "i2", "i1_prev", "i2_prev", "tag"):
+            mism |= got[k] != q[k]
+        assert not (mism & ~flagged).any() and mism.sum() <= 8
+        ok = ~mism
+        for k in ("dx1", "dx2", "ux1", "ux2", "ux3"):
+            np.testing.assert_allclose(got[k][ok], q[k][ok], rtol=2e-6, atol=2e-6, err_msg=k)
+    np.testing.assert_allclose(flds1["cur"], cur_ref, rtol=0,
+                               atol=4e-6 * np.abs(cur_ref).max())
+    np.testing.assert_allclose(flds1["em"], em_ref, rtol=0, atol=4e-6 * np.abs(em_ref).max())
+    # graph replays through two re-sorts
+    ens.capture()
+    nsteps = 10
+    for r in range(nsteps - 1):
+        ens.step_graph()
+        if r % 3 == 0:   # eager work and a synchronisation between replays
+            _ = (torch.ones(1000, device="cuda") * r).sum().item()
+    torch.cuda.synchronize()
+    assert device_errors() == 0 and ens.step == nsteps
+    # every box against the same box stepped alone
+    for k, spec in enumerate(specs):
+        alone = ensemble_box(spec)
+        for _ in range(nsteps):
+            alone.step_forward()
+        torch.cuda.synchronize()
+        _assert_box_equal(boxes[k], alone, f"box {k} ({spec['n']}^2, order {spec['order']})")
+        del alone
+    assert device_errors() == 0
+    # a last batched re-sort: tile order and offsets against a recount
+    from entity_amd import _abi
+    _abi.check(ens.lib.ntt_ensemble_sort(ens._plan, torch.cuda.current_stream().cuda_stream),
+               "ntt_ensemble_sort")
+    torch.cuda.synchronize()
+    for eng in boxes:
+        t1, t2 = eng.params.tile
+        nt1 = (eng.params.n[0] + t1 - 1) // t1
+        for k, sp in enumerate(eng.species):
+            a = sp.arrays()
+            key = (a["i1"] // t1 + nt1 * (a["i2"] // t2)).to(torch.int64)
+            assert bool((key[1:] >= key[:-1]).all())
+            counts = torch.bincount(key, minlength=eng.ntiles)
+            off = torch.zeros(eng.ntiles + 1, dtype=torch.int64, device="cuda")
+            off[1:] = torch.cumsum(counts, 0)
+            assert torch.equal(eng.tile_offsets[k].to(torch.int64), off)
+
+
+def test_adjoint_workload_gradients(hip):
+    from bench import (ADJ_B, ADJ_CHECKPOINT, ADJ_SIGN, ADJ_STEPS, ADJ_UD, adjoint_box)
+    from entity_amd.adjoint import SRAdjoint
+    from entity_amd.kernels import device_errors
+    h = 1e-3
+
+    def loss(ud, b):
+        return SRAdjoint(adjoint_box(ud, b), ADJ_STEPS, ADJ_CHECKPOINT).forward()
+
+    def grads(prec):
+        L, lam, lam_u = SRAdjoint(adjoint_box(ADJ_UD, ADJ_B), ADJ_STEPS, ADJ_CHECKPOINT,
+                                  prec).gradient()
+        g_ud = sum(s * float(lu[0].double().sum()) for s, lu in zip(ADJ_SIGN, lam_u))
+        return L, g_ud, float(lam[5].double().sum())
+
+    L64, gu64, gb64 = grads(64)
+    L32, gu32, gb32 = grads(32)
+    assert device_errors() == 0
+    L = loss(ADJ_UD, ADJ_B)
+    assert L64 == pytest.approx(L, rel=1e-12) and L32 == pytest.approx(L, rel=1e-12)
+    fd_u = (loss(ADJ_UD + h, ADJ_B) - loss(ADJ_UD - h, ADJ_B)) / (2 * h)
+    fd_b = (loss(ADJ_UD, ADJ_B + h) - loss(ADJ_UD, ADJ_B - h)) / (2 * h)
+    print(f"L={L64:.8e} dL/dud adj64={gu64:.8e} adj32={gu32:.8e} fd={fd_u:.8e}; "
+          f"dL/db adj64={gb64:.8e} adj32={gb32:.8e} fd={fd_b:.8e}", flush=True)
+    assert gu64 == pytest.approx(fd_u, rel=1e-3)
+    assert gb64 == pytest.approx(fd_b, rel=1e-3)
+    assert gu32 == pytest.approx(gu64, rel=1e-3)
+    assert gb32 == pytest.approx(gb64, rel=1e-3)
