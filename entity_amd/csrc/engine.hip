@@ -160,10 +160,14 @@ ntt_status plan_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t* buff,
 
 }  // namespace
 
-extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t* buff,
-                                  ntt_species_t* species, int32_t nspecies,
-                                  const ntt_sr_step_params_t* prm, const ntt_minkowski_t* metric,
-                                  void* workspace, uint64_t workspace_bytes, void* stream) {
+namespace {
+
+// One SRPIC step (ntt_sr_step); `timed` lets the fused launches be bracketed
+// by the profiling events when profiling is on.
+ntt_status sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t* buff, ntt_species_t* species,
+                   int32_t nspecies, const ntt_sr_step_params_t* prm,
+                   const ntt_minkowski_t* metric, void* workspace, uint64_t workspace_bytes,
+                   void* stream, bool timed_ok) {
   StepPlan P;
   ntt_status s = plan_step(em, cur, buff, species, nspecies, prm, metric, workspace,
                            workspace_bytes, &P);
@@ -199,7 +203,7 @@ extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t
     if (!p.run) continue;
     const int ppt = prm->particles_per_thread > 0 ? prm->particles_per_thread : 4;
     std::pair<hipEvent_t, hipEvent_t> ev;
-    const bool timed = prof_begin(st, &ev);
+    const bool timed = timed_ok && prof_begin(st, &ev);
     s = launch_push_deposit_periodic(p.pv, E, J, P.acc, P.D, prm->shape_order, p.alg, p.ndth,
                                      P.dt, metric->dx, species[sp].charge, p.dep, P.fscale,
                                      species[sp].tile_offsets, species[sp].ntiles, ppt, st);
@@ -229,6 +233,59 @@ extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t
                                     prm->deposit_enable != 0, st);
   }
   return check_launch();
+}
+
+// The fixed-point current accumulator of ntt_sr_step_static: one array per
+// device (each GPU holds its own copy of this module's globals), used through
+// the address hipGetSymbolAddress returns on that device.  Every call on a
+// device shares it; the step clears it before depositing.
+constexpr uint64_t kStaticScratchBytes = NTT_SR_STEP_STATIC_MAX_BYTES;
+static_assert(kStaticScratchBytes % 256 == 0, "scratch size");
+__device__ __align__(256) unsigned long long scratch[kStaticScratchBytes / 8];
+
+constexpr int kScratchDevices = 64;
+std::mutex    g_scratch_mu;
+void*         g_scratch_ptr[kScratchDevices] = {};
+
+// the current device's address of `scratch` (looked up once per device)
+ntt_status static_scratch(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return NTT_ERR_HIP;
+  if (dev < 0 || dev >= kScratchDevices) return NTT_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr)
+      return NTT_ERR_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return NTT_OK;
+}
+
+}  // namespace
+
+extern "C" ntt_status ntt_sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t* buff,
+                                  ntt_species_t* species, int32_t nspecies,
+                                  const ntt_sr_step_params_t* prm, const ntt_minkowski_t* metric,
+                                  void* workspace, uint64_t workspace_bytes, void* stream) {
+  return sr_step(em, cur, buff, species, nspecies, prm, metric, workspace, workspace_bytes,
+                 stream, true);
+}
+
+extern "C" ntt_status ntt_sr_step_static(ntt_field_t* em, ntt_field_t* cur, ntt_field_t* buff,
+                                         ntt_species_t* species, int32_t nspecies,
+                                         const ntt_sr_step_params_t* prm,
+                                         const ntt_minkowski_t* metric, void* stream) {
+  uint64_t need = 0;
+  ntt_status s = ntt_sr_step_workspace_size(cur, &need);
+  if (s != NTT_OK) return s;
+  if (need > kStaticScratchBytes) return NTT_ERR_CAPACITY;
+  void* ws = nullptr;
+  s = static_scratch(&ws);
+  if (s != NTT_OK) return s;
+  return sr_step(em, cur, buff, species, nspecies, prm, metric, ws, kStaticScratchBytes, stream,
+                 false);
 }
 
 // ---- ensembles of independent periodic boxes ------------------------------------

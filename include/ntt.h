@@ -355,6 +355,31 @@ NTT_API ntt_status ntt_sr_step(ntt_field_t*                em,
 /* bytes of the workspace ntt_sr_step needs for a current field of this shape */
 NTT_API ntt_status ntt_sr_step_workspace_size(const ntt_field_t* cur, uint64_t* bytes);
 
+/* Largest workspace (ntt_sr_step_workspace_size) ntt_sr_step_static serves:
+ * 32 MiB, i.e. 1,398,101 ghost-extended cells (2D up to 1178 x 1178 active
+ * cells, 3D up to 107^3). */
+#define NTT_SR_STEP_STATIC_MAX_BYTES (32ull << 20)
+
+/* ntt_sr_step without the workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the fixed-point current accumulator is a
+ * statically allocated device array of NTT_SR_STEP_STATIC_MAX_BYTES that the
+ * library holds on every GPU it runs on until the process ends.  The call
+ * allocates nothing (no hipMalloc/hipFree); fields and particles are the
+ * caller's, as for ntt_sr_step, and results are identical to it.  A box whose
+ * workspace would exceed the maximum is rejected with NTT_ERR_CAPACITY.  All
+ * calls on one GPU share the array (the step clears it before depositing),
+ * so calls that overlap on one GPU (concurrent streams or host threads) race.
+ * No counterpart in the reference. */
+NTT_API ntt_status ntt_sr_step_static(ntt_field_t*                em,
+                                      ntt_field_t*                cur,
+                                      ntt_field_t*                buff,
+                                      ntt_species_t*              species,
+                                      int32_t                     nspecies,
+                                      const ntt_sr_step_params_t* prm,
+                                      const ntt_minkowski_t*      metric,
+                                      void*                       stream);
+
 /* Cells per dimension (w[2] = 1 in 2D) of the LDS window of ntt_sr_step's
  * fused push + deposit: a chunk whose live particles span a box of b cells
  * runs from LDS when b + 3 <= w in every dimension (the [-1, +2] reach of
