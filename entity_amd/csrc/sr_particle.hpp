@@ -531,58 +531,66 @@ __device__ __forceinline__ void deposit_esirkepov_3d(Sink& J, int G, const PStat
   i2_min += G;
   i3_min += G;
   J.set_origin(i1_min, i2_min, i3_min);
-  // Per-slot form: J = prefix over the longitudinal index of
-  // -Qdt/3 (fS - iS) times the transverse weight t = iS iS' + fS fS' +
-  // (iS fS' + fS iS')/2; the longitudinal coefficients are formed once, so a
-  // slot costs one FMA (a reassociation of the reference's product:
-  // round-off only)
-  float c1[M], c2[M], c3[M];
+  // Separable form of the reference's weights: the current along x at slot
+  // (a, b, g) is C1[a] * T23[b][g], with C1[a] the prefix over a' <= a of
+  // -Qdt/3 (fS1 - iS1)[a'] and the transverse weight
+  //   T23 = iS2 iS3 + fS2 fS3 + (iS2 fS3 + fS2 iS3) / 2
+  //       = (iS2 + fS2 / 2) iS3 + (fS2 + iS2 / 2) fS3,
+  // two VALU per transverse pair and one per slot (a reassociation of the
+  // reference's products and prefix sums: round-off only).  The prefix over
+  // all M slots vanishes (charge conservation), so the longitudinal slots
+  // run over a <= O only, the last of them when the particle changed its
+  // stencil along that direction (a < d).
+  const float q3 = -Qdt * THIRD;
+  float C1[O + 1], C2[O + 1], C3[O + 1];
+  C1[0] = q3 * (fS1[0] - iS1[0]);
+  C2[0] = q3 * (fS2[0] - iS2[0]);
+  C3[0] = q3 * (fS3[0] - iS3[0]);
+#pragma unroll
+  for (int a = 1; a <= O; ++a) {
+    C1[a] = fmaf(q3, fS1[a] - iS1[a], C1[a - 1]);
+    C2[a] = fmaf(q3, fS2[a] - iS2[a], C2[a - 1]);
+    C3[a] = fmaf(q3, fS3[a] - iS3[a], C3[a - 1]);
+  }
+  float P1[M], Q1[M], P2[M], Q2[M];
 #pragma unroll
   for (int a = 0; a < M; ++a) {
-    c1[a] = -Qdt * (THIRD * (fS1[a] - iS1[a]));
-    c2[a] = -Qdt * (THIRD * (fS2[a] - iS2[a]));
-    c3[a] = -Qdt * (THIRD * (fS3[a] - iS3[a]));
+    P1[a] = fmaf(HALF, fS1[a], iS1[a]);
+    Q1[a] = fmaf(HALF, iS1[a], fS1[a]);
+    P2[a] = fmaf(HALF, fS2[a], iS2[a]);
+    Q2[a] = fmaf(HALF, iS2[a], fS2[a]);
   }
-  // jx1 (running prefix over a)
+  // jx1
 #pragma unroll
   for (int b = 0; b < M; ++b) {
 #pragma unroll
     for (int g = 0; g < M; ++g) {
-      const float t23 = (iS2[b] * iS3[g] + fS2[b] * fS3[g]) + HALF * (iS3[g] * fS2[b] + iS2[b] * fS3[g]);
-      float acc = ZERO;
+      const float t = fmaf(P2[b], iS3[g], Q2[b] * fS3[g]);
 #pragma unroll
-      for (int a = 0; a < M; ++a) {
-        acc = (a == 0) ? c1[0] * t23 : fmaf(c1[a], t23, acc);
-        if (a < d1 && b <= d2 && g <= d3) J.add_rel(a, b, g, 0, acc);
-      }
+      for (int a = 0; a <= O; ++a)
+        if (a < d1 && b <= d2 && g <= d3) J.add_rel(a, b, g, 0, C1[a] * t);
     }
   }
-  // jx2 (running prefix over b)
+  // jx2
 #pragma unroll
   for (int a = 0; a < M; ++a) {
 #pragma unroll
     for (int g = 0; g < M; ++g) {
-      const float t13 = iS1[a] * iS3[g] + fS1[a] * fS3[g] + HALF * (iS3[g] * fS1[a] + iS1[a] * fS3[g]);
-      float acc = ZERO;
+      const float t = fmaf(P1[a], iS3[g], Q1[a] * fS3[g]);
 #pragma unroll
-      for (int b = 0; b < M; ++b) {
-        acc = (b == 0) ? c2[0] * t13 : fmaf(c2[b], t13, acc);
-        if (a <= d1 && b < d2 && g <= d3) J.add_rel(a, b, g, 1, acc);
-      }
+      for (int b = 0; b <= O; ++b)
+        if (a <= d1 && b < d2 && g <= d3) J.add_rel(a, b, g, 1, C2[b] * t);
     }
   }
-  // jx3 (running prefix over g)
+  // jx3
 #pragma unroll
   for (int a = 0; a < M; ++a) {
 #pragma unroll
     for (int b = 0; b < M; ++b) {
-      const float t12 = iS1[a] * iS2[b] + fS1[a] * fS2[b] + HALF * (iS1[a] * fS2[b] + iS2[b] * fS1[a]);
-      float acc = ZERO;
+      const float t = fmaf(P1[a], iS2[b], Q1[a] * fS2[b]);
 #pragma unroll
-      for (int g = 0; g < M; ++g) {
-        acc = (g == 0) ? c3[0] * t12 : fmaf(c3[g], t12, acc);
-        if (a <= d1 && b <= d2 && g < d3) J.add_rel(a, b, g, 2, acc);
-      }
+      for (int g = 0; g <= O; ++g)
+        if (a <= d1 && b <= d2 && g < d3) J.add_rel(a, b, g, 2, C3[g] * t);
     }
   }
 }
