@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--batched", type=int, default=1, help="also time the 64-box ensemble")
     ap.add_argument("--batched-steps", type=int, default=12)
     ap.add_argument("--gr", type=int, default=1, help="also time the GR pusher runs")
-    ap.add_argument("--gr-steps", type=int, default=5)
+    ap.add_argument("--gr-steps", type=int, default=300)
     ap.add_argument("--adjoint", type=int, default=1, help="also time the adjoint (gradient) run")
     ap.add_argument("--autotune", type=int, default=1, help="also run the launch autotuner")
     ap.add_argument("--c0", type=int, default=1, help="also run the 2D 256^2 CPU-runnable case")
@@ -309,82 +309,106 @@ def wald_vertical_b(N, rmin, rmax, a, B0=1.0):
     return np.stack([b1 + 0 * i, b2 + 0 * j, b3]).astype(np.float32), m
 
 
-def gr_leg(args):
-    """GR pusher: 16 runs on 768 x 768 quasi-spherical Kerr-Schild grids
-    (r in [0.9 r_h, 50], theta in [0, pi], h = 0), spins linspace(0, 0.99, 16),
-    Wald's uniform magnetic field aligned with the spin (B0 = 1, no electric
-    field), e-/e+ at 8 ppc per species with theta = 0.1 thermal momenta in the
-    local frame (covariant code components scaled by sqrt(h_ii)), FP32 storage
-    with FP64 metric and pusher, first order shape, 10 iterations, absorbing r
-    boundaries and axis reflection."""
+def gr_run(N, a, *, ppc=8, rmax=50.0, theta=0.1, seed=4242, clear_interval=100, B0=1.0,
+           cfl=0.5):
+    """One GR test-particle run of the batched GR workload: a quasi-spherical
+    Kerr-Schild grid of N cells (h = 0, r0 = 0: log-r in [0.9 r_h, rmax],
+    theta in [0, pi]) with spin a, Wald's uniform field aligned with the spin
+    (B0, no electric field), e-/e+ at ppc per species with thermal momenta of
+    temperature theta in the local frame (covariant code components scaled by
+    sqrt(h_ii)), uniform positions, CFL `cfl` on the smallest cell, FP64 metric
+    and pusher on FP32 storage, first order shape, 10 iterations, absorbing r
+    boundaries, axis reflection, dead particles removed every clear_interval
+    steps.  Returns the GRRun at step 0."""
     import numpy as np
     import torch
     from entity_amd import _abi
-    from entity_amd.containers import GridShape, Particles
-    from entity_amd.kernels import PusherBoundaries, gr
-    N = (768, 768)
-    ppc = 8
+    from entity_amd.grpic import GRRun, GRSpeciesSpec
+    from entity_amd.kernels import gr
     dev = torch.device("cuda")
-    shape = GridShape(2, N)
     gen = torch.Generator(device=dev)
-    gen.manual_seed(4242)
-    runs = []
+    gen.manual_seed(seed)
+    rh = 1 + np.sqrt(1 - a * a)
+    met = gr.KerrSchild(_abi.METRIC_QKERR_SCHILD, N, (0.9 * rh, rmax), (0.0, np.pi), a=float(a))
+    bw, mfun = wald_vertical_b(N, 0.9 * rh, rmax, float(a), B0)
+    ic = np.arange(N[0]) + 0.5
+    jc = np.arange(N[1]) + 0.5
+    q = mfun(ic[None, :], jc[:, None])
+    dt = float(cfl * min(q["drdx"].min(), (q["r"] * q["dthdx"]).min()) / np.sqrt(2))
     npart = N[0] * N[1] * ppc
-    ext = [(e + 2 * 2) for e in N]
-    for r, a in enumerate(np.linspace(0.0, 0.99, 16)):
-        rh = 1 + np.sqrt(1 - a * a)
-        met = gr.KerrSchild(_abi.METRIC_QKERR_SCHILD, N, (0.9 * rh, 50.0), (0.0, np.pi), a=float(a))
-        bw, mfun = wald_vertical_b(N, 0.9 * rh, 50.0, float(a))
-        em = torch.zeros((6, ext[1], ext[0]), device=dev)
-        em[3:6] = torch.from_numpy(bw).to(dev)
-        em0 = em.clone()
-        # sqrt(h_ii) at the cell centres: local-frame momenta to covariant code components
-        ic = np.arange(N[0]) + 0.5
-        jc = np.arange(N[1]) + 0.5
-        q = mfun(ic[None, :], jc[:, None])
-        sc = [torch.from_numpy(np.sqrt(q[k]).astype(np.float32).ravel()).to(dev)
-              for k in ("h11", "h22", "h33")]
-        sig = float(np.sqrt(0.1))   # theta = 0.1, non-relativistic thermal spread
-        # CFL 0.5 on the smallest cell (the innermost ring)
-        dt = float(0.5 * min(q["drdx"].min(), (q["r"] * q["dthdx"]).min()) / np.sqrt(2))
-        sps = []
-        for q in (-1.0, 1.0):
-            P = Particles(2, npart, device=dev, with_phi=True)
-            cell = torch.arange(npart, device=dev) // ppc
-            P.load(i1=(cell % N[0]).to(torch.int32), i2=(cell // N[0]).to(torch.int32),
-                   dx1=torch.rand(npart, device=dev, generator=gen),
-                   dx2=torch.rand(npart, device=dev, generator=gen),
-                   ux1=torch.randn(npart, device=dev, generator=gen) * sig * sc[0][cell],
-                   ux2=torch.randn(npart, device=dev, generator=gen) * sig * sc[1][cell],
-                   ux3=torch.randn(npart, device=dev, generator=gen) * sig * sc[2][cell],
-                   weight=torch.ones(npart, device=dev), tag=torch.ones(npart, dtype=torch.int16,
-                                                                        device=dev))
-            ctx = gr.PusherContext(_abi.PUSHER_BORIS, 1.0, q, dt, 1.0, N[0], N[1])
-            bc = PusherBoundaries([(_abi.PRTL_BC_ABSORB, _abi.PRTL_BC_ABSORB),
-                                   (_abi.PRTL_BC_AXIS, _abi.PRTL_BC_AXIS), (0, 0)])
-            sps.append(gr.Pusher_kernel(ctx, bc, P, em, em0, shape, met, 1))
-        runs.append(sps)
-    for sps in runs:
-        for k in sps:
-            k()
+    run = GRRun(met, [GRSpeciesSpec("e-", 1.0, -1.0), GRSpeciesSpec("e+", 1.0, 1.0)], npart,
+                dt=dt, clear_interval=clear_interval)
+    run.em[3:6] = torch.from_numpy(bw).to(dev)
+    sc = [torch.from_numpy(np.sqrt(q[k]).astype(np.float32).ravel()).to(dev)
+          for k in ("h11", "h22", "h33")]
+    sig = float(np.sqrt(theta))      # non-relativistic thermal spread
+    cell = torch.arange(npart, device=dev) // ppc
+    for P in run.species:
+        P.load(i1=(cell % N[0]).to(torch.int32), i2=(cell // N[0]).to(torch.int32),
+               dx1=torch.rand(npart, device=dev, generator=gen),
+               dx2=torch.rand(npart, device=dev, generator=gen),
+               ux1=torch.randn(npart, device=dev, generator=gen) * sig * sc[0][cell],
+               ux2=torch.randn(npart, device=dev, generator=gen) * sig * sc[1][cell],
+               ux3=torch.randn(npart, device=dev, generator=gen) * sig * sc[2][cell],
+               weight=torch.ones(npart, device=dev),
+               tag=torch.ones(npart, dtype=torch.int16, device=dev))
+    return run
+
+
+# FP64 flop per particle-step of the GR push on the batched workload, from the
+# SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 counters (FMA counted twice)
+GR_FP64_FLOP_PER_PARTICLE = 3506.0
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+
+
+def gr_leg(args):
+    """GR pusher-only runs, batched: 16 runs on 768 x 768 quasi-spherical
+    Kerr-Schild grids (r in [0.9 r_h, 50], theta in [0, pi], h = 0), spins
+    linspace(0, 0.99, 16), Wald's uniform magnetic field aligned with the spin
+    (B0 = 1, no electric field), e-/e+ at 8 ppc per species with theta = 0.1
+    thermal momenta in the local frame, FP32 storage with FP64 metric and
+    pusher, first order shape, 10 iterations, absorbing r boundaries and axis
+    reflection, dead particles removed every 100 steps.  Step 0 (em0 <- em and
+    the first push) runs eagerly; steps 1..gr_steps-1 replay one HIP graph
+    holding the single batched launch over the 32 (run, species) entries."""
+    import numpy as np
+    import torch
+    from entity_amd.grpic import GRBatch
+    N = (768, 768)
+    runs = [gr_run(N, float(a), seed=4242 + k) for k, a in enumerate(np.linspace(0.0, 0.99, 16))]
+    total0 = sum(sp.npart for r in runs for sp in r.species)
+    batch = GRBatch(runs)
+    batch.step_forward()
+    batch.capture()
     torch.cuda.synchronize()
-    K = args.gr_steps
+    K = args.gr_steps - 1
+    pushed = 0
     s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
     s0.record()
     for _ in range(K):
-        for sps in runs:
-            for k in sps:
-                k()
+        pushed += sum(sp.npart for r in runs for sp in r.species)
+        batch.step_graph()
     s1.record()
     torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
     ms = s0.elapsed_time(s1) / K
-    total = 16 * 2 * npart
+    rate = pushed / (ms * 1e-3 * K)
+    alive = sum(sp.npart for r in runs for sp in r.species)
+    tflops = rate * GR_FP64_FLOP_PER_PARTICLE / 1e12
     return {"workload": "16 QKS runs 768x768 (r in [0.9 r_h, 50]), spins 0..0.99, Wald field "
                         "B0 = 1, e-/e+ 8 ppc/species theta 0.1 (151M particles), CFL 0.5, "
-                        "FP32 storage / FP64 metric, 10 iterations",
-            "particles": total, "steps": K, "value": total / (ms * 1e-3),
-            "unit": "particle-steps/s", "ms_per_step": ms,
-            "hbm_gbs": total * 78 / (ms * 1e-3) / 1e9}
+                        "FP32 storage / FP64 metric, 10 iterations, absorbing r / axis theta, "
+                        "dead removed every 100 steps; one batched launch per step (graph)",
+            "particles": total0, "particles_at_end": alive, "steps": K + 1, "timed_steps": K,
+            "value": rate, "unit": "particle-steps/s (particles held at each step)",
+            "ms_per_step": ms, "wall_s": wall,
+            "hbm_gbs": rate * 78 / 1e9,
+            "roofline": {"bound": "valu_fp64", "achieved": tflops,
+                         "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": tflops / FP64_VECTOR_PEAK_TFLOPS,
+                         "flop_per_particle_step": GR_FP64_FLOP_PER_PARTICLE,
+                         "flop_source": "profiles/k_gr_push_qks_768x768_counters.md"}}
 
 
 # the adjoint workload: 128 x 128 cells, 4 species x 16 ppc, 40 steps,

@@ -21,222 +21,15 @@
 //   computed on the host.
 // * UpdatePhi of the reference takes phi by value and has no effect; it is
 //   not evaluated.
-#include "internal.hpp"
+#include "gr_metric.hpp"
 
 #include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <vector>
 
 namespace ntt {
 namespace {
-
-constexpr double kPi          = 3.14159265358979323846;
-constexpr double kInvPi       = 0.31830988618379067154;
-constexpr double kPiSqr       = 9.86960440108935861882;
-constexpr double kInvPiSqr    = 0.10132118364233777144;
-constexpr double kSmallAngleGR = 1e-5;
-
-template <typename R> __host__ __device__ __forceinline__ R sqr(R x) { return x * x; }
-template <typename R> __host__ __device__ __forceinline__ R cube(R x) { return x * x * x; }
-
-template <typename R>
-__host__ __device__ __forceinline__ bool almost_zero(R a) {
-  const R eps = sizeof(R) == 4 ? static_cast<R>(1.1920928955078125e-7)
-                               : static_cast<R>(2.220446049250313e-16);
-  return (a < 0 ? -a : a) <= eps;
-}
-
-__device__ __forceinline__ void sin_cos(float x, float* s, float* c) { sincosf(x, s, c); }
-__device__ __forceinline__ void sin_cos(double x, double* s, double* c) { sincos(x, s, c); }
-
-// Metric constants (code-unit spacings) computed on the host in R
-template <typename R>
-struct KSConst {
-  R a, r0, h0, x1_min, x2_min;
-  R dr, dtheta, dr_inv, dtheta_inv;
-  R chi_min, eta_min, dchi, deta, dchi_inv, deta_inv;
-  R th_small_cd, th_large_cd;  // clamp targets: convert<2, Ph, Cd>(small / large)
-  double large_angle;          // PI - SMALL_ANGLE_GR, compared in double
-  R      small_angle;
-};
-
-// Metric quantities at one point.  Lower components h_ij carry an underscore.
-template <typename R>
-struct MPt {
-  R r, th, s, c;       // r, theta, sin, cos
-  R S, Dl, Av, z;      // Sigma, Delta, A, z = 2r/Sigma
-  R e, eta, dtde;      // QKS: exp(chi), eta, dtheta/deta
-  R h11, h22, h33, h13;
-  R h_22, h_33, h_13;
-  R alpha, beta1;
-};
-
-template <typename R, int KIND>
-struct KSMetric {
-  KSConst<R> k;
-
-  __device__ __forceinline__ R eta2theta(R eta) const {
-    if (almost_zero(k.h0)) return eta;
-    return eta + R(2) * k.h0 * eta * (static_cast<R>(kPi) - R(2) * eta) *
-                   (static_cast<R>(kPi) - eta) * static_cast<R>(kInvPiSqr);
-  }
-  __device__ __forceinline__ R dtheta_deta(R eta) const {
-    if (almost_zero(k.h0)) return R(1);
-    return (R(1) + R(2) * k.h0 +
-            static_cast<R>(12.0) * k.h0 * (eta * static_cast<R>(kInvPi)) *
-              ((eta * static_cast<R>(kInvPi)) - R(1)));
-  }
-  __device__ __forceinline__ R dx_dt(R eta) const {
-    if (almost_zero(k.h0)) return k.deta;
-    return k.deta * (R(1) + R(2) * k.h0 * static_cast<R>(kInvPiSqr) *
-                              (R(2) * R(3) * sqr(eta) - R(2) * R(3) * static_cast<R>(kPi) * eta +
-                               static_cast<R>(kPiSqr)));
-  }
-  // convert<2, Cd, Ph>
-  __device__ __forceinline__ R theta_ph(R x2) const {
-    if constexpr (KIND == NTT_METRIC_QKERR_SCHILD) return eta2theta(x2 * k.deta + k.eta_min);
-    else return x2 * k.dtheta + k.x2_min;
-  }
-
-  // r, theta, Sigma, ..., lapse, shift, inverse metric and the lower
-  // components the transformations use (kerr_schild.h:143-330,
-  // qkerr_schild.h:145-385, kerr_schild_0.h)
-  __device__ __forceinline__ void eval(R x1, R x2, MPt<R>& m) const {
-    const R a2 = sqr(k.a);
-    if constexpr (KIND == NTT_METRIC_QKERR_SCHILD) {
-      const R chi = x1 * k.dchi + k.chi_min;
-      m.e   = exp(chi);
-      m.r   = k.r0 + m.e;
-      m.eta = x2 * k.deta + k.eta_min;
-      m.th  = eta2theta(m.eta);
-      m.dtde = dtheta_deta(m.eta);
-    } else {
-      m.e = R(0);
-      m.eta = R(0);
-      m.dtde = R(1);
-      m.r  = x1 * k.dr + k.x1_min;
-      m.th = x2 * k.dtheta + k.x2_min;
-    }
-    sin_cos(m.th, &m.s, &m.c);
-    const R s2 = sqr(m.s);
-    if constexpr (KIND == NTT_METRIC_KERR_SCHILD_0) {
-      m.S = sqr(m.r);
-      m.Dl = m.Av = m.z = R(0);
-      m.h11 = sqr(k.dr_inv);
-      m.h22 = sqr(k.dtheta_inv / m.r);
-      m.h33 = R(1) / sqr(m.r * m.s);
-      m.h13 = R(0);
-      m.h_22 = sqr(k.dtheta) * sqr(m.r);
-      m.h_33 = sqr(m.r * m.s);
-      m.h_13 = R(0);
-      m.alpha = R(1);
-      m.beta1 = R(0);
-    } else {
-      m.S  = sqr(m.r) + a2 * sqr(m.c);
-      m.Dl = sqr(m.r) - R(2) * m.r + a2;
-      m.Av = sqr(sqr(m.r) + a2) - a2 * m.Dl * s2;
-      m.z  = R(2) * m.r / m.S;
-      const R opz = R(1) + m.z;
-      m.h33  = R(1) / (m.S * s2);
-      m.h_33 = m.Av * s2 / m.S;
-      m.alpha = R(1) / sqrt(opz);
-      if constexpr (KIND == NTT_METRIC_KERR_SCHILD) {
-        m.h11  = sqr(k.dr_inv) * m.Av / (m.S * (m.S + R(2) * m.r));
-        m.h22  = sqr(k.dtheta_inv) / m.S;
-        m.h13  = k.dr_inv * k.a / m.S;
-        m.h_22 = sqr(k.dtheta) * m.S;
-        m.h_13 = -k.dr * k.a * opz * s2;
-        m.beta1 = k.dr_inv * m.z / opz;
-      } else {
-        const R einv = R(1) / m.e;
-        m.h11  = (sqr(einv) / sqr(k.dchi)) * m.Av / (m.S * (m.S + R(2) * m.r));
-        m.h22  = R(1) / (m.S * sqr(m.dtde) * sqr(k.deta));
-        m.h13  = (einv * k.dchi_inv) * k.a / m.S;
-        m.h_22 = sqr(k.deta) * sqr(m.dtde) * m.S;
-        m.h_13 = -k.dchi * m.e * k.a * opz * s2;
-        m.beta1 = einv * k.dchi_inv * m.z / opz;
-      }
-    }
-  }
-
-  // d/dx1 and d/dx2 of alpha, beta1, h^11, h^22, h^33, h^13 at m (the
-  // metrics' dr_* / dt_* functions, same quirks: the quasi-spherical h^22
-  // derivatives omit the angle-stretching factor)
-  struct Deriv {
-    R dr_alpha, dt_alpha, dr_beta1, dt_beta1;
-    R dr_h11, dr_h22, dr_h33, dr_h13, dt_h11, dt_h22, dt_h33, dt_h13;
-  };
-  __device__ __forceinline__ void derivs(const MPt<R>& m, Deriv& d) const {
-    const R a2 = sqr(k.a);
-    const R s2 = sqr(m.s);
-    if constexpr (KIND == NTT_METRIC_KERR_SCHILD_0) {
-      d.dr_alpha = d.dt_alpha = d.dr_beta1 = d.dt_beta1 = R(0);
-      d.dr_h11 = d.dr_h13 = d.dt_h11 = d.dt_h22 = d.dt_h13 = R(0);
-      d.dr_h22 = -R(2) / cube(m.r) * sqr(k.dtheta_inv) * k.dr;
-      d.dr_h33 = -R(2) / cube(m.r) / s2 * k.dr;
-      d.dt_h33 = -R(2) * m.c / sqr(m.r) / cube(m.s) * k.dtheta;
-      return;
-    }
-    const R S = m.S, r = m.r;
-    const R opz = R(1) + m.z;
-    const R al3 = cube(m.alpha);
-    if constexpr (KIND == NTT_METRIC_KERR_SCHILD) {
-      const R dr_S = R(2) * r * k.dr;
-      R dt_S = -R(2) * a2 * m.s * m.c * k.dtheta;
-      if (almost_zero(dt_S)) dt_S = R(0);
-      R dt_A = -R(2) * a2 * m.s * m.c * m.Dl * k.dtheta;
-      if (almost_zero(dt_A)) dt_A = R(0);
-      const R dr_D = R(2) * k.dr * (r - R(1));
-      const R dr_A = R(4) * r * k.dr * (sqr(r) + a2) - a2 * s2 * dr_D;
-      d.dr_alpha = -(k.dr * S - r * dr_S) * al3 / sqr(S);
-      d.dt_alpha = al3 * r * dt_S / sqr(S);
-      d.dr_beta1 = k.dr_inv * R(2) * (k.dr * S - r * dr_S) / sqr(S + R(2) * r);
-      d.dt_beta1 = -k.dr_inv * R(2) * r * dt_S / sqr(S * opz);
-      const R SS = S * (S + R(2) * r);
-      d.dr_h11 = (SS * dr_A - R(2) * m.Av * (r * dr_S + S * (dr_S + k.dr))) / sqr(SS) *
-                 sqr(k.dr_inv);
-      d.dr_h22 = -dr_S / sqr(S) * sqr(k.dtheta_inv);
-      d.dr_h33 = -dr_S / sqr(S) / s2;
-      d.dr_h13 = -k.a * dr_S / sqr(S) * k.dr_inv;
-      d.dt_h11 = (SS * dt_A - R(2) * m.Av * dt_S * (r + S)) / sqr(SS) * sqr(k.dr_inv);
-      d.dt_h22 = -dt_S / sqr(S) * sqr(k.dtheta_inv);
-      d.dt_h33 = -R(2) * k.dtheta * m.c * (S - a2 * s2) / cube(m.s) / sqr(S);
-      d.dt_h13 = -k.a * dt_S / sqr(S) * k.dr_inv;
-    } else {
-      const R eta  = m.eta;
-      const R einv = R(1) / m.e;
-      const R dx_r = k.dchi * m.e;
-      const R dr_S = R(2) * r * dx_r;
-      const R dxdt = dx_dt(eta);
-      // dt_alpha uses the stretching derivative without the AlmostZero cuts
-      const R dxdt_inline = k.deta * (R(1) + R(2) * k.h0 * static_cast<R>(kInvPiSqr) *
-                                                  (R(2) * R(3) * sqr(eta) -
-                                                   R(2) * R(3) * static_cast<R>(kPi) * eta +
-                                                   static_cast<R>(kPiSqr)));
-      R dt_S = -R(2) * a2 * m.s * m.c * dxdt;
-      if (almost_zero(dt_S)) dt_S = R(0);
-      R dt_A = -R(2) * a2 * m.s * m.c * m.Dl * dxdt;
-      if (almost_zero(dt_A)) dt_A = R(0);
-      const R dr_D = R(2) * dx_r * (r - R(1));
-      const R dr_A = R(4) * r * dx_r * (sqr(r) + a2) - a2 * s2 * dr_D;
-      const R e2inv = sqr(einv) / sqr(k.dchi);
-      d.dr_alpha = -(dx_r * S - r * dr_S) * al3 / sqr(S);
-      d.dt_alpha = r * (-R(2) * a2 * m.s * m.c * dxdt_inline) * al3 / sqr(S);
-      d.dr_beta1 = einv * k.dchi_inv * R(2) * (dx_r * S - r * dr_S) / sqr(S + R(2) * r) -
-                   k.dchi * einv * k.dchi_inv * m.z / opz;
-      d.dt_beta1 = -einv * k.dchi_inv * R(2) * r * dt_S / sqr(S * opz);
-      const R SS = S * (S + R(2) * r);
-      d.dr_h11 = (e2inv * (SS * dr_A - R(2) * m.Av * (r * dr_S + S * (dr_S + dx_r))) / sqr(SS)) -
-                 R(2) * k.dchi * e2inv * m.Av / SS;
-      d.dr_h22 = -dr_S / sqr(S) / sqr(k.deta);
-      d.dr_h33 = -dr_S / sqr(S) / s2;
-      d.dr_h13 = -k.a * dr_S / sqr(S) * (einv * k.dchi_inv) -
-                 k.dchi * (einv * k.dchi_inv) * k.a / S;
-      d.dt_h11 = e2inv * (SS * dt_A - R(2) * m.Av * dt_S * (r + S)) / sqr(SS);
-      d.dt_h22 = -dt_S / sqr(S) / sqr(k.deta);
-      d.dt_h33 = -(dt_S + R(2) * m.c / m.s * S * dxdt) / sqr(S * m.s);
-      d.dt_h13 = -k.a * dt_S / sqr(S) * (einv * k.dchi_inv);
-    }
-  }
-};
 
 // tetrad transformations at a point (kerr_schild.h:599-630)
 template <typename R, int KIND>
@@ -396,10 +189,10 @@ struct Gather2D {
   }
 };
 
+// The push of particle p of one species (the body of Pusher_kernel::operator()).
 template <typename R, int KIND, int O, bool MASSLESS>
-__global__ __launch_bounds__(256) void k_gr_push(GRArgs g, KSMetric<R, KIND> M, R dt, R ndth) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= g.P.n) return;
+__device__ __forceinline__ void gr_push_one(const GRArgs& g, const KSMetric<R, KIND>& M, R dt,
+                                            R ndth, int64_t p) {
   const int16_t tag = g.P.tag[p];
   if (tag != NTT_TAG_ALIVE) {
     if (tag != NTT_TAG_DEAD) raise_device_error(g.err, NTT_DEVERR_INVALID_TAG);
@@ -566,63 +359,45 @@ __global__ __launch_bounds__(256) void k_gr_push(GRArgs g, KSMetric<R, KIND> M, 
   if (ntag != NTT_TAG_ALIVE) g.P.tag[p] = ntag;
 }
 
-// host: metric constants in R, as the reference's metric constructors
-// compute them (kerr_schild.h:79-90, qkerr_schild.h:79-94)
-template <typename R>
-R theta2eta_host(R theta, R h0) {
-  if (almost_zero(h0)) return theta;
-  const double hd = static_cast<double>(h0), th = static_cast<double>(theta);
-  const double two_pi = 6.28318530717958647692, sqrt3 = 1.73205080756887729352;
-  const double Rv = std::pow(
-    -9.0 * hd * hd * (kPi - 2.0 * th) +
-      sqrt3 * std::sqrt(hd * hd * hd *
-                        ((4.0 - hd) * (kPi + hd * two_pi) * (kPi + hd * two_pi) -
-                         108.0 * hd * kPi * th + 108.0 * hd * th * th)),
-    1.0 / 3.0);
-  const double PI_23 = 2.14502939711102560008, PI_13 = 1.46459188756152326302;
-  const double TWO_23 = 1.58740105196819947475, THREE_13 = 1.442249570307408382321;
-  const double TWO_13 = 1.2599210498948731647672, THREE_PI_23 = 4.46184094890142313715794;
-  return static_cast<R>(PI_23 *
-                        (6.0 * PI_13 + 2.0 * TWO_13 * (hd - 1.0) * THREE_PI_23 / Rv +
-                         TWO_23 * THREE_13 * Rv / hd) /
-                        12.0);
+template <typename R, int KIND, int O, bool MASSLESS>
+__global__ __launch_bounds__(256) void k_gr_push(GRArgs g, KSMetric<R, KIND> M, R dt, R ndth) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= g.P.n) return;
+  gr_push_one<R, KIND, O, MASSLESS>(g, M, dt, ndth, p);
 }
 
+// One species of one run in a batched launch: the push's arguments and its
+// metric constants, as the single-species launch derives them.
 template <typename R>
-ntt_status make_const(const ntt_ks_metric_t* m, KSConst<R>* k) {
-  if (m->nx1 < 1 || m->nx2 < 1) return NTT_ERR_ARG;
-  if (!(m->x1_max > m->x1_min) || !(m->x2_max > m->x2_min)) return NTT_ERR_ARG;
-  k->a      = (m->kind == NTT_METRIC_KERR_SCHILD_0) ? R(0) : static_cast<R>(m->a);
-  k->r0     = static_cast<R>(m->r0);
-  k->h0     = static_cast<R>(m->h);
-  k->x1_min = static_cast<R>(m->x1_min);
-  k->x2_min = static_cast<R>(m->x2_min);
-  const R x1_max = static_cast<R>(m->x1_max), x2_max = static_cast<R>(m->x2_max);
-  k->dr         = (x1_max - k->x1_min) / static_cast<R>(m->nx1);
-  k->dtheta     = (x2_max - k->x2_min) / static_cast<R>(m->nx2);
-  k->dr_inv     = R(1) / k->dr;
-  k->dtheta_inv = R(1) / k->dtheta;
-  k->chi_min = k->eta_min = k->dchi = k->deta = k->dchi_inv = k->deta_inv = R(0);
-  if (m->kind == NTT_METRIC_QKERR_SCHILD) {
-    if (!(k->x1_min > k->r0)) return NTT_ERR_ARG;
-    k->chi_min  = std::log(k->x1_min - k->r0);
-    k->eta_min  = theta2eta_host(k->x2_min, k->h0);
-    k->dchi     = (std::log(x1_max - k->r0) - k->chi_min) / static_cast<R>(m->nx1);
-    k->deta     = (theta2eta_host(x2_max, k->h0) - k->eta_min) / static_cast<R>(m->nx2);
-    k->dchi_inv = R(1) / k->dchi;
-    k->deta_inv = R(1) / k->deta;
+struct GREntry {
+  GRArgs     g;
+  KSConst<R> k;
+  R          dt, ndth;
+};
+
+// Every (run, species) entry of a batch in one launch: entry e owns the
+// blocks [blk0[e], blk0[e + 1]), one particle per thread.  The entry index is
+// uniform across the block, so its arguments are read in place from the
+// table (scalar loads) rather than copied into registers.
+template <typename R, int KIND, int O, bool MASSLESS>
+__global__ __launch_bounds__(256) void k_gr_push_batched(const GREntry<R>* __restrict__ table,
+                                                         const int64_t* __restrict__ blk0,
+                                                         int nent) {
+  const int64_t b = blockIdx.x;
+  int lo = 0, hi = nent - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (blk0[mid] <= b) lo = mid;
+    else hi = mid - 1;
   }
-  k->small_angle = static_cast<R>(kSmallAngleGR);
-  k->large_angle = kPi - static_cast<double>(k->small_angle);
-  const R large  = static_cast<R>(k->large_angle);
-  if (m->kind == NTT_METRIC_QKERR_SCHILD) {
-    k->th_small_cd = (theta2eta_host(k->small_angle, k->h0) - k->eta_min) * k->deta_inv;
-    k->th_large_cd = (theta2eta_host(large, k->h0) - k->eta_min) * k->deta_inv;
-  } else {
-    k->th_small_cd = (k->small_angle - k->x2_min) * k->dtheta_inv;
-    k->th_large_cd = (large - k->x2_min) * k->dtheta_inv;
-  }
-  return NTT_OK;
+  const int        e = __builtin_amdgcn_readfirstlane(lo);
+  const GREntry<R>& E = table[e];
+  const int64_t    p = (b - blk0[e]) * 256 + threadIdx.x;
+  if (p >= E.g.P.n) return;
+  // KSMetric is KSConst plus methods (same layout): read the constants in place
+  static_assert(sizeof(KSMetric<R, KIND>) == sizeof(KSConst<R>), "KSMetric layout");
+  const KSMetric<R, KIND>& M = *reinterpret_cast<const KSMetric<R, KIND>*>(&E.k);
+  gr_push_one<R, KIND, O, MASSLESS>(E.g, M, E.dt, E.ndth, p);
 }
 
 template <typename R, int KIND, int O>
@@ -666,15 +441,13 @@ ntt_status run_gr(const ntt_gr_pusher_ctx_t* ctx, const GRArgs& g, const ntt_ks_
   return check_launch();
 }
 
-}  // namespace
-}  // namespace ntt
-
-using namespace ntt;
-
-extern "C" ntt_status ntt_gr_push(const ntt_gr_pusher_ctx_t* ctx, const ntt_prtl_bc_t* bc,
-                                  ntt_particles_t* prtls, const ntt_field_t* em,
-                                  const ntt_field_t* em0, const ntt_ks_metric_t* metric,
-                                  int32_t shape_order, void* stream) {
+// Validates one species' push as ntt_gr_push does and fills its GRArgs;
+// *skip is set for a species with nothing to push.
+ntt_status prepare_gr(const ntt_gr_pusher_ctx_t* ctx, const ntt_prtl_bc_t* bc,
+                      const ntt_particles_t* prtls, const ntt_field_t* em,
+                      const ntt_field_t* em0, const ntt_ks_metric_t* metric, int32_t shape_order,
+                      GRArgs* g, bool* massless, bool* skip) {
+  *skip = false;
   if (ctx == nullptr || bc == nullptr || prtls == nullptr || metric == nullptr)
     return NTT_ERR_ARG;
   if (shape_order < 0 || shape_order > 2) return NTT_ERR_UNSUPPORTED;
@@ -683,40 +456,237 @@ extern "C" ntt_status ntt_gr_push(const ntt_gr_pusher_ctx_t* ctx, const ntt_prtl
     return NTT_ERR_UNSUPPORTED;
   // grpic::ParticlePush (particle_pusher.h:53-87): PHOTON -> massless,
   // BORIS -> massive, anything else is not implemented
-  bool massless;
-  if (ctx->pusher_flags == NTT_PUSHER_PHOTON) massless = true;
-  else if (ctx->pusher_flags == NTT_PUSHER_BORIS) massless = false;
-  else if (ctx->pusher_flags == NTT_PUSHER_NONE) return NTT_OK;
-  else return NTT_ERR_UNSUPPORTED;
+  if (ctx->pusher_flags == NTT_PUSHER_PHOTON) *massless = true;
+  else if (ctx->pusher_flags == NTT_PUSHER_BORIS) *massless = false;
+  else if (ctx->pusher_flags == NTT_PUSHER_NONE) {
+    *skip = true;
+    return NTT_OK;
+  } else return NTT_ERR_UNSUPPORTED;
   if (ctx->niter < 0) return NTT_ERR_ARG;
-  if (!massless && !(ctx->mass != 0.0f)) return NTT_ERR_ARG;
+  if (!*massless && !(ctx->mass != 0.0f)) return NTT_ERR_ARG;
   if (ctx->precision != 0 && ctx->precision != 32 && ctx->precision != 64)
     return NTT_ERR_ARG;
-  GRArgs g{};
-  ntt_status s = make_particle_view(prtls, 2, true, &g.P);
+  *g = GRArgs{};
+  ntt_status s = make_particle_view(prtls, 2, true, &g->P);
   if (s != NTT_OK) return s;
-  if (g.P.n == 0) return NTT_OK;
-  if (!massless) {
+  if (!*massless) {
     if (em == nullptr || em0 == nullptr) return NTT_ERR_ARG;
-    s = make_field_view(em, 6, &g.DB);
+    s = make_field_view(em, 6, &g->DB);
     if (s != NTT_OK) return s;
-    s = make_field_view(em0, 6, &g.DB0);
+    s = make_field_view(em0, 6, &g->DB0);
     if (s != NTT_OK) return s;
     if (em->dim != 2 || em0->dim != 2) return NTT_ERR_UNSUPPORTED;
-    if (g.DB.N1 != g.DB0.N1 || g.DB.N2 != g.DB0.N2 || g.DB.ngh != g.DB0.ngh)
+    if (g->DB.N1 != g->DB0.N1 || g->DB.N2 != g->DB0.N2 || g->DB.ngh != g->DB0.ngh)
       return NTT_ERR_ARG;
-    if (g.DB.n1 != ctx->ni1 || g.DB.n2 != ctx->ni2) return NTT_ERR_ARG;
+    if (g->DB.n1 != ctx->ni1 || g->DB.n2 != ctx->ni2) return NTT_ERR_ARG;
   }
-  g.ni1 = ctx->ni1;
-  g.ni2 = ctx->ni2;
-  g.niter = ctx->niter;
+  g->ni1 = ctx->ni1;
+  g->ni2 = ctx->ni2;
+  g->niter = ctx->niter;
   // kernel::gr::PusherBoundaries (context.h:213-228)
-  g.absorb_i1min = bc->bc[0][0] == NTT_PRTL_BC_ABSORB || bc->bc[0][0] == NTT_PRTL_BC_HORIZON;
-  g.absorb_i1max = bc->bc[0][1] == NTT_PRTL_BC_ABSORB || bc->bc[0][1] == NTT_PRTL_BC_HORIZON;
-  g.axis_i2min   = bc->bc[1][0] == NTT_PRTL_BC_AXIS;
-  g.axis_i2max   = bc->bc[1][1] == NTT_PRTL_BC_AXIS;
-  g.err = device_error_word();
-  if (g.err == nullptr) return NTT_ERR_HIP;
+  g->absorb_i1min = bc->bc[0][0] == NTT_PRTL_BC_ABSORB || bc->bc[0][0] == NTT_PRTL_BC_HORIZON;
+  g->absorb_i1max = bc->bc[0][1] == NTT_PRTL_BC_ABSORB || bc->bc[0][1] == NTT_PRTL_BC_HORIZON;
+  g->axis_i2min   = bc->bc[1][0] == NTT_PRTL_BC_AXIS;
+  g->axis_i2max   = bc->bc[1][1] == NTT_PRTL_BC_AXIS;
+  g->err = device_error_word();
+  if (g->err == nullptr) return NTT_ERR_HIP;
+  if (g->P.n == 0) *skip = true;
+  return NTT_OK;
+}
+
+template <typename R>
+ntt_status make_entry(const ntt_gr_pusher_ctx_t* ctx, const GRArgs& g,
+                      const ntt_ks_metric_t* metric, GREntry<R>* e) {
+  e->g = g;
+  ntt_status s = make_const<R>(metric, &e->k);
+  if (s != NTT_OK) return s;
+  e->dt   = static_cast<R>(ctx->dt);
+  e->ndth = R(0.5) * static_cast<R>(ctx->charge / ctx->mass) * static_cast<R>(ctx->omegaB0) *
+            e->dt;
+  return NTT_OK;
+}
+
+}  // namespace
+}  // namespace ntt
+
+using namespace ntt;
+
+// ---- batches of (run, species) entries ---------------------------------------
+struct ntt_gr_batch {
+  struct Group {
+    int      precision, kind;       // 32 / 64, NTT_METRIC_*
+    bool     massless;
+    std::vector<char>    host;      // GREntry<R> table
+    std::vector<int64_t> blk0;      // nent + 1 block offsets
+    std::vector<int32_t> entry_of;  // batch entry index of each table entry
+    void*    d_table = nullptr;
+    int64_t* d_blk0  = nullptr;
+  };
+  int32_t                 order = 1;
+  std::vector<Group>      groups;
+  std::vector<std::pair<int, int>> where;  // batch entry -> (group, table index) or (-1, -1)
+  std::vector<int64_t>    planned;          // particles each entry was planned for
+};
+
+namespace {
+
+template <typename R>
+size_t entry_size() { return sizeof(GREntry<R>); }
+
+template <typename R, int KIND, bool MASSLESS>
+void launch_batched_k(int order, const void* table, const int64_t* blk0, int nent, int64_t nb,
+                      hipStream_t st) {
+  const auto* t = static_cast<const GREntry<R>*>(table);
+  if (order == 0) k_gr_push_batched<R, KIND, 0, MASSLESS><<<nb, 256, 0, st>>>(t, blk0, nent);
+  else if (order == 1) k_gr_push_batched<R, KIND, 1, MASSLESS><<<nb, 256, 0, st>>>(t, blk0, nent);
+  else k_gr_push_batched<R, KIND, 2, MASSLESS><<<nb, 256, 0, st>>>(t, blk0, nent);
+}
+
+template <typename R, bool MASSLESS>
+void launch_batched_kind(int kind, int order, const void* table, const int64_t* blk0, int nent,
+                         int64_t nb, hipStream_t st) {
+  if (kind == NTT_METRIC_KERR_SCHILD)
+    launch_batched_k<R, NTT_METRIC_KERR_SCHILD, MASSLESS>(order, table, blk0, nent, nb, st);
+  else if (kind == NTT_METRIC_QKERR_SCHILD)
+    launch_batched_k<R, NTT_METRIC_QKERR_SCHILD, MASSLESS>(order, table, blk0, nent, nb, st);
+  else
+    launch_batched_k<R, NTT_METRIC_KERR_SCHILD_0, MASSLESS>(order, table, blk0, nent, nb, st);
+}
+
+void free_batch(ntt_gr_batch* b) {
+  for (auto& g : b->groups) {
+    if (g.d_table) (void)hipFree(g.d_table);
+    if (g.d_blk0) (void)hipFree(g.d_blk0);
+  }
+  delete b;
+}
+
+}  // namespace
+
+extern "C" ntt_status ntt_gr_batch_create(const ntt_gr_batch_entry_t* entries, int32_t nentries,
+                                          int32_t shape_order, ntt_gr_batch_t* out) {
+  if (out == nullptr || nentries < 0 || (nentries > 0 && entries == nullptr)) return NTT_ERR_ARG;
+  *out = nullptr;
+  if (shape_order < 0 || shape_order > 2) return NTT_ERR_UNSUPPORTED;
+  auto* b = new ntt_gr_batch;
+  b->order = shape_order;
+  b->where.assign(nentries, { -1, -1 });
+  b->planned.assign(nentries, 0);
+  ntt_status s = NTT_OK;
+  for (int32_t i = 0; i < nentries && s == NTT_OK; ++i) {
+    const ntt_gr_batch_entry_t& E = entries[i];
+    GRArgs g;
+    bool   massless = false, skip = false;
+    s = prepare_gr(&E.ctx, &E.bc, &E.prtls, &E.em, &E.em0, &E.metric, shape_order, &g,
+                   &massless, &skip);
+    if (s != NTT_OK || skip) continue;
+    const int prec = E.ctx.precision == 32 ? 32 : 64;
+    int gi = -1;
+    for (size_t k = 0; k < b->groups.size(); ++k) {
+      const auto& G = b->groups[k];
+      if (G.precision == prec && G.kind == E.metric.kind && G.massless == massless)
+        gi = static_cast<int>(k);
+    }
+    if (gi < 0) {
+      ntt_gr_batch::Group G;
+      G.precision = prec;
+      G.kind      = E.metric.kind;
+      G.massless  = massless;
+      G.blk0      = { 0 };
+      b->groups.push_back(std::move(G));
+      gi = static_cast<int>(b->groups.size()) - 1;
+    }
+    auto&        G  = b->groups[gi];
+    const size_t sz = prec == 32 ? entry_size<float>() : entry_size<double>();
+    const size_t at = G.host.size();
+    G.host.resize(at + sz);
+    if (prec == 32) {
+      GREntry<float> e;
+      s = make_entry<float>(&E.ctx, g, &E.metric, &e);
+      std::memcpy(G.host.data() + at, &e, sz);
+    } else {
+      GREntry<double> e;
+      s = make_entry<double>(&E.ctx, g, &E.metric, &e);
+      std::memcpy(G.host.data() + at, &e, sz);
+    }
+    const int64_t nb = (g.P.n + 255) / 256;
+    b->where[i]   = { gi, static_cast<int>(G.entry_of.size()) };
+    b->planned[i] = g.P.n;
+    G.entry_of.push_back(i);
+    G.blk0.push_back(G.blk0.back() + nb);
+    if (G.blk0.back() > 0x7FFFFFFF) s = NTT_ERR_UNSUPPORTED;
+  }
+  for (auto& G : b->groups) {
+    if (s != NTT_OK) break;
+    if (hipMalloc(&G.d_table, G.host.size()) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&G.d_blk0), G.blk0.size() * sizeof(int64_t)) !=
+          hipSuccess ||
+        hipMemcpy(G.d_table, G.host.data(), G.host.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(G.d_blk0, G.blk0.data(), G.blk0.size() * sizeof(int64_t),
+                  hipMemcpyHostToDevice) != hipSuccess)
+      s = NTT_ERR_HIP;
+  }
+  if (s != NTT_OK) {
+    free_batch(b);
+    return s;
+  }
+  *out = b;
+  return NTT_OK;
+}
+
+extern "C" ntt_status ntt_gr_batch_push(ntt_gr_batch_t b, void* stream) {
+  if (b == nullptr) return NTT_ERR_ARG;
+  hipStream_t st = as_stream(stream);
+  for (const auto& G : b->groups) {
+    const int     nent = static_cast<int>(G.entry_of.size());
+    const int64_t nb   = G.blk0.back();
+    if (nent == 0 || nb == 0) continue;
+    if (G.precision == 32) {
+      if (G.massless) launch_batched_kind<float, true>(G.kind, b->order, G.d_table, G.d_blk0, nent, nb, st);
+      else launch_batched_kind<float, false>(G.kind, b->order, G.d_table, G.d_blk0, nent, nb, st);
+    } else {
+      if (G.massless) launch_batched_kind<double, true>(G.kind, b->order, G.d_table, G.d_blk0, nent, nb, st);
+      else launch_batched_kind<double, false>(G.kind, b->order, G.d_table, G.d_blk0, nent, nb, st);
+    }
+  }
+  return check_launch();
+}
+
+extern "C" ntt_status ntt_gr_batch_set_npart(ntt_gr_batch_t b, int32_t entry, int64_t npart,
+                                             void* stream) {
+  if (b == nullptr || entry < 0 || entry >= static_cast<int32_t>(b->where.size()) || npart < 0)
+    return NTT_ERR_ARG;
+  const auto [gi, ti] = b->where[entry];
+  if (gi < 0) return npart == 0 ? NTT_OK : NTT_ERR_ARG;
+  if (npart > b->planned[entry]) return NTT_ERR_CAPACITY;
+  auto&        G  = b->groups[gi];
+  const size_t sz = G.precision == 32 ? entry_size<float>() : entry_size<double>();
+  const size_t off = static_cast<size_t>(ti) * sz +
+                     (G.precision == 32 ? offsetof(GREntry<float>, g) : offsetof(GREntry<double>, g)) +
+                     offsetof(GRArgs, P) + offsetof(ParticleView, n);
+  // the host copy stays current, so the copy source outlives the stream
+  std::memcpy(G.host.data() + off, &npart, sizeof(npart));
+  if (hipMemcpyAsync(static_cast<char*>(G.d_table) + off, G.host.data() + off, sizeof(npart),
+                     hipMemcpyHostToDevice, as_stream(stream)) != hipSuccess)
+    return NTT_ERR_HIP;
+  return NTT_OK;
+}
+
+extern "C" ntt_status ntt_gr_batch_destroy(ntt_gr_batch_t b) {
+  if (b == nullptr) return NTT_OK;
+  (void)hipDeviceSynchronize();
+  free_batch(b);
+  return NTT_OK;
+}
+
+extern "C" ntt_status ntt_gr_push(const ntt_gr_pusher_ctx_t* ctx, const ntt_prtl_bc_t* bc,
+                                  ntt_particles_t* prtls, const ntt_field_t* em,
+                                  const ntt_field_t* em0, const ntt_ks_metric_t* metric,
+                                  int32_t shape_order, void* stream) {
+  GRArgs     g;
+  bool       massless = false, skip = false;
+  ntt_status s = prepare_gr(ctx, bc, prtls, em, em0, metric, shape_order, &g, &massless, &skip);
+  if (s != NTT_OK || skip) return s;
   hipStream_t st = as_stream(stream);
   if (ctx->precision == 32) return run_gr<float>(ctx, g, metric, shape_order, massless, st);
   return run_gr<double>(ctx, g, metric, shape_order, massless, st);
