@@ -36,6 +36,10 @@ PRTL_BC_HORIZON = 6
 PRTL_BC_AXIS = 7
 
 FLDS_BC_PERIODIC = 1
+FLDS_BC_MATCH = 2
+FLDS_BC_FIXED = 3
+FLDS_BC_HORIZON = 6
+FLDS_BC_AXIS = 7
 FLDS_BC_CONDUCTOR = 8
 
 _i32p = C.POINTER(C.c_int32)
@@ -130,6 +134,11 @@ class NttGrPusherCtx(C.Structure):
                 ("pusher_flags", C.c_uint8), ("precision", C.c_uint8)]
 
 
+class NttGrBatchEntry(C.Structure):
+    _fields_ = [("ctx", NttGrPusherCtx), ("bc", NttPrtlBC), ("prtls", NttParticles),
+                ("em", NttField), ("em0", NttField), ("metric", NttKsMetric)]
+
+
 class NttParticleAdjoint(C.Structure):
     _fields_ = [("lx1", C.c_void_p), ("lx2", C.c_void_p), ("lu1", C.c_void_p),
                 ("lu2", C.c_void_p), ("lu3", C.c_void_p), ("size", C.c_int64)]
@@ -172,6 +181,23 @@ SIGNATURES: dict[str, tuple] = {
                                       _P(NttParticleAdjoint), C.c_void_p, C.c_void_p]),
     "ntt_yee_vjp": (C.c_int, [C.c_int32, _P(C.c_int32), C.c_float, C.c_float, C.c_int32,
                               C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ntt_gr_batch_create": (C.c_int, [_P(NttGrBatchEntry), C.c_int32, C.c_int32,
+                                      _P(C.c_void_p)]),
+    "ntt_gr_batch_push": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "ntt_gr_batch_set_npart": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p]),
+    "ntt_gr_batch_destroy": (C.c_int, [C.c_void_p]),
+    "ntt_gr_aux_e": (C.c_int, [_P(NttField), _P(NttField), _P(NttField), _P(NttKsMetric),
+                               _P(NttFldsBC), C.c_void_p]),
+    "ntt_gr_aux_h": (C.c_int, [_P(NttField), _P(NttField), _P(NttField), _P(NttKsMetric),
+                               _P(NttFldsBC), C.c_void_p]),
+    "ntt_gr_faraday": (C.c_int, [_P(NttField), _P(NttField), _P(NttField), _P(NttKsMetric),
+                                 C.c_float, _P(NttFldsBC), C.c_void_p]),
+    "ntt_gr_ampere": (C.c_int, [_P(NttField), _P(NttField), _P(NttField), _P(NttKsMetric),
+                                C.c_float, _P(NttFldsBC), C.c_void_p]),
+    "ntt_gr_currents_ampere": (C.c_int, [_P(NttField), _P(NttField), _P(NttKsMetric), C.c_float,
+                                         _P(NttFldsBC), C.c_void_p]),
+    "ntt_gr_time_average_db": (C.c_int, [_P(NttField), _P(NttField), C.c_void_p]),
+    "ntt_gr_time_average_j": (C.c_int, [_P(NttField), _P(NttField), C.c_void_p]),
     "ntt_deposit_currents": (C.c_int, [_P(NttParticles), _P(NttField), _P(NttMinkowski),
                                        C.c_float, C.c_float, C.c_int32, C.c_void_p]),
     "ntt_faraday_mink": (C.c_int, [_P(NttField), C.c_float, C.c_float, _P(NttFdCoeffs),

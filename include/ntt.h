@@ -87,6 +87,10 @@ typedef enum ntt_status {
 
 /* FldsBC values (src/global/enums.h:215-226) */
 #define NTT_FLDS_BC_PERIODIC  1u
+#define NTT_FLDS_BC_MATCH     2u
+#define NTT_FLDS_BC_FIXED     3u
+#define NTT_FLDS_BC_HORIZON   6u
+#define NTT_FLDS_BC_AXIS      7u
 #define NTT_FLDS_BC_CONDUCTOR 8u
 
 typedef struct ntt_particles {
@@ -226,6 +230,94 @@ NTT_API ntt_status ntt_gr_push(const ntt_gr_pusher_ctx_t* ctx,
                                const ntt_ks_metric_t*     metric,
                                int32_t                    shape_order,
                                void*                      stream);
+
+/* One species of one run in a batched GR push: the arguments ntt_gr_push
+ * takes, by value (pointers inside are device pointers). */
+typedef struct ntt_gr_batch_entry {
+  ntt_gr_pusher_ctx_t ctx;
+  ntt_prtl_bc_t       bc;
+  ntt_particles_t     prtls;
+  ntt_field_t         em;   /* D (ignored for photons)        */
+  ntt_field_t         em0;  /* B (ignored for photons)        */
+  ntt_ks_metric_t     metric;
+} ntt_gr_batch_entry_t;
+
+typedef struct ntt_gr_batch* ntt_gr_batch_t;
+
+/* Plans the GR push of many independent runs (each a box with its own metric,
+ * spin, grid, fields and species) as one launch per (precision, metric kind,
+ * massless/massive) group -- a single launch when all entries share them.
+ * Every entry is validated as ntt_gr_push validates it; the results are
+ * bit-identical to ntt_gr_push of each entry.  The plan records the entries'
+ * device pointers and particle counts (which bound the launch), and
+ * allocates device memory for its tables.  No reference counterpart: the
+ * reference runs such scans as separate simulations, each launching
+ * kernel::gr::Pusher_kernel per species (src/engines/grpic/particle_pusher.h:26-89). */
+NTT_API ntt_status ntt_gr_batch_create(const ntt_gr_batch_entry_t* entries, int32_t nentries,
+                                       int32_t shape_order, ntt_gr_batch_t* batch);
+
+/* One push of every entry of the plan.  Enqueues kernels only (no
+ * allocation, copy or synchronisation): graph-capturable. */
+NTT_API ntt_status ntt_gr_batch_push(ntt_gr_batch_t batch, void* stream);
+
+/* Sets the particle count of one entry (e.g. after dead particles were
+ * compacted away), at most its count at planning; stream-ordered. */
+NTT_API ntt_status ntt_gr_batch_set_npart(ntt_gr_batch_t batch, int32_t entry, int64_t npart,
+                                          void* stream);
+
+NTT_API ntt_status ntt_gr_batch_destroy(ntt_gr_batch_t batch);
+
+/* ------------------------------------------------ GR field solver (2D) */
+/* Fields of the GR step on the (r, theta) grid of a Kerr-Schild metric, each
+ * 6 components with ghosts: em/em0 hold D (0..2) and B (3..5), aux holds E
+ * (0..2) and H (3..5), all contravariant code components except the
+ * covariant auxiliary E and H; cur/cur0 hold J (3 components).  bc gives the
+ * field boundaries (NTT_FLDS_BC_AXIS at theta flags the polar rows); it may
+ * be NULL (no axis).  The metric is evaluated in FP64, the fields are FP32.
+ * The launch ranges are the reference drivers'
+ * (src/engines/grpic/fieldsolvers.h:53-266). */
+
+/* E = alpha D_cov + beta x B over active cells plus one cell below in r (and
+ * one row past the last theta row on an axis).  Replaces
+ * kernel::gr::ComputeAuxE_kernel (src/kernels/aux_fields_gr.hpp:31-125). */
+NTT_API ntt_status ntt_gr_aux_e(const ntt_field_t* D, const ntt_field_t* B, ntt_field_t* E,
+                                const ntt_ks_metric_t* metric, const ntt_flds_bc_t* bc,
+                                void* stream);
+
+/* H = alpha B_cov - beta x D over the same range as ntt_gr_aux_e (D and B
+ * may be the same array).  Replaces kernel::gr::ComputeAuxH_kernel
+ * (src/kernels/aux_fields_gr.hpp:136-234). */
+NTT_API ntt_status ntt_gr_aux_h(const ntt_field_t* D, const ntt_field_t* B, ntt_field_t* H,
+                                const ntt_ks_metric_t* metric, const ntt_flds_bc_t* bc,
+                                void* stream);
+
+/* Bout = Bin - coeff curl E / sqrt(det h) over active cells (Bin and Bout
+ * may be the same array; B^theta = 0 on an axis row).  Replaces
+ * kernel::gr::Faraday_kernel (src/kernels/faraday_gr.hpp:32-106). */
+NTT_API ntt_status ntt_gr_faraday(const ntt_field_t* Bin, ntt_field_t* Bout, const ntt_field_t* E,
+                                  const ntt_ks_metric_t* metric, float coeff,
+                                  const ntt_flds_bc_t* bc, void* stream);
+
+/* Dout = Din + coeff curl H / sqrt(det h) over active cells plus one row
+ * past the last theta row (Din and Dout may be the same array; on an axis
+ * row D^r takes the value of the row next to it, read before the update).
+ * Replaces kernel::gr::Ampere_kernel (src/kernels/ampere_gr.hpp:33-100). */
+NTT_API ntt_status ntt_gr_ampere(const ntt_field_t* Din, ntt_field_t* Dout, const ntt_field_t* H,
+                                 const ntt_ks_metric_t* metric, float coeff,
+                                 const ntt_flds_bc_t* bc, void* stream);
+
+/* D += coeff J / sqrt(det h) (the polar area on an axis row), over the
+ * range of ntt_gr_ampere.  Replaces kernel::gr::CurrentsAmpere_kernel
+ * (src/kernels/ampere_gr.hpp:105-178). */
+NTT_API ntt_status ntt_gr_currents_ampere(ntt_field_t* D, const ntt_field_t* J,
+                                          const ntt_ks_metric_t* metric, float coeff,
+                                          const ntt_flds_bc_t* bc, void* stream);
+
+/* BD0 = (BD0 + BD) / 2 (6 components) and J = (J + J0) / 2 (3 components)
+ * over active cells.  Replace kernel::gr::TimeAverageDB_kernel and
+ * TimeAverageJ_kernel (src/kernels/aux_fields_gr.hpp:240-303). */
+NTT_API ntt_status ntt_gr_time_average_db(const ntt_field_t* BD, ntt_field_t* BD0, void* stream);
+NTT_API ntt_status ntt_gr_time_average_j(ntt_field_t* J, const ntt_field_t* J0, void* stream);
 
 /* Charge-conserving current deposit, accumulated (atomically) into `cur`
  * including its ghost cells, exactly where the reference writes.
