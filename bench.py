@@ -14,8 +14,8 @@ re-sort every --sort-interval steps, all inside the timed region.
 
 Prints one JSON line.  Also reports the fused push+deposit kernel against the
 HBM roofline (HIP events around every launch in the timed region) and a CPU
-baseline: the oracle (a sequential C++ restatement, not the reference binary)
-on a sub-box of the same workload, on this host.
+baseline: the C++ restatement of the step (not the reference binary) on
+OpenMP threads, on a sub-box of the same workload, on this host.
 """
 from __future__ import annotations
 
@@ -49,8 +49,8 @@ def parse():
     ap.add_argument("--double-buffer", type=int, default=1,
                     help="re-sort out of place into a second set of particle arrays")
     ap.add_argument("--cpu-baseline", type=int, default=1)
-    ap.add_argument("--cpu-cells", type=str, default="64,64,64")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-cells", type=str, default="128,128,64")
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--batched", type=int, default=1, help="also time the 64-box ensemble")
     ap.add_argument("--batched-steps", type=int, default=12)
     ap.add_argument("--gr", type=int, default=1, help="also time the GR pusher runs")
@@ -127,8 +127,20 @@ def measured_traffic(cells, ppc, tile):
     return None
 
 
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(args, cells, order, pusher_flag, nfilt):
-    """The oracle (sequential C++, FP32) on a sub-box of the workload."""
+    """The oracle's C++ restatement (FP32) on a sub-box of the workload, its
+    step's loops on OpenMP threads (OMP_NUM_THREADS of the environment) with
+    per-thread current copies."""
     import numpy as np
     sys.path.insert(0, str(ROOT / "oracle"))
     import ntt_oracle
@@ -161,17 +173,20 @@ def cpu_baseline(args, cells, order, pusher_flag, nfilt):
     prm = dict(dt=dt, correction=1.0, omegaB0=1.0 / 8, q0=1.0 / (2 * ppc * 64.0), B0=1.0 / 8,
                V0=1.0, ppc0=2.0 * ppc, dx=1.0, shape_order=order, n_filters=nfilt,
                fieldsolver_enable=1, deposit_enable=1)
+    threads = ntt_oracle.omp_threads()
     t0 = time.perf_counter()
     for k in range(args.cpu_steps):
         prm["step"] = k
-        rc = ntt_oracle.sr_step(em, cur, buff, species, dim=3, prm=prm, bc=((1, 1),) * 3)
+        rc = ntt_oracle.sr_step(em, cur, buff, species, dim=3, prm=prm, bc=((1, 1),) * 3,
+                                threads=True)
         assert rc == 0, rc
     el = time.perf_counter() - t0
-    return {"value": 2 * npart * args.cpu_steps / el, "unit": "particle-steps/s", "cores": 1,
-            "kind": "port",
+    return {"value": 2 * npart * args.cpu_steps / el, "unit": "particle-steps/s",
+            "cores": threads, "kind": "port", "cpu_model": cpu_model(),
             "sample": f"{args.cpu_steps} full steps of a {n[0]}x{n[1]}x{n[2]} sub-box of the same "
                       f"workload ({2 * npart} particles, order {order}, {nfilt} filter passes), "
-                      f"oracle C++ restatement in FP32, one core, {el:.1f} s"}
+                      f"C++ restatement in FP32 on {threads} OpenMP threads (per-thread current "
+                      f"copies), {el:.1f} s"}
 
 
 def ensemble_specs(sizes=(64, 128, 256, 512), nboxes=64, seed=777):
@@ -595,8 +610,8 @@ def c0_leg(args):
 
 
 def c0_cpu_baseline(eng, n, ppc, drifts):
-    """The oracle (sequential C++, FP32, one core) on 10 steps of the same
-    box, from the GPU engine's state copied to the host."""
+    """The oracle's C++ restatement (FP32, on OpenMP threads) on 20 steps of
+    the same box, from the GPU engine's state copied to the host."""
     import numpy as np
     sys.path.insert(0, str(ROOT / "oracle"))
     import ntt_oracle
@@ -609,18 +624,20 @@ def c0_cpu_baseline(eng, n, ppc, drifts):
     prm = dict(dt=p.dt, correction=p.correction, omegaB0=p.omegaB0, q0=p.q0, B0=p.B0, V0=p.V0,
                ppc0=p.ppc0, dx=p.dx, shape_order=p.shape_order, n_filters=p.n_filters,
                fieldsolver_enable=1, deposit_enable=1)
-    nst = 10
+    nst = 20
+    threads = ntt_oracle.omp_threads()
     t0 = time.perf_counter()
     for k in range(nst):
         prm["step"] = eng.step + k
         rc = ntt_oracle.sr_step(em, cur, buff, species, dim=2, prm=prm,
-                                bc=((1, 1), (1, 1), (0, 0)))
+                                bc=((1, 1), (1, 1), (0, 0)), threads=True)
         assert rc == 0, rc
     el = time.perf_counter() - t0
     npart = sum(int(sp["p"]["_n"]) for sp in species)
-    return {"value": npart * nst / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{nst} full steps of the same 256x256 box ({npart} particles), oracle C++ "
-                      f"restatement in FP32, one core, {el:.1f} s"}
+    return {"value": npart * nst / el, "unit": "particle-steps/s", "cores": threads,
+            "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"{nst} full steps of the same 256x256 box ({npart} particles), C++ "
+                      f"restatement in FP32 on {threads} OpenMP threads, {el:.1f} s"}
 
 def main():
     args = parse()

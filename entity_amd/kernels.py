@@ -16,6 +16,10 @@ Reference functors (paths relative to the Entity source tree):
 * DigitalFilter_kernel       src/kernels/digital_filter.hpp:58-389
 * gr.Pusher_kernel           src/kernels/pushers/gr.hpp:71-865 (metrics
                              src/metrics/kerr_schild*.h, qkerr_schild.h)
+* gr.Faraday_kernel          src/kernels/faraday_gr.hpp:32-106
+* gr.Ampere_kernel, gr.CurrentsAmpere_kernel   src/kernels/ampere_gr.hpp:33-181
+* gr.ComputeAuxE_kernel, gr.ComputeAuxH_kernel, gr.TimeAverageDB_kernel,
+  gr.TimeAverageJ_kernel     src/kernels/aux_fields_gr.hpp:31-306
 """
 from __future__ import annotations
 
@@ -157,6 +161,122 @@ class gr:  # namespace kernel::gr
                                       C.byref(self.metric.abi()), self.shape_order,
                                       _stream(stream))
             _abi.check(st, "ntt_gr_push")
+
+
+    # ---- field solver (src/kernels/faraday_gr.hpp, ampere_gr.hpp,
+    # aux_fields_gr.hpp): 2D fields on the metric's (r, theta) grid,
+    # boundaries a list of FldsBC pairs per direction (AXIS at theta marks
+    # the polar rows)
+    class _Field(_Launch):
+        @staticmethod
+        def _bc(boundaries):
+            if boundaries is None:
+                return None
+            b = _abi.NttFldsBC()
+            for d, (lo, hi) in enumerate(boundaries):
+                b.bc[d][0], b.bc[d][1] = lo, hi
+            return b
+
+        def _call(self, name, *fields, coeff=None, stream=None, with_bc=True):
+            args = [C.byref(field_abi(f, self.shape)) for f in fields]
+            args.append(C.byref(self.metric.abi()))
+            if coeff is not None:
+                args.append(C.c_float(coeff))
+            if with_bc:
+                bc = self._bc(self.boundaries)
+                args.append(C.byref(bc) if bc is not None else None)
+            args.append(_stream(stream))
+            _abi.check(getattr(self.lib, name)(*args), name)
+
+    class ComputeAuxE_kernel(_Field):
+        """kernel::gr::ComputeAuxE_kernel(Df, Bf, Ef, metric)."""
+
+        def __init__(self, Df, Bf, Ef, shape: GridShape, metric, boundaries=None):
+            self.D, self.B, self.E, self.shape = Df, Bf, Ef, shape
+            self.metric, self.boundaries = metric, boundaries
+
+        def __call__(self, stream=None) -> None:
+            self._call("ntt_gr_aux_e", self.D, self.B, self.E, stream=stream)
+
+    class ComputeAuxH_kernel(_Field):
+        """kernel::gr::ComputeAuxH_kernel(Df, Bf, Hf, metric)."""
+
+        def __init__(self, Df, Bf, Hf, shape: GridShape, metric, boundaries=None):
+            self.D, self.B, self.H, self.shape = Df, Bf, Hf, shape
+            self.metric, self.boundaries = metric, boundaries
+
+        def __call__(self, stream=None) -> None:
+            self._call("ntt_gr_aux_h", self.D, self.B, self.H, stream=stream)
+
+    class Faraday_kernel(_Field):
+        """kernel::gr::Faraday_kernel(Bin, Bout, E, metric, coeff, ni2, boundaries)."""
+
+        def __init__(self, Bin, Bout, E, shape: GridShape, metric, coeff: float,
+                     boundaries=None):
+            self.Bin, self.Bout, self.E, self.shape = Bin, Bout, E, shape
+            self.metric, self.coeff, self.boundaries = metric, coeff, boundaries
+
+        def __call__(self, stream=None) -> None:
+            f = [field_abi(t, self.shape) for t in (self.Bin, self.Bout, self.E)]
+            bc = self._bc(self.boundaries)
+            _abi.check(self.lib.ntt_gr_faraday(C.byref(f[0]), C.byref(f[1]), C.byref(f[2]),
+                                               C.byref(self.metric.abi()), self.coeff,
+                                               C.byref(bc) if bc is not None else None,
+                                               _stream(stream)), "ntt_gr_faraday")
+
+    class Ampere_kernel(_Field):
+        """kernel::gr::Ampere_kernel(Din, Dout, H, metric, coeff, ni2, boundaries)."""
+
+        def __init__(self, Din, Dout, H, shape: GridShape, metric, coeff: float,
+                     boundaries=None):
+            self.Din, self.Dout, self.H, self.shape = Din, Dout, H, shape
+            self.metric, self.coeff, self.boundaries = metric, coeff, boundaries
+
+        def __call__(self, stream=None) -> None:
+            f = [field_abi(t, self.shape) for t in (self.Din, self.Dout, self.H)]
+            bc = self._bc(self.boundaries)
+            _abi.check(self.lib.ntt_gr_ampere(C.byref(f[0]), C.byref(f[1]), C.byref(f[2]),
+                                              C.byref(self.metric.abi()), self.coeff,
+                                              C.byref(bc) if bc is not None else None,
+                                              _stream(stream)), "ntt_gr_ampere")
+
+    class CurrentsAmpere_kernel(_Field):
+        """kernel::gr::CurrentsAmpere_kernel(Df, J, metric, coeff, ni2, boundaries)."""
+
+        def __init__(self, Df, J, shape: GridShape, metric, coeff: float, boundaries=None):
+            self.D, self.J, self.shape = Df, J, shape
+            self.metric, self.coeff, self.boundaries = metric, coeff, boundaries
+
+        def __call__(self, stream=None) -> None:
+            f = [field_abi(t, self.shape) for t in (self.D, self.J)]
+            bc = self._bc(self.boundaries)
+            _abi.check(self.lib.ntt_gr_currents_ampere(C.byref(f[0]), C.byref(f[1]),
+                                                       C.byref(self.metric.abi()), self.coeff,
+                                                       C.byref(bc) if bc is not None else None,
+                                                       _stream(stream)),
+                       "ntt_gr_currents_ampere")
+
+    class TimeAverageDB_kernel(_Launch):
+        """kernel::gr::TimeAverageDB_kernel(BDf, BDf0): BDf0 = (BDf0 + BDf) / 2."""
+
+        def __init__(self, BDf, BDf0, shape: GridShape):
+            self.BD, self.BD0, self.shape = BDf, BDf0, shape
+
+        def __call__(self, stream=None) -> None:
+            a, b = field_abi(self.BD, self.shape), field_abi(self.BD0, self.shape)
+            _abi.check(self.lib.ntt_gr_time_average_db(C.byref(a), C.byref(b), _stream(stream)),
+                       "ntt_gr_time_average_db")
+
+    class TimeAverageJ_kernel(_Launch):
+        """kernel::gr::TimeAverageJ_kernel(Jf, Jf0): Jf = (Jf0 + Jf) / 2."""
+
+        def __init__(self, Jf, Jf0, shape: GridShape):
+            self.J, self.J0, self.shape = Jf, Jf0, shape
+
+        def __call__(self, stream=None) -> None:
+            a, b = field_abi(self.J, self.shape), field_abi(self.J0, self.shape)
+            _abi.check(self.lib.ntt_gr_time_average_j(C.byref(a), C.byref(b), _stream(stream)),
+                       "ntt_gr_time_average_j")
 
 
 class DepositCurrents_kernel(_Launch):

@@ -17,12 +17,20 @@
 #include "ntt_oracle.h"
 
 #include <array>
+#include <omp.h>
+
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <vector>
 
 #include "ntt_oracle_common.hpp"
+
+// When set (by the *_omp entry points only), the step's loops run with OpenMP
+// threads and the deposit goes to per-thread copies of J summed afterwards,
+// as Kokkos' OpenMP ScatterView does.  The checker entry points leave it off
+// and keep the sequential order.
+static thread_local bool t_par = false;
 
 namespace {
 
@@ -348,6 +356,7 @@ int sr_push(const ort_sr_params_t* prm, ort_particles_t* prtls, const ort_field_
   Prt<R> P(prtls);
   if (prm->shape_order < 0 || prm->shape_order > 2) return 8;
   int err = 0;
+#pragma omp parallel for schedule(static) reduction(| : err) if (t_par)
   for (int64_t p = 0; p < P.n; ++p) err |= sr_push_one<R>(prm, EB, P, p);
   return err;
 }
@@ -549,6 +558,56 @@ void deposit_esirkepov_3d(JAcc<R>& J, int G, const Prt<R>& P, int64_t p, R coeff
       for (int k = 0; k < d3; ++k) J.add(i1_min + i, i2_min + j, i3_min + k, 2, jx3[i][j][k]);
 }
 
+// One particle's deposit into the accumulator (the loop body of deposit()).
+template <typename R>
+void deposit_one(JAcc<R>& acc, const Prt<R>& P, int64_t p, int D, int G, R charge, R inv_dt,
+                 R dxm, int order) {
+  using C = K<R>;
+  R vp[3];
+  const R u[3] = { P.ux1[p], P.ux2[p], P.ux3[p] };
+  for (int d = 0; d < 3; ++d) vp[d] = (d < D) ? u[d] / dxm : u[d];
+  const R inv_energy = C::ONE / std::sqrt(C::ONE + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+  if (std::isnan(vp[2]) || std::isinf(vp[2])) vp[2] = C::ZERO;
+  vp[0] *= inv_energy; vp[1] *= inv_energy; vp[2] *= inv_energy;
+  const R coeff = P.w[p] * charge;
+  if (order == 0) deposit_zigzag<R>(acc, D, G, P, p, vp, coeff, inv_dt);
+  else if (D == 2 && order == 1) deposit_esirkepov_2d<R, 1>(acc, G, P, p, vp, coeff, inv_dt);
+  else if (D == 2 && order == 2) deposit_esirkepov_2d<R, 2>(acc, G, P, p, vp, coeff, inv_dt);
+  else if (D == 3 && order == 1) deposit_esirkepov_3d<R, 1>(acc, G, P, p, coeff, inv_dt);
+  else deposit_esirkepov_3d<R, 2>(acc, G, P, p, coeff, inv_dt);
+}
+
+// The deposit over OpenMP threads: each thread adds its share of the
+// particles into a private copy of J, and the copies are summed into J.
+template <typename R>
+int deposit_parallel(const Prt<R>& P, Fld<R>& J, R charge, R inv_dt, R dxm, int order) {
+  const int     D = J.dim, G = J.ngh;
+  const int64_t size = J.size();
+  int           err  = 0;
+  std::vector<std::vector<R>> copies;
+#pragma omp parallel reduction(| : err)
+  {
+#pragma omp single
+    copies.resize(static_cast<size_t>(omp_get_num_threads()));
+    const int t = omp_get_thread_num();
+    copies[t].assign(static_cast<size_t>(size), R(0));
+    Fld<R> Jt = J;
+    Jt.d = copies[t].data();
+    JAcc<R> acc{ &Jt };
+#pragma omp for schedule(static)
+    for (int64_t p = 0; p < P.n; ++p)
+      if (P.tag[p] != TAG_DEAD) deposit_one<R>(acc, P, p, D, G, charge, inv_dt, dxm, order);
+    err |= acc.err;
+#pragma omp for schedule(static)
+    for (int64_t k = 0; k < size; ++k) {
+      R v = J.d[k];
+      for (const auto& c : copies) v += c[k];
+      J.d[k] = v;
+    }
+  }
+  return err;
+}
+
 template <typename R>
 int deposit(const ort_particles_t* prtls, ort_field_t* cur, double charge_d, double dt_d,
             double dx_d, int order) {
@@ -561,6 +620,7 @@ int deposit(const ort_particles_t* prtls, ort_field_t* cur, double charge_d, dou
   const R inv_dt = C::ONE / static_cast<R>(dt_d);
   const R dxm    = static_cast<R>(dx_d);
   const int D = J.dim, G = J.ngh;
+  if (t_par) return deposit_parallel<R>(P, J, charge, inv_dt, dxm, order);
   for (int64_t p = 0; p < P.n; ++p) {
     if (P.tag[p] == TAG_DEAD) continue;
     // velocity for the out-of-plane component (currents_deposit.hpp:112-166)
@@ -594,6 +654,7 @@ int faraday(ort_field_t* em, double c1d, double c2d, const ort_fd_coeffs_t* fd) 
   if (EB.dim == 2) {
     const R alphax = C::ONE - C::TWO * betaxy - C::THREE * deltax;
     const R alphay = C::ONE - C::TWO * betayx - C::THREE * deltay;
+#pragma omp parallel for schedule(static) if (t_par)
     for (int64_t i2 = G; i2 < G + EB.n[1]; ++i2)
       for (int64_t i1 = G; i1 < G + EB.n[0]; ++i1) {
         auto E = [&](int64_t a, int64_t b, int c) -> R { return EB(a, b, c); };
@@ -618,6 +679,7 @@ int faraday(ort_field_t* em, double c1d, double c2d, const ort_fd_coeffs_t* fd) 
     const R alphax = C::ONE - C::TWO * betaxy - C::TWO * betaxz - C::THREE * deltax;
     const R alphay = C::ONE - C::TWO * betayx - C::TWO * betayz - C::THREE * deltay;
     const R alphaz = C::ONE - C::TWO * betazx - C::TWO * betazy - C::THREE * deltaz;
+#pragma omp parallel for schedule(static) if (t_par)
     for (int64_t i3 = G; i3 < G + EB.n[2]; ++i3)
       for (int64_t i2 = G; i2 < G + EB.n[1]; ++i2)
         for (int64_t i1 = G; i1 < G + EB.n[0]; ++i1) {
@@ -675,6 +737,7 @@ int ampere(ort_field_t* em, double c1d, double c2d) {
   const R coeff1 = c1d, coeff2 = c2d;
   const int G = EB.ngh;
   if (EB.dim == 2) {
+#pragma omp parallel for schedule(static) if (t_par)
     for (int64_t i2 = G; i2 < G + EB.n[1]; ++i2)
       for (int64_t i1 = G; i1 < G + EB.n[0]; ++i1) {
         EB(i1, i2, 0) += coeff1 * (EB(i1, i2, 5) - EB(i1, i2 - 1, 5));
@@ -683,6 +746,7 @@ int ampere(ort_field_t* em, double c1d, double c2d) {
                                    EB(i1 - 1, i2, 4));
       }
   } else if (EB.dim == 3) {
+#pragma omp parallel for schedule(static) if (t_par)
     for (int64_t i3 = G; i3 < G + EB.n[2]; ++i3)
       for (int64_t i2 = G; i2 < G + EB.n[1]; ++i2)
         for (int64_t i1 = G; i1 < G + EB.n[0]; ++i1) {
@@ -707,10 +771,12 @@ int currents_ampere(ort_field_t* em, ort_field_t* cur, double coeff_d, double pp
   const int G = E.ngh;
   const int64_t n3 = E.dim >= 3 ? E.n[2] : 1;
   const int64_t k0 = E.dim >= 3 ? G : 0;
+#pragma omp parallel for schedule(static) if (t_par)
   for (int64_t i3 = k0; i3 < k0 + n3; ++i3)
     for (int64_t i2 = G; i2 < G + E.n[1]; ++i2)
       for (int64_t i1 = G; i1 < G + E.n[0]; ++i1)
         for (int c = 0; c < 3; ++c) E.at(i1, i2, i3, c) += J.at(i1, i2, i3, c) * coeff;
+#pragma omp parallel for schedule(static) if (t_par)
   for (int64_t i3 = k0; i3 < k0 + n3; ++i3)
     for (int64_t i2 = G; i2 < G + E.n[1]; ++i2)
       for (int64_t i1 = G; i1 < G + E.n[0]; ++i1)
@@ -727,6 +793,7 @@ int filter(ort_field_t* arr, const ort_field_t* buf) {
   const int G = A.ngh;
   if (A.dim == 2) {
     for (int c = 0; c < 3; ++c)
+#pragma omp parallel for schedule(static) if (t_par)
       for (int64_t i2 = G; i2 < G + A.n[1]; ++i2)
         for (int64_t i1 = G; i1 < G + A.n[0]; ++i1)
           A(i1, i2, c) = C::INV_4 * B(i1, i2, c) +
@@ -736,6 +803,7 @@ int filter(ort_field_t* arr, const ort_field_t* buf) {
                                       B(i1 - 1, i2 + 1, c) + B(i1 + 1, i2 - 1, c));
   } else if (A.dim == 3) {
     for (int c = 0; c < 3; ++c)
+#pragma omp parallel for schedule(static) if (t_par)
       for (int64_t k = G; k < G + A.n[2]; ++k)
         for (int64_t j = G; j < G + A.n[1]; ++j)
           for (int64_t i = G; i < G + A.n[0]; ++i) {
@@ -774,6 +842,7 @@ int communicate(ort_field_t* fld, int c0, int c1) {
   };
   const int64_t N3 = D >= 3 ? F.N3 : 1;
   for (int c = c0; c < c1; ++c)
+#pragma omp parallel for schedule(static) if (t_par)
     for (int64_t k = 0; k < N3; ++k)
       for (int64_t j = 0; j < F.N2; ++j)
         for (int64_t i = 0; i < F.N1; ++i) {
@@ -818,6 +887,7 @@ int synchronize(ort_field_t* cur, ort_field_t* buff) {
       lo[d] = r0; hi[d] = r1; off[d] = r0 - s0;
     }
     for (int c = 0; c < 3; ++c)
+#pragma omp parallel for schedule(static) if (t_par)
       for (int64_t k = lo[2]; k < hi[2]; ++k)
         for (int64_t j = lo[1]; j < hi[1]; ++j)
           for (int64_t i = lo[0]; i < hi[0]; ++i)
@@ -825,6 +895,7 @@ int synchronize(ort_field_t* cur, ort_field_t* buff) {
   }
   const int64_t k0 = D >= 3 ? G : 0, k1 = D >= 3 ? G + J.n[2] : 1;
   for (int c = 0; c < 3; ++c)
+#pragma omp parallel for schedule(static) if (t_par)
     for (int64_t k = k0; k < k1; ++k)
       for (int64_t j = G; j < G + J.n[1]; ++j)
         for (int64_t i = G; i < G + J.n[0]; ++i) J.at(i, j, k, c) += B.at(i, j, k, c);
@@ -950,3 +1021,17 @@ int sr_step(ort_field_t* em, ort_field_t* cur, ort_field_t* buff, ort_species_t*
 
 ORT_DEFINE(f32, float)
 ORT_DEFINE(f64, double)
+
+// The step with OpenMP threads (the CPU baseline; results differ from the
+// sequential step in the current's summation order only).
+extern "C" int ntt_oracle_sr_step_omp_f32(ort_field_t* em, ort_field_t* cur, ort_field_t* buff,
+                                          ort_species_t* species, int32_t nspecies,
+                                          const ort_step_params_t* prm,
+                                          const uint8_t prtl_bc[3][2]) {
+  t_par = true;
+  const int r = sr_step<float>(em, cur, buff, species, nspecies, prm, prtl_bc);
+  t_par = false;
+  return r;
+}
+
+extern "C" int ntt_oracle_omp_threads(void) { return omp_get_max_threads(); }

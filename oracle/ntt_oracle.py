@@ -198,9 +198,13 @@ def synchronize(cur, buff, *, dim, prec="f32"):
                  C.byref(field_struct(buff, dim)))
 
 
-def sr_step(em, cur, buff, species: list[dict], *, dim, prm: dict, bc, prec="f32"):
+def sr_step(em, cur, buff, species: list[dict], *, dim, prm: dict, bc, prec="f32",
+            threads=False):
     """One SRPIC step (srpic.hpp:65-188) over normalized species dicts.
-    species entries: {"p": normalized particle dict, "mass", "charge", "pusher"}."""
+    species entries: {"p": normalized particle dict, "mass", "charge", "pusher"}.
+    threads=True (FP32 only) runs the step's loops on OpenMP threads with
+    per-thread current copies (the CPU baseline); the default is the
+    sequential checker."""
     arr = (OrtSpecies * len(species))()
     for k, sp in enumerate(species):
         arr[k].prtls = particles_struct(sp["p"])
@@ -214,9 +218,18 @@ def sr_step(em, cur, buff, species: list[dict], *, dim, prm: dict, bc, prec="f32
     bcs = ((C.c_uint8 * 2) * 3)()
     for d in range(3):
         bcs[d][0], bcs[d][1] = bc[d]
-    return _call("sr_step", prec, C.byref(field_struct(em, dim)), C.byref(field_struct(cur, dim)),
-                 C.byref(field_struct(buff, dim)), arr, C.c_int32(len(species)), C.byref(sp_),
-                 bcs)
+    args = (C.byref(field_struct(em, dim)), C.byref(field_struct(cur, dim)),
+            C.byref(field_struct(buff, dim)), arr, C.c_int32(len(species)), C.byref(sp_), bcs)
+    if threads:
+        if prec != "f32":
+            raise ValueError("the threaded step is FP32")
+        return lib().ntt_oracle_sr_step_omp_f32(*args)
+    return _call("sr_step", prec, *args)
+
+
+def omp_threads() -> int:
+    """Threads the threaded step uses (omp_get_max_threads)."""
+    return int(lib().ntt_oracle_omp_threads())
 
 
 # ---- GR pusher over the Kerr-Schild metrics ---------------------------------------
@@ -226,7 +239,8 @@ METRIC_KIND = {"kerr_schild": 0, "qkerr_schild": 1, "kerr_schild_0": 2}
 GR_METRIC_QUANTITIES = (
     "h_11", "h_22", "h_33", "h_13", "h11", "h22", "h33", "h13", "alpha", "beta1",
     "dr_alpha", "dt_alpha", "dr_beta1", "dt_beta1", "dr_h11", "dr_h22", "dr_h33", "dr_h13",
-    "dt_h11", "dt_h22", "dt_h33", "dt_h13", "r", "theta")
+    "dt_h11", "dt_h22", "dt_h33", "dt_h13", "r", "theta", "sqrt_det_h", "sqrt_det_h_tilde",
+    "polar_area")
 
 GR_TRANSFORMS = {"D->U": 0, "U->D": 1, "D->T": 2, "T->D": 3, "U->T": 4, "T->U": 5}
 
@@ -294,3 +308,25 @@ def gr_transform(prm: OrtGrParams, which: str, x1, x2, v, *, prec="f64") -> np.n
     assert fn(C.addressof(prm), GR_TRANSFORMS[which], x1.size, _ptr(x1), _ptr(x2),
               _ptr(v), _ptr(out)) == 0
     return out
+
+
+GR_FIELD_OPS = {"aux_e": 0, "aux_h": 1, "faraday": 2, "ampere": 3, "currents_ampere": 4,
+                "time_average_db": 5, "time_average_j": 6}
+
+
+def gr_field(op: str, prm: OrtGrParams, a, b, c=None, *, coeff=0.0, axis=(True, True),
+             mutate=0, prec="f64") -> int:
+    """One GR field-solver operation on 2D fields (arrays (ncomp, N2, N1)):
+    aux_e(D=a, B=b, E=c), aux_h(D=a, B=b, H=c), faraday(Bin=a, Bout=b, E=c),
+    ampere(Din=a, Dout=b, H=c), currents_ampere(D=a, J=b), time_average_db
+    (BD=a, BD0=b: b = (a + b) / 2), time_average_j(J0=a, J=b)."""
+    fa = field_struct(a, 2)
+    fb = field_struct(b, 2)
+    fc = field_struct(c, 2) if c is not None else None
+    fn = getattr(lib(), f"ntt_oracle_gr_field_{prec}")
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                   C.c_int32, C.c_int32, C.c_int32]
+    return fn(GR_FIELD_OPS[op], C.addressof(prm), C.addressof(fa), C.addressof(fb),
+              C.addressof(fc) if fc is not None else None, float(coeff), int(axis[0]),
+              int(axis[1]), int(mutate))

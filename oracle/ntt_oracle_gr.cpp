@@ -11,9 +11,11 @@
  * template on the real type (float: the reference's SINGLE_PRECISION build;
  * double: its double build).
  */
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <limits>
+#include <vector>
 
 #include "ntt_oracle_common.hpp"
 
@@ -206,6 +208,52 @@ struct GRMetric {
     const R th = eta2theta(x[1] * deta + eta_min);
     return (std::exp(-(x[0] * dchi + chi_min)) * dchi_inv) * a /
            Sigma(r0 + std::exp(x[0] * dchi + chi_min), th);
+  }
+  // ---- sqrt(det h), sqrt(det h) / sin(theta), polar area ----
+  // kerr_schild.h:486-532, qkerr_schild.h:553-609, kerr_schild_0.h:346-372
+  R sqrt_det_h_tilde(const R* x) const {
+    if (kind == 0) {
+      const R r = x[0] * dr + x1_min, th = x[1] * dtheta + x2_min;
+      return dr * dtheta * Sigma(r, th) * std::sqrt(K<R>::ONE + z(r, th));
+    }
+    if (kind == 2) return dr * dtheta * sqr(x[0] * dr + x1_min);
+    const R expchi = std::exp(x[0] * dchi + chi_min);
+    const R th     = eta2theta(x[1] * deta + eta_min);
+    return dchi * expchi * dtheta_deta(x[1] * deta + eta_min) * deta * Sigma(r0 + expchi, th) *
+           std::sqrt(K<R>::ONE + z(r0 + expchi, th));
+  }
+  R sqrt_det_h(const R* x) const {
+    if (kind == 0) {
+      const R r = x[0] * dr + x1_min, th = x[1] * dtheta + x2_min;
+      return dr * dtheta * Sigma(r, th) * std::sin(th) * std::sqrt(K<R>::ONE + z(r, th));
+    }
+    if (kind == 2) return dr * dtheta * sqr(x[0] * dr + x1_min) * std::sin(x[1] * dtheta + x2_min);
+    const R expchi = std::exp(x[0] * dchi + chi_min);
+    const R th     = eta2theta(x[1] * deta + eta_min);
+    return dchi * expchi * dtheta_deta(x[1] * deta + eta_min) * deta * Sigma(r0 + expchi, th) *
+           std::sin(th) * std::sqrt(K<R>::ONE + z(r0 + expchi, th));
+  }
+  R polar_area(R x1) const {
+    constexpr double SMALL_ANGLE_D = 1e-3;  // numeric.h:102
+    if (kind == 2) return dr * sqr(x1 * dr + x1_min) * (K<R>::ONE - std::cos(K<R>::HALF * dtheta));
+    R dx, r, ang;
+    if (kind == 0) {
+      dx = dr;
+      r  = x1 * dr + x1_min;
+      const bool small = static_cast<double>(K<R>::HALF * dtheta) < SMALL_ANGLE_D;
+      ang = small ? (static_cast<R>(48) - sqr(dtheta)) * sqr(dtheta) / static_cast<R>(384)
+                  : (K<R>::ONE - std::cos(K<R>::HALF * dtheta));
+    } else {
+      const R e = std::exp(x1 * dchi + chi_min);
+      dx = dchi * e;
+      r  = r0 + e;
+      const R dth = eta2theta(K<R>::HALF * deta);
+      const bool small = static_cast<double>(dth) < SMALL_ANGLE_D;
+      ang = small ? (static_cast<R>(48) - sqr(dth)) * sqr(dth) / static_cast<R>(384)
+                  : (K<R>::ONE - std::cos(dth));
+    }
+    const R ra = sqr(r) + sqr(a);
+    return dx * ra * std::sqrt(K<R>::ONE + K<R>::TWO * r / ra) * ang;
   }
   // ---- lapse, shift ----
   R alpha(const R* x) const {
@@ -724,8 +772,169 @@ int gr_push(const ort_gr_params_t* prm, ort_particles_t* prtls, const ort_field_
   return err;
 }
 
+/* ---- GR field solver: src/kernels/faraday_gr.hpp:32-106,
+ * ampere_gr.hpp:33-181, aux_fields_gr.hpp:31-306, with the ranges of
+ * src/engines/grpic/fieldsolvers.h:53-266.  `mutate` plants a mistake (for
+ * the tests that show a check sees it): 1 drops beta x B from aux E, 2 flips
+ * the sign of the beta x D term of aux H, 3 drops the 1/sqrt(det h) of B^r in
+ * Faraday. */
+enum { GRF_AUX_E = 0, GRF_AUX_H = 1, GRF_FARADAY = 2, GRF_AMPERE = 3, GRF_CUR_AMPERE = 4,
+       GRF_TAVG_DB = 5, GRF_TAVG_J = 6 };
+
+template <typename R>
+int gr_field(int op, const ort_gr_params_t* prm, ort_field_t* fa, ort_field_t* fb,
+             ort_field_t* fc, double coeff_d, int axis_min, int axis_max, int mutate) {
+  GRMetric<R> m(prm);
+  const R     coeff = static_cast<R>(coeff_d);
+  const R     H = K<R>::HALF, ONE = K<R>::ONE, TWO = K<R>::TWO;
+  Fld<R>      A(fa);
+  const int   G = A.ngh, n1 = A.n[0], n2 = A.n[1];
+  auto pt = [](R a_, R b_) { return std::array<R, 2>{ a_, b_ }; };
+  auto sdh  = [&](R a_, R b_) { auto x = pt(a_, b_); return m.sqrt_det_h(x.data()); };
+  auto sdht = [&](R a_, R b_) { auto x = pt(a_, b_); return m.sqrt_det_h_tilde(x.data()); };
+  auto h_11 = [&](R a_, R b_) { auto x = pt(a_, b_); return m.h_11(x.data()); };
+  auto h_22 = [&](R a_, R b_) { auto x = pt(a_, b_); return m.h_22(x.data()); };
+  auto h_33 = [&](R a_, R b_) { auto x = pt(a_, b_); return m.h_33(x.data()); };
+  auto h_13 = [&](R a_, R b_) { auto x = pt(a_, b_); return m.h_13(x.data()); };
+  auto alp  = [&](R a_, R b_) { auto x = pt(a_, b_); return m.alpha(x.data()); };
+  auto bet  = [&](R a_, R b_) { auto x = pt(a_, b_); return m.beta1(x.data()); };
+  const int i2top = G + n2 + (axis_max ? 1 : 0);
+  if (op == GRF_AUX_E) {
+    Fld<R> Df(fa), Bf(fb), Ef(fc);
+    for (int i2 = G; i2 < i2top; ++i2)
+      for (int i1 = G - 1; i1 < G + n1; ++i1) {
+        const R i1_ = static_cast<R>(i1 - G), i2_ = static_cast<R>(i2 - G);
+        R w1 = sdht(i1_ - H, i2_), w2 = sdht(i1_ + H, i2_), ww = TWO * sdht(i1_, i2_);
+        const R D1_half = (w1 * h_13(i1_ - H, i2_) * Df(i1 - 1, i2, 0) +
+                           w2 * h_13(i1_ + H, i2_) * Df(i1, i2, 0)) / ww;
+        const R B2_half = (w1 * sdh(i1_ - H, i2_) * bet(i1_ - H, i2_) * Bf(i1 - 1, i2, 4) +
+                           w2 * sdh(i1_ + H, i2_) * bet(i1_ + H, i2_) * Bf(i1, i2, 4)) / ww;
+        w1 = sdht(i1_ - H, i2_ + H);
+        w2 = sdht(i1_ + H, i2_ + H);
+        ww = TWO * sdht(i1_, i2_ + H);
+        const R B3_half =
+          (w1 * sdh(i1_ - H, i2_ + H) * bet(i1_ - H, i2_ + H) * Bf(i1 - 1, i2, 5) +
+           w2 * sdh(i1_ + H, i2_ + H) * bet(i1_ + H, i2_ + H) * Bf(i1, i2, 5)) / ww;
+        w1 = sdht(i1_, i2_);
+        w2 = sdht(i1_ + ONE, i2_);
+        ww = TWO * sdht(i1_ + H, i2_);
+        const R D3_half = (w1 * h_13(i1_, i2_) * Df(i1, i2, 2) +
+                           w2 * h_13(i1_ + ONE, i2_) * Df(i1 + 1, i2, 2)) / ww;
+        const R D1_cov = h_11(i1_ + H, i2_) * Df(i1, i2, 0) + D3_half;
+        const R D2_cov = h_22(i1_, i2_ + H) * Df(i1, i2, 1);
+        const R D3_cov = h_33(i1_, i2_) * Df(i1, i2, 2) + D1_half;
+        const R keep = mutate == 1 ? K<R>::ZERO : ONE;
+        Ef(i1, i2, 0) = alp(i1_ + H, i2_) * D1_cov;
+        Ef(i1, i2, 1) = alp(i1_, i2_ + H) * D2_cov - keep * B3_half;
+        Ef(i1, i2, 2) = alp(i1_, i2_) * D3_cov + keep * B2_half;
+      }
+    return 0;
+  }
+  if (op == GRF_AUX_H) {
+    Fld<R> Df(fa), Bf(fb), Hf(fc);
+    for (int i2 = G; i2 < i2top; ++i2)
+      for (int i1 = G - 1; i1 < G + n1; ++i1) {
+        const R i1_ = static_cast<R>(i1 - G), i2_ = static_cast<R>(i2 - G);
+        R w1 = sdht(i1_, i2_ + H), w2 = sdht(i1_ + ONE, i2_ + H), ww = TWO * sdht(i1_ + H, i2_ + H);
+        const R B1_half = (w1 * h_13(i1_, i2_ + H) * Bf(i1, i2, 3) +
+                           w2 * h_13(i1_ + ONE, i2_ + H) * Bf(i1 + 1, i2, 3)) / ww;
+        const R D2_half =
+          (w1 * sdh(i1_, i2_ + H) * bet(i1_, i2_ + H) * Df(i1, i2, 1) +
+           w2 * sdh(i1_ + ONE, i2_ + H) * bet(i1_ + ONE, i2_ + H) * Df(i1 + 1, i2, 1)) / ww;
+        w1 = sdht(i1_, i2_);
+        w2 = sdht(i1_ + ONE, i2_);
+        ww = TWO * sdht(i1_ + H, i2_);
+        const R D3_half = (w1 * sdh(i1_, i2_) * bet(i1_, i2_) * Df(i1, i2, 2) +
+                           w2 * sdh(i1_ + ONE, i2_) * bet(i1_ + ONE, i2_) * Df(i1 + 1, i2, 2)) /
+                          ww;
+        w1 = sdht(i1_ - H, i2_ + H);
+        w2 = sdht(i1_ + H, i2_ + H);
+        ww = TWO * sdht(i1_, i2_ + H);
+        const R B3_half = (w1 * h_13(i1_ - H, i2_ + H) * Bf(i1 - 1, i2, 5) +
+                           w2 * h_13(i1_ + H, i2_ + H) * Bf(i1, i2, 5)) / ww;
+        const R B1_cov = h_11(i1_, i2_ + H) * Bf(i1, i2, 3) + B3_half;
+        const R B2_cov = h_22(i1_ + H, i2_) * Bf(i1, i2, 4);
+        const R B3_cov = h_33(i1_ + H, i2_ + H) * Bf(i1, i2, 5) + B1_half;
+        const R sgn = mutate == 2 ? -ONE : ONE;
+        Hf(i1, i2, 3) = alp(i1_, i2_ + H) * B1_cov;
+        Hf(i1, i2, 4) = alp(i1_ + H, i2_) * B2_cov + sgn * D3_half;
+        Hf(i1, i2, 5) = alp(i1_ + H, i2_ + H) * B3_cov - sgn * D2_half;
+      }
+    return 0;
+  }
+  if (op == GRF_FARADAY) {
+    Fld<R> Bin(fa), Bout(fb), E(fc);
+    for (int i2 = G; i2 < G + n2; ++i2)
+      for (int i1 = G; i1 < G + n1; ++i1) {
+        const R i1_ = static_cast<R>(i1 - G), i2_ = static_cast<R>(i2 - G);
+        const R inv_0p = mutate == 3 ? ONE : ONE / sdh(i1_, i2_ + H);
+        const R inv_pp = ONE / sdh(i1_ + H, i2_ + H);
+        const R b1 = Bin(i1, i2, 3), b2 = Bin(i1, i2, 4), b3 = Bin(i1, i2, 5);
+        Bout(i1, i2, 3) = b1 + coeff * inv_0p * (E(i1, i2, 2) - E(i1, i2 + 1, 2));
+        if (i2 == G && axis_min) Bout(i1, i2, 4) = K<R>::ZERO;
+        else Bout(i1, i2, 4) = b2 + coeff * (ONE / sdh(i1_ + H, i2_)) * (E(i1 + 1, i2, 2) - E(i1, i2, 2));
+        Bout(i1, i2, 5) = b3 + coeff * inv_pp * (E(i1, i2 + 1, 0) - E(i1, i2, 0) + E(i1, i2, 1) -
+                                                 E(i1 + 1, i2, 1));
+      }
+    return 0;
+  }
+  if (op == GRF_AMPERE || op == GRF_CUR_AMPERE) {
+    // rows [G, G + n2]; the axis rows first (they read the row next to them
+    // as it was before the update)
+    std::vector<int> rows;
+    if (axis_min) rows.push_back(G);
+    if (axis_max) rows.push_back(G + n2);
+    for (int i2 = G; i2 <= G + n2; ++i2)
+      if (!((i2 == G && axis_min) || (i2 == G + n2 && axis_max))) rows.push_back(i2);
+    for (int i2 : rows)
+      for (int i1 = G; i1 < G + n1; ++i1) {
+        const R i1_ = static_cast<R>(i1 - G), i2_ = static_cast<R>(i2 - G);
+        const bool at_min = (i2 == G) && axis_min, at_max = (i2 == G + n2) && axis_max;
+        if (op == GRF_AMPERE) {
+          Fld<R> Din(fa), Dout(fb), Hh(fc);
+          if (at_min) {
+            const R d1 = Din(i1, i2 + 1, 0), d2 = Din(i1, i2, 1);
+            Dout(i1, i2, 0) = d1;
+            Dout(i1, i2, 1) = d2 + coeff * (ONE / sdh(i1_, H)) * (Hh(i1 - 1, i2, 5) - Hh(i1, i2, 5));
+          } else if (at_max) {
+            Dout(i1, i2, 0) = Din(i1, i2 - 1, 0);
+          } else {
+            const R d1 = Din(i1, i2, 0), d2 = Din(i1, i2, 1), d3 = Din(i1, i2, 2);
+            Dout(i1, i2, 0) = d1 + coeff * (ONE / sdh(i1_ + H, i2_)) * (Hh(i1, i2, 5) - Hh(i1, i2 - 1, 5));
+            Dout(i1, i2, 1) = d2 + coeff * (ONE / sdh(i1_, i2_ + H)) * (Hh(i1 - 1, i2, 5) - Hh(i1, i2, 5));
+            Dout(i1, i2, 2) = d3 + coeff * (ONE / sdh(i1_, i2_)) *
+                                     ((Hh(i1, i2 - 1, 3) - Hh(i1, i2, 3)) +
+                                      (Hh(i1, i2, 4) - Hh(i1 - 1, i2, 4)));
+          }
+        } else {
+          Fld<R> Df(fa), J(fb);
+          if (at_min) {
+            Df(i1, i2, 0) += J(i1, i2, 0) * H * coeff / m.polar_area(i1_ + H);
+            Df(i1, i2, 1) += J(i1, i2, 1) * coeff * (ONE / sdh(i1_, i2_ + H));
+          } else if (at_max) {
+            Df(i1, i2, 0) += J(i1, i2, 0) * H * coeff / m.polar_area(i1_ + H);
+          } else {
+            Df(i1, i2, 0) += J(i1, i2, 0) * coeff * (ONE / sdh(i1_ + H, i2_));
+            Df(i1, i2, 1) += J(i1, i2, 1) * coeff * (ONE / sdh(i1_, i2_ + H));
+            Df(i1, i2, 2) += J(i1, i2, 2) * coeff * (ONE / sdh(i1_, i2_));
+          }
+        }
+      }
+    return 0;
+  }
+  if (op == GRF_TAVG_DB || op == GRF_TAVG_J) {
+    Fld<R> src(fa), dst(fb);
+    const int nc = op == GRF_TAVG_DB ? 6 : 3;
+    for (int c = 0; c < nc; ++c)
+      for (int i2 = G; i2 < G + n2; ++i2)
+        for (int i1 = G; i1 < G + n1; ++i1) dst(i1, i2, c) = H * (dst(i1, i2, c) + src(i1, i2, c));
+    return 0;
+  }
+  return 1;
+}
+
 // Metric quantities at code coordinates, for the tests: per point, NQ values
-constexpr int NQ = 24;
+constexpr int NQ = 27;
 template <typename R>
 int gr_metric(const ort_gr_params_t* prm, int64_t n, const double* x1, const double* x2,
               double* out) {
@@ -740,6 +949,7 @@ int gr_metric(const ort_gr_params_t* prm, int64_t n, const double* x1, const dou
     o[14] = m.dr_h11(x);   o[15] = m.dr_h22(x);   o[16] = m.dr_h33(x);   o[17] = m.dr_h13(x);
     o[18] = m.dt_h11(x);   o[19] = m.dt_h22(x);   o[20] = m.dt_h33(x);   o[21] = m.dt_h13(x);
     o[22] = m.r_of(x);     o[23] = m.th_of(x);
+    o[24] = m.sqrt_det_h(x); o[25] = m.sqrt_det_h_tilde(x); o[26] = m.polar_area(x[0]);
   }
   return 0;
 }
@@ -781,6 +991,12 @@ int gr_transform(const ort_gr_params_t* prm, int32_t which, int64_t n, const dou
                                             const double* x1, const double* x2,             \
                                             double* out) {                                  \
     return gr_metric<R>(prm, n, x1, x2, out);                                               \
+  }                                                                                         \
+  extern "C" int ntt_oracle_gr_field_##SUF(int32_t op, const ort_gr_params_t* prm,          \
+                                           ort_field_t* fa, ort_field_t* fb,                \
+                                           ort_field_t* fc, double coeff, int32_t axis_min, \
+                                           int32_t axis_max, int32_t mutate) {               \
+    return gr_field<R>(op, prm, fa, fb, fc, coeff, axis_min, axis_max, mutate);             \
   }                                                                                         \
   extern "C" int ntt_oracle_gr_transform_##SUF(const ort_gr_params_t* prm, int32_t which,   \
                                                int64_t n, const double* x1,                 \
