@@ -58,6 +58,10 @@ def parse():
     ap.add_argument("--adjoint", type=int, default=1, help="also time the adjoint (gradient) run")
     ap.add_argument("--autotune", type=int, default=1, help="also run the launch autotuner")
     ap.add_argument("--c0", type=int, default=1, help="also run the 2D 256^2 CPU-runnable case")
+    ap.add_argument("--parity", type=int, default=1,
+                    help="also count the one-step differences from the oracle on a small box")
+    ap.add_argument("--functor", type=int, default=1,
+                    help="also time the kernel-level push and deposit entry points")
     return ap.parse_args()
 
 
@@ -125,6 +129,113 @@ def measured_traffic(cells, ppc, tile):
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def parity_leg(args):
+    """The C1 step's physics (order 2, Vay, 2 filter passes, B0 z plus the
+    random-phase modes, theta = 0.3) on a 32 x 16 x 16 box, one ntt_sr_step
+    against one step of the oracle from the same state: the particles whose
+    cell index or tag differs (all within 4 float ulps of a cell face, which
+    is required), the largest deviations of the others, and of E, B and J."""
+    import numpy as np
+    import torch
+    from entity_amd import _abi
+    from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
+    from entity_amd.kernels import device_errors
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import ntt_oracle
+    n, ppc = (32, 16, 16), args.ppc
+    pusher = _abi.PUSHER_VAY if args.pusher == "vay" else _abi.PUSHER_BORIS
+    prm = SRParams(dim=3, n=n, cfl=0.5, shape_order=args.order, n_filters=args.filters,
+                   ppc0=2.0 * ppc, larmor0=8.0, skindepth0=8.0, sort_interval=4, tile=(8, 4, 4))
+    eng = SRPICEngine(prm, [SpeciesSpec("e-", 1.0, -1.0, pusher),
+                            SpeciesSpec("e+", 1.0, 1.0, pusher)], int(np.prod(n)) * ppc)
+    eng.inject_uniform(ppc, temperature=0.3, seed=2024, position_seed=99)
+    init_fields(eng)
+    parts0 = [sp.to_host() for sp in eng.species]
+    em = eng.fields.em.cpu().numpy().copy()
+    cur = eng.fields.cur.cpu().numpy().copy()
+    eng.step_forward()
+    torch.cuda.synchronize()
+    assert device_errors() == 0
+    p = eng.params
+    sp = [{"p": ntt_oracle.normalize_particles(h, "f32"), "mass": s.mass, "charge": s.charge,
+           "pusher": s.pusher} for s, h in zip(eng.species, parts0)]
+    step = dict(dt=p.dt, correction=p.correction, omegaB0=p.omegaB0, q0=p.q0, B0=p.B0, V0=p.V0,
+                ppc0=p.ppc0, dx=p.dx, shape_order=p.shape_order, n_filters=p.n_filters,
+                fieldsolver_enable=1, deposit_enable=1, step=0)
+    assert ntt_oracle.sr_step(em, cur, np.zeros_like(cur), sp, dim=3, prm=step,
+                              bc=((1, 1),) * 3) == 0
+    eps = np.float32(4) * np.finfo(np.float32).eps
+    mism = off_face = 0
+    dev = {"dx": 0.0, "u": 0.0}
+    for s, ref in zip(eng.species, sp):
+        got, q = s.to_host(), ref["p"]
+        near = np.zeros(len(q["tag"]), bool)
+        for d in (1, 2, 3):
+            for x in (got[f"dx{d}"], q[f"dx{d}"]):
+                near |= (x < eps) | (x > np.float32(1) - eps)
+        diff = np.zeros_like(near)
+        for k in ("i1", "i2", "i3", "tag"):
+            diff |= got[k] != q[k]
+        mism += int(diff.sum())
+        off_face += int((diff & ~near).sum())
+        ok = ~diff
+        for k in ("dx1", "dx2", "dx3"):
+            dev["dx"] = max(dev["dx"], float(np.abs(got[k][ok] - q[k][ok]).max()))
+        for k in ("ux1", "ux2", "ux3"):
+            dev["u"] = max(dev["u"], float((np.abs(got[k][ok] - q[k][ok]) /
+                                            np.maximum(np.abs(q[k][ok]), 1e-3)).max()))
+    rel = {}
+    for name, ref in (("em", em), ("cur", cur)):
+        got = getattr(eng.fields, name).cpu().numpy()
+        rel[name] = float(np.abs(got - ref).max() / np.abs(ref).max())
+    return {"workload": f"3D periodic 32x16x16, e-/e+ {ppc} ppc, the benchmark's physics, one "
+                        "step against the oracle's step (product FP32 build)",
+            "particles": 2 * int(np.prod(n)) * ppc,
+            "cell_or_tag_mismatches_near_face": mism, "cell_or_tag_mismatches_off_face": off_face,
+            "max_abs_dx_deviation": dev["dx"], "max_rel_u_deviation": dev["u"],
+            "max_rel_em_deviation": rel["em"], "max_rel_current_deviation": rel["cur"]}
+
+
+def functor_leg(eng, frac=16):
+    """The kernel-level entry points a driver calls one by one
+    (ntt_sr_push, then ntt_deposit_currents into the current array), timed
+    with HIP events on the first 1/frac of species 0 of the benchmark's box
+    after the timed steps: the functor-level integration path."""
+    import ctypes as C
+    import torch
+    from entity_amd import _abi
+    from entity_amd.kernels import Minkowski, PusherBoundaries, PusherContext
+    sp = eng.species[0]
+    p = eng.params
+    abi = sp.abi()
+    abi.npart = sp.npart // frac
+    n = list(p.n) + [1] * (3 - p.dim)
+    ctx = PusherContext(sp.pusher, sp.mass, sp.charge, p.dt, p.omegaB0, *n).abi()
+    bc = PusherBoundaries().abi()
+    em = eng.fields.abi("em")
+    cur = eng.fields.abi("cur")
+    met = Minkowski(p.dim, p.dx).abi()
+    st = torch.cuda.current_stream().cuda_stream
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    eng.fields.cur.zero_()
+    torch.cuda.synchronize()
+    ev[0].record()
+    _abi.check(eng.lib.ntt_sr_push(C.byref(ctx), C.byref(bc), C.byref(abi), C.byref(em),
+                                   C.byref(met), p.shape_order, st), "ntt_sr_push")
+    ev[1].record()
+    _abi.check(eng.lib.ntt_deposit_currents(C.byref(abi), C.byref(cur), C.byref(met), sp.charge,
+                                            p.dt, p.shape_order, st), "ntt_deposit_currents")
+    ev[2].record()
+    torch.cuda.synchronize()
+    t_push, t_dep = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+    npart = int(abi.npart)
+    return {"workload": f"ntt_sr_push + ntt_deposit_currents (one thread per particle, fields "
+                        f"from HBM, float atomics into cur) on {npart} particles (1/{frac} of "
+                        f"one species of the benchmark's box)",
+            "particles": npart, "push_ms": t_push, "deposit_ms": t_dep,
+            "value": npart / ((t_push + t_dep) * 1e-3), "unit": "particle-steps/s (push + deposit)"}
 
 
 def cpu_model() -> str:
@@ -732,12 +843,21 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(cells, args.ppc, tile) if args.order == 2 and
                        args.pusher == "vay" else None,
+            "traffic_source": "profiles/k_push_deposit_3d_o2_vay_512x256x256_traffic.md "
+                              "(rocprofv3 FETCH_SIZE and WRITE_SIZE of this kernel on this "
+                              "workload, copy-kernel calibrated)",
             "avg_launch_ms": avg_ms,
             "launches": int(nl.value),
             "algorithmic_bytes_per_launch": bytes_launch,
         },
         "device_errors": errs,
     }
+    if args.functor:
+        out["functor_path"] = functor_leg(eng)
+    del eng
+    torch.cuda.empty_cache()
+    if args.parity:
+        out["parity"] = parity_leg(args)
     if args.batched:
         out["batched"] = batched_leg(args)
         # the same scan restricted to the smallest box size, where one box

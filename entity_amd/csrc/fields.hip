@@ -359,6 +359,20 @@ __global__ __launch_bounds__(BX* BY) void k_ampere_currents_periodic(FieldView F
   ampere_currents_periodic_cell(F, J, c.i, c.j, c.k, c1, c2, coeff, ppc0, with_j);
 }
 
+// The 3D binomial stencil as written in digital_filter.hpp:349-378 (the 1/32
+// group lists (0,0,-1) and (0,0,+1) where the edge pair (0,-1,+1), (0,+1,-1)
+// would be), from the 27 source values b[dz + 1][dy + 1][dx + 1], in the
+// reference's operation order.  Every filter kernel evaluates it here, so the
+// compiler's contraction of the sum is the same in all of them.
+__device__ __forceinline__ float filter3d_value(const float (&b)[3][3][3]) {
+  return INV_8 * b[1][1][1] +
+         INV_16 * (b[1][1][0] + b[1][1][2] + b[1][0][1] + b[1][2][1] + b[0][1][1] + b[2][1][1]) +
+         INV_32 * (b[1][0][0] + b[1][2][2] + b[1][2][0] + b[1][0][2] + b[0][0][1] + b[2][2][1] +
+                   b[0][1][1] + b[2][1][1] + b[0][1][0] + b[2][1][2] + b[2][1][0] + b[0][1][2]) +
+         INV_64 * (b[0][0][0] + b[2][2][2] + b[2][2][0] + b[0][0][2] + b[2][0][0] + b[0][2][2] +
+                   b[0][2][0] + b[2][0][2]);
+}
+
 // one filter pass dst = K * src (periodic reads), 2D and 3D
 __device__ __forceinline__ void filter_periodic_cell(const FieldView& Dst, const FieldView& Src,
                                                      int i, int j, int k) {
@@ -372,14 +386,15 @@ __device__ __forceinline__ void filter_periodic_cell(const FieldView& Dst, const
           INV_16 * (b(im, jm) + b(ip, jp) + b(im, jp) + b(ip, jm));
     } else {
       const int km = Z.at(k, -1), kp = Z.at(k, 1);
-      auto b = [&](int x, int y, int z) -> float { return Src(x, y, z, comp); };
-      v = INV_8 * b(i, j, k) +
-          INV_16 * (b(im, j, k) + b(ip, j, k) + b(i, jm, k) + b(i, jp, k) + b(i, j, km) + b(i, j, kp)) +
-          INV_32 * (b(im, jm, k) + b(ip, jp, k) + b(im, jp, k) + b(ip, jm, k) + b(i, jm, km) +
-                    b(i, jp, kp) + b(i, j, km) + b(i, j, kp) + b(im, j, km) + b(ip, j, kp) +
-                    b(im, j, kp) + b(ip, j, km)) +
-          INV_64 * (b(im, jm, km) + b(ip, jp, kp) + b(im, jp, kp) + b(ip, jm, km) + b(im, jm, kp) +
-                    b(ip, jp, km) + b(im, jp, km) + b(ip, jm, kp));
+      const int xs[3] = { im, i, ip }, ys[3] = { jm, j, jp }, zs[3] = { km, k, kp };
+      float b[3][3][3];
+#pragma unroll
+      for (int z = 0; z < 3; ++z)
+#pragma unroll
+        for (int y = 0; y < 3; ++y)
+#pragma unroll
+          for (int x = 0; x < 3; ++x) b[z][y][x] = Src(xs[x], ys[y], zs[z], comp);
+      v = filter3d_value(b);
     }
     store_with_ghosts(Dst, i, j, k, comp, v);
   }
@@ -389,6 +404,64 @@ __global__ __launch_bounds__(BX* BY) void k_filter_periodic(FieldView Dst, Field
   const CellIdx c = active_cell(Dst);
   if (!c.ok) return;
   filter_periodic_cell(Dst, Src, c.i, c.j, c.k);
+}
+
+// The 3D pass of k_filter_periodic, tiled for gfx950: a workgroup owns a
+// 64 x 4 column of cells and marches it through KC planes.  The three source
+// planes a plane's stencil reads (each with a one-cell halo, 66 x 6 x 3
+// components) sit in an LDS ring, so every source value is read from HBM once
+// per workgroup instead of 27 times through the caches, and the 27 reads of
+// each output come from LDS with immediate offsets.  Neighbours are read at
+// their ghost-extended positions: the source's periodic ghosts are filled
+// (the current from the fixed-point accumulator and every filter pass write
+// them).  The expression and its operation order are those of
+// filter_periodic_cell.
+constexpr int FTX = 64, FTY = 4, FKC = 16;
+constexpr int FHX = FTX + 2, FHY = FTY + 2, FPL = FHX * FHY;
+
+__global__ __launch_bounds__(FTX* FTY) void k_filter_periodic_3d(FieldView Dst, FieldView Src) {
+  __shared__ float ring[3][3][FPL];
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * FTX + tx;
+  const int G = Src.ngh;
+  const int i0 = blockIdx.x * FTX, j0 = blockIdx.y * FTY, k0 = blockIdx.z * FKC;
+  const int kend = min(k0 + FKC, Src.n3);
+  // loads the source plane kk (active index, may be -1 or n3: a ghost plane)
+  // with its halo into ring slot `slot`
+  auto load_plane = [&](int kk, int slot) {
+    for (int e = tid; e < 3 * FPL; e += FTX * FTY) {
+      const int c = e / FPL, r = e - c * FPL;
+      const int hy = r / FHX, hx = r - hy * FHX;
+      const int gi = i0 + hx - 1 + G, gj = j0 + hy - 1 + G;
+      float v = ZERO;
+      if (gi < Src.N1 && gj < Src.N2) v = Src(gi, gj, kk + G, c);
+      ring[slot][c][r] = v;
+    }
+  };
+  load_plane(k0 - 1, (k0 + 2) % 3);
+  load_plane(k0, k0 % 3);
+  const int i = i0 + tx, j = j0 + ty;
+  const bool active = i < Src.n1 && j < Src.n2;
+  const int  o = (ty + 1) * FHX + (tx + 1);
+  for (int k = k0; k < kend; ++k) {
+    load_plane(k + 1, (k + 1) % 3);
+    __syncthreads();
+    if (active) {
+      const int sm = (k + 2) % 3, s0 = k % 3, sp = (k + 1) % 3;
+#pragma unroll
+      for (int comp = 0; comp < 3; ++comp) {
+        const float* pl[3] = { ring[sm][comp], ring[s0][comp], ring[sp][comp] };
+        float b[3][3][3];
+#pragma unroll
+        for (int z = 0; z < 3; ++z)
+#pragma unroll
+          for (int y = 0; y < 3; ++y)
+#pragma unroll
+            for (int x = 0; x < 3; ++x) b[z][y][x] = pl[z][o + (y - 1) * FHX + (x - 1)];
+        store_with_ghosts(Dst, i + G, j + G, k + G, comp, filter3d_value(b));
+      }
+    }
+    __syncthreads();
+  }
 }
 
 __global__ void k_fill_zero(float* __restrict__ d, int64_t n) {
@@ -606,6 +679,11 @@ void launch_ampere_currents_periodic(const FieldView& F, const FieldView& J, flo
 }
 
 void launch_filter_periodic(const FieldView& dst, const FieldView& src, hipStream_t st) {
+  if (dst.N3 > 1) {
+    const dim3 grid((dst.n1 + FTX - 1) / FTX, (dst.n2 + FTY - 1) / FTY, (dst.n3 + FKC - 1) / FKC);
+    k_filter_periodic_3d<<<grid, dim3(FTX, FTY), 0, st>>>(dst, src);
+    return;
+  }
   k_filter_periodic<<<active_grid(dst), dim3(BX, BY), 0, st>>>(dst, src);
 }
 
