@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--batched-steps", type=int, default=12)
     ap.add_argument("--gr", type=int, default=1, help="also time the GR pusher runs")
     ap.add_argument("--gr-steps", type=int, default=300)
+    ap.add_argument("--gr-full", type=int, default=1,
+                    help="also time the full GRPIC step (field solver and deposit) on one run")
+    ap.add_argument("--gr-full-steps", type=int, default=20)
     ap.add_argument("--adjoint", type=int, default=1, help="also time the adjoint (gradient) run")
     ap.add_argument("--autotune", type=int, default=1, help="also run the launch autotuner")
     ap.add_argument("--c0", type=int, default=1, help="also run the 2D 256^2 CPU-runnable case")
@@ -537,6 +540,48 @@ def gr_leg(args):
                          "flop_source": "profiles/k_gr_push_qks_768x768_counters.md"}}
 
 
+def gr_full_leg(args):
+    """The full GRPIC step (field solver, push, deposit, one filter pass,
+    boundaries) on one run of the GR workload: 768 x 768 QKS grid, a = 0.9,
+    Wald's field, e-/e+ at 8 ppc per species, horizon fields / particles at
+    r_min, match layer (ds = 5) and absorbing current layer at r_max = 50,
+    axis rows, q0 = 1e-3; the engine's launches run eagerly (the step swaps
+    em / em0 and cur / cur0)."""
+    import torch
+    from entity_amd import _abi
+    from entity_amd.grpic import GRPICEngine, GRSpeciesSpec
+    N, a = (768, 768), 0.9
+    run = gr_run(N, a, seed=4343, cfl=0.5)
+    flds = ((_abi.FLDS_BC_HORIZON, _abi.FLDS_BC_MATCH), (_abi.FLDS_BC_AXIS, _abi.FLDS_BC_AXIS))
+    prtl = ((_abi.PRTL_BC_HORIZON, _abi.PRTL_BC_ABSORB), (_abi.PRTL_BC_AXIS, _abi.PRTL_BC_AXIS))
+    specs = [GRSpeciesSpec(sp.label, sp.mass, sp.charge) for sp in run.species]
+    eng = GRPICEngine(run.metric, specs, 1, dt=run.dt, flds_bc=flds, prtl_bc=prtl,
+                      ds=((0.5, 5.0),), target=run.em.clone(), nfilter=1, q0=1e-3, B0=1.0)
+    eng.species = run.species      # the run's particles, in place
+    eng.em.copy_(run.em)
+    del run
+    npart = sum(sp.npart for sp in eng.species)
+    for _ in range(2):
+        eng.step_forward()
+    torch.cuda.synchronize()
+    K = args.gr_full_steps
+    s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    pushed = 0
+    s0.record()
+    for _ in range(K):
+        pushed += sum(sp.npart for sp in eng.species)
+        eng.step_forward()
+    s1.record()
+    torch.cuda.synchronize()
+    ms = s0.elapsed_time(s1) / K
+    return {"workload": "full GRPIC step, QKS 768x768 (r in [0.9 r_h, 50]), a = 0.9, Wald field, "
+                        "e-/e+ 8 ppc/species (9.4M particles), deposit + 1 filter pass, "
+                        "horizon / match (ds 5) / axis boundaries, q0 = 1e-3, FP64 metric",
+            "particles": npart, "timed_steps": K, "ms_per_step": ms,
+            "value": pushed / (ms * 1e-3 * K), "unit": "particle-steps/s",
+            "cell_updates_per_s": N[0] * N[1] / (ms * 1e-3)}
+
+
 # the adjoint workload: 128 x 128 cells, 4 species x 16 ppc, 40 steps,
 # checkpoint every 8, drift +-u_d on the species with ADJ_SIGN, uniform B_z = b
 ADJ_N, ADJ_PPC, ADJ_STEPS, ADJ_CHECKPOINT = (128, 128), 16, 40, 8
@@ -626,32 +671,40 @@ def adjoint_leg(args):
                                   "b": abs(gb64 - fd_b) / abs(fd_b)}}
 
 
+C0_N, C0_PPC = (256, 256), 16
+C0_DRIFTS = [(0.57735, 0, 0), (0.57735, 0, 0), (-0.57735, 0, 0), (-0.57735, 0, 0)]
+C0_CHARGES = (-1.0, 1.0, -1.0, 1.0)
+
+
+def c0_params(tile=None, sort_interval=4, particles_per_thread=0):
+    from entity_amd.engine import SRParams
+    return SRParams(dim=2, n=C0_N, cfl=0.5, shape_order=1, n_filters=1, ppc0=4.0 * C0_PPC,
+                    larmor0=8.0, skindepth0=8.0, sort_interval=sort_interval, tile=tile,
+                    particles_per_thread=particles_per_thread)
+
+
 def autotune_leg(args):
-    """Autotuner on a 2D periodic 256 x 256 box, e-/e+ 16 ppc per species
-    (2,097,152 particles), theta 0.1, counter-drift u = 0.57735, order 1, Boris,
-    1 filter pass: every candidate (tile shape x re-sort cadence, and the
-    unsorted layout x chunk size) runs 16 timed steps after 32 steps that age
-    the layout; reports the fastest and its gain over 8x8 tiles re-sorted
-    every 4 steps (the layout before tuning)."""
+    """Autotuner on the C0 box (2D periodic 256 x 256, e-/e+ beams u_x =
+    +-0.57735 over theta 0.1, 4 species x 16 ppc, order 1, Boris, 1 filter
+    pass): every candidate (tile shape x re-sort cadence, and the unsorted
+    layout x chunk size) runs 16 timed steps after 32 steps that age the
+    layout; reports the fastest and its gain over 8x8 tiles re-sorted every 4
+    steps.  The C0 leg then runs with the pick."""
     import torch
     from entity_amd.autotune import Candidate, candidates_for, engine_runner, tune
-    from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
-    n, ppc = (256, 256), 16
-    ref = SRPICEngine(SRParams(dim=2, n=n, tile=(8, 8)), [SpeciesSpec("e-", 1.0, -1.0),
-                                                          SpeciesSpec("e+", 1.0, 1.0)],
-                      n[0] * n[1] * ppc)
-    ref.inject_uniform(ppc, temperature=0.1, drifts=[(0.57735, 0, 0), (-0.57735, 0, 0)], seed=11)
-    prts = [sp.to_host() for sp in ref.species]
+    from entity_amd.engine import SpeciesSpec, SRPICEngine
+    n, ppc = C0_N, C0_PPC
+    sp = [SpeciesSpec(f"s{k}", 1.0, q) for k, q in enumerate(C0_CHARGES)]
+    ref = SRPICEngine(c0_params(tile=(8, 8)), sp, n[0] * n[1] * ppc)
+    ref.inject_uniform(ppc, temperature=0.1, drifts=C0_DRIFTS, seed=101)
+    prts = [s_.to_host() for s_ in ref.species]
     del ref
 
     def make(c):
-        prm = SRParams(dim=2, n=n, shape_order=1, n_filters=1, ppc0=2.0 * ppc, larmor0=8.0,
-                       skindepth0=8.0, tile=c.tile, sort_interval=c.sort_interval,
-                       particles_per_thread=c.particles_per_thread)
-        eng = SRPICEngine(prm, [SpeciesSpec("e-", 1.0, -1.0), SpeciesSpec("e+", 1.0, 1.0)],
+        eng = SRPICEngine(c0_params(c.tile, c.sort_interval, c.particles_per_thread), sp,
                           n[0] * n[1] * ppc)
-        for sp, h in zip(eng.species, prts):
-            sp.load(**h)
+        for s_, h in zip(eng.species, prts):
+            s_.load(**h)
         if c.sort_interval:
             eng.sort()
         return eng
@@ -660,8 +713,8 @@ def autotune_leg(args):
     default = Candidate((8, 8), 4)
     t_def = res.ms_per_step.get(default)
     torch.cuda.synchronize()
-    return {"workload": "2D periodic 256x256, e-/e+ 16 ppc per species (2,097,152 particles), "
-                        "order 1, Boris, 1 filter pass",
+    return {"workload": "the C0 box: 2D periodic 256x256, e-/e+ beams, 4 species x 16 ppc "
+                        "(4,194,304 particles), order 1, Boris, 1 filter pass",
             "best": {"tile": list(res.best.tile), "sort_interval": res.best.sort_interval,
                      "particles_per_thread": res.best.particles_per_thread},
             "best_ms_per_step": res.ms_per_step[res.best], "default_ms_per_step": t_def,
@@ -669,24 +722,33 @@ def autotune_leg(args):
             "candidates": len(res.ms_per_step)}
 
 
-def c0_leg(args):
+def c0_leg(args, tuned=None):
     """The reference's CPU-runnable case on the GPU: 2D periodic 256 x 256,
     dx = 1, CFL 0.5, e-/e+ beams at u_x = +-0.57735 (beta 0.5) over
     theta = 0.1, 16 ppc per species x 4 species (4,194,304 particles), zero
-    initial fields, shape order 1, Boris, 1 filter pass, re-sort every 4
-    steps, 100 steps.  Reports particle-steps/s, cell-updates/s and the 2D
-    fused kernel against the HBM roofline (78 B per particle + 36 B per cell
-    per launch), with the oracle timed on the host for 10 steps of the same box."""
+    initial fields, shape order 1, Boris, 1 filter pass, 100 steps, with the
+    autotuner's layout when the autotune leg ran (else the default tile,
+    re-sorted every 4 steps).  Reports particle-steps/s, cell-updates/s and
+    the 2D fused kernel against the HBM roofline (78 B per particle + 36 B
+    per cell per launch), with the oracle timed on the host for 20 steps of
+    the same box."""
     import ctypes as C
     import numpy as np
     import torch
     from entity_amd import _abi
-    from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
-    n, ppc, nsteps = (256, 256), 16, 100
-    drifts = [(0.57735, 0, 0), (0.57735, 0, 0), (-0.57735, 0, 0), (-0.57735, 0, 0)]
-    prm = SRParams(dim=2, n=n, cfl=0.5, shape_order=1, n_filters=1, ppc0=4.0 * ppc, larmor0=8.0,
-                   skindepth0=8.0, sort_interval=4)   # default tile: 16 x 8 at 256^2
-    sp = [SpeciesSpec(f"s{k}", 1.0, q) for k, q in enumerate((-1.0, 1.0, -1.0, 1.0))]
+    from entity_amd.engine import SpeciesSpec, SRPICEngine
+    n, ppc, nsteps = C0_N, C0_PPC, 100
+    drifts = C0_DRIFTS
+    if tuned is not None:
+        b = tuned["best"]
+        prm = c0_params(tuple(b["tile"]) if b["tile"] else None, b["sort_interval"],
+                        b["particles_per_thread"])
+        layout = dict(b, source="autotune leg")
+    else:
+        prm = c0_params()      # default tile: 16 x 8 at 256^2, re-sort every 4
+        layout = {"tile": "default", "sort_interval": 4, "particles_per_thread": 1,
+                  "source": "defaults"}
+    sp = [SpeciesSpec(f"s{k}", 1.0, q) for k, q in enumerate(C0_CHARGES)]
     eng = SRPICEngine(prm, sp, n[0] * n[1] * ppc)
     eng.inject_uniform(ppc, temperature=0.1, drifts=drifts, seed=101)
     for _ in range(4):
@@ -709,7 +771,8 @@ def c0_leg(args):
     achieved = (per * 78 + n[0] * n[1] * 36) / (avg * 1e-3) / 1e9
     out = {"workload": "2D periodic 256x256, e-/e+ beams u_x +-0.57735 over theta 0.1, "
                        "4 species x 16 ppc (4,194,304 particles), order 1, Boris, 1 filter pass, "
-                       "re-sort every 4 steps, 100 steps",
+                       "100 steps",
+           "layout": layout,
            "particles": npart, "steps": nsteps, "value": npart * nsteps / el,
            "unit": "particle-steps/s", "ms_per_step": 1e3 * el / nsteps,
            "cell_updates_per_s": n[0] * n[1] * nsteps / el,
@@ -865,12 +928,16 @@ def main():
         out["batched_small_boxes"] = batched_leg(args, sizes=(64,))
     if args.gr:
         out["gr"] = gr_leg(args)
+        torch.cuda.empty_cache()
+    if args.gr_full:
+        out["gr_full_step"] = gr_full_leg(args)
+        torch.cuda.empty_cache()
     if args.adjoint:
         out["adjoint"] = adjoint_leg(args)
     if args.autotune:
         out["autotune"] = autotune_leg(args)
     if args.c0:
-        out["c0"] = c0_leg(args)
+        out["c0"] = c0_leg(args, out.get("autotune"))
     if args.cpu_baseline:
         ccells = tuple(int(x) for x in args.cpu_cells.split(","))
         out["cpu_baseline"] = cpu_baseline(args, ccells, args.order, pusher, args.filters)

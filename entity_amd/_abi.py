@@ -134,6 +134,14 @@ class NttGrPusherCtx(C.Structure):
                 ("pusher_flags", C.c_uint8), ("precision", C.c_uint8)]
 
 
+BC_D, BC_B = 0x7, 0x38   # NTT_BC_D, NTT_BC_B
+
+
+class NttGrMatch(C.Structure):
+    _fields_ = [("r_edge", C.c_double), ("ds", C.c_double), ("i1_lo", C.c_int32),
+                ("i1_hi", C.c_int32), ("tags", C.c_uint32), ("comps", C.c_uint32)]
+
+
 class NttGrBatchEntry(C.Structure):
     _fields_ = [("ctx", NttGrPusherCtx), ("bc", NttPrtlBC), ("prtls", NttParticles),
                 ("em", NttField), ("em0", NttField), ("metric", NttKsMetric)]
@@ -198,6 +206,15 @@ SIGNATURES: dict[str, tuple] = {
                                          _P(NttFldsBC), C.c_void_p]),
     "ntt_gr_time_average_db": (C.c_int, [_P(NttField), _P(NttField), C.c_void_p]),
     "ntt_gr_time_average_j": (C.c_int, [_P(NttField), _P(NttField), C.c_void_p]),
+    "ntt_gr_match_boundaries": (C.c_int, [_P(NttField), _P(NttField), _P(NttKsMetric),
+                                          _P(NttFldsBC), _P(NttGrMatch), C.c_void_p]),
+    "ntt_gr_horizon_boundaries": (C.c_int, [_P(NttField), C.c_uint32, C.c_int32, C.c_void_p]),
+    "ntt_gr_axis_boundaries": (C.c_int, [_P(NttField), C.c_int32, C.c_uint32, C.c_void_p]),
+    "ntt_gr_absorb_currents": (C.c_int, [_P(NttField), _P(NttKsMetric), C.c_double, C.c_double,
+                                         C.c_int32, C.c_int32, C.c_void_p]),
+    "ntt_gr_digital_filter": (C.c_int, [_P(NttField), _P(NttField), _P(NttFldsBC), C.c_void_p]),
+    "ntt_gr_deposit_currents": (C.c_int, [_P(NttParticles), _P(NttField), _P(NttKsMetric),
+                                          C.c_float, C.c_float, C.c_int32, C.c_void_p]),
     "ntt_deposit_currents": (C.c_int, [_P(NttParticles), _P(NttField), _P(NttMinkowski),
                                        C.c_float, C.c_float, C.c_int32, C.c_void_p]),
     "ntt_faraday_mink": (C.c_int, [_P(NttField), C.c_float, C.c_float, _P(NttFdCoeffs),

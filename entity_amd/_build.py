@@ -28,7 +28,8 @@ LIBDIR = PKG / "lib"
 INCLUDE = ROOT / "include"
 
 HIP_SOURCES = ["common.hip", "fields.hip", "sr_kernels.hip", "push_deposit.hip",
-               "sort.hip", "init.hip", "engine.hip", "gr_pusher.hip", "gr_fields.hip", "adjoint.hip"]
+               "sort.hip", "init.hip", "engine.hip", "gr_pusher.hip", "gr_fields.hip", "gr_bcs.hip",
+               "adjoint.hip"]
 
 ARCH = os.environ.get("NTT_OFFLOAD_ARCH", "gfx950")
 

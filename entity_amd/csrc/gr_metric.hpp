@@ -81,6 +81,11 @@ struct KSMetric {
                               (R(2) * R(3) * sqr(eta) - R(2) * R(3) * static_cast<R>(kPi) * eta +
                                static_cast<R>(kPiSqr)));
   }
+  // convert<1, Cd, Ph>: the radius of code x1
+  __device__ __forceinline__ R r_of(R x1) const {
+    if constexpr (KIND == NTT_METRIC_QKERR_SCHILD) return k.r0 + exp(x1 * k.dchi + k.chi_min);
+    else return x1 * k.dr + k.x1_min;
+  }
   // convert<2, Cd, Ph>
   __device__ __forceinline__ R theta_ph(R x2) const {
     if constexpr (KIND == NTT_METRIC_QKERR_SCHILD) return eta2theta(x2 * k.deta + k.eta_min);

@@ -21,6 +21,7 @@ on that run alone bit for bit.
 from __future__ import annotations
 
 import ctypes as C
+import math
 from dataclasses import dataclass
 
 import torch
@@ -232,3 +233,191 @@ class GRBatch:
         self._check_plan()
         self._graph.replay()
         self._after_push()
+
+
+def match_range(metric: gr.KerrSchild, upper: bool, ds: float, ngh: int = 2):
+    """x1 cells [lo, hi) (ghost-extended) of a match layer of width ds at one r
+    boundary, and its edge radius: the box grpic::MatchFieldsIn builds
+    (src/engines/grpic/fields_bcs.h:41-94), ghosts excluded at its lower end
+    and included at its upper end, through Mesh::ExtentToRange
+    (src/framework/domain/mesh.h:133-203)."""
+    rmin, rmax = metric.r_extent
+    n1 = metric.n[0]
+
+    def x1_of(r):   # convert<1, Ph, Cd>
+        if metric.kind == _abi.METRIC_QKERR_SCHILD:
+            c0 = math.log(rmin - metric.r0)
+            return (math.log(r - metric.r0) - c0) / ((math.log(rmax - metric.r0) - c0) / n1)
+        return (r - rmin) / ((rmax - rmin) / n1)
+
+    lo_r, hi_r, edge = (rmax - ds, rmax, rmax) if upper else (rmin, rmin + ds, rmin)
+    lo_r = min(max(rmin, lo_r), rmax)
+    hi_r = max(min(rmax, hi_r), rmin)
+    lo = max(math.floor(x1_of(lo_r)), 0) + ngh
+    hi = min(math.ceil(x1_of(hi_r)) + 2 * ngh, n1 + 2 * ngh)
+    return lo, hi, edge
+
+
+class GRPICEngine(GRRun):
+    """The full GRPIC step of one 2D (r, theta) box: field solver, pusher,
+    deposit, current filter and boundaries (GRPICEngine::step_forward,
+    src/engines/grpic/grpic.hpp:66-634).
+
+    Fields: em, em0 and aux (D or E in 0..2, B or H in 3..5), cur, cur0, buff.
+    flds_bc / prtl_bc: ((x1 min, x1 max), (x2 min, x2 max)) of _abi.FLDS_BC_* /
+    PRTL_BC_* codes (grid.boundaries.fields / particles; theta sides AXIS).
+    ds: match layer widths per side of x1 (grid.boundaries.match.ds).
+    target: the fields the match layer relaxes to (the problem generator's
+    field at every staggered point), comps: the components it defines.
+    Single domain: the halo exchanges of the reference (CommunicateFields,
+    SynchronizeFields) have no neighbour to exchange with on an r-theta box
+    with these boundaries, so they do nothing."""
+
+    def __init__(self, metric: gr.KerrSchild, species: list, maxnpart: int, *, dt: float,
+                 flds_bc, prtl_bc, ds=((1.0, 1.0),), target=None, comps: int = 0x3F,
+                 nfilter: int = 0, q0: float = 1.0, B0: float = 1.0, fieldsolver: bool = True,
+                 deposit: bool = True, correction: float = 1.0, **kw):
+        pbc = PusherBoundaries([tuple(prtl_bc[0]), tuple(prtl_bc[1]), (0, 0)])
+        super().__init__(metric, species, maxnpart, dt=dt, bc=pbc, **kw)
+        for side in (0, 1):
+            if flds_bc[0][side] not in (_abi.FLDS_BC_MATCH, _abi.FLDS_BC_HORIZON):
+                raise ValueError("r boundaries of the fields: MATCH or HORIZON")
+            if prtl_bc[0][side] not in (_abi.PRTL_BC_ABSORB, _abi.PRTL_BC_HORIZON):
+                raise ValueError("r boundaries of the particles: ABSORB or HORIZON")
+            if flds_bc[1][side] != _abi.FLDS_BC_AXIS:
+                raise ValueError("theta boundaries of the fields: AXIS")
+        if deposit and prtl_bc[0][0] == _abi.PRTL_BC_ABSORB:
+            # grpic/fields_bcs.h:58-60
+            raise ValueError("absorption of currents only possible in +x1 (+r)")
+        self.flds_bc = (tuple(flds_bc[0]), tuple(flds_bc[1]), (0, 0))
+        self.prtl_bc = prtl_bc
+        self.ds = ds
+        self.aux = torch.zeros_like(self.em)
+        self.cur = torch.zeros(self.shape.tensor_shape(3), device=self.device)
+        self.cur0 = torch.zeros_like(self.cur)
+        self.buff = torch.zeros_like(self.cur)
+        self.target = target if target is not None else torch.zeros_like(self.em)
+        self.comps, self.nfilter = comps, nfilter
+        self.fieldsolver, self.deposit = fieldsolver, deposit
+        self.dT = correction * dt
+        self.coeff_j = -dt * q0 / B0                    # grpic/fieldsolvers.h:237-240
+        self._layers = {side: match_range(metric, side == 1, ds[0][side]) for side in (0, 1)}
+
+    def _bcs(self):
+        return [self.flds_bc[0], self.flds_bc[1]]
+
+    # ---- grpic/fields_bcs.h:234-266 ----
+    def boundaries(self, tags: int, kind: str = "main", stream=None) -> None:
+        if kind == "aux":       # the horizon acts on main fields only
+            return
+        if kind == "curr":
+            if self.prtl_bc[0][1] == _abi.PRTL_BC_ABSORB:
+                lo, hi, edge = self._layers[1]
+                gr.AbsorbCurrents_kernel(self.cur0, self.shape, self.metric, edge,
+                                         self.ds[0][1], (lo, hi))(stream)
+            return
+        for d, side in ((0, 0), (1, 0), (1, 1), (0, 1)):   # Directions<2D>::orth
+            bc = self.flds_bc[d][side]
+            for F in (self.em, self.em0):
+                if bc == _abi.FLDS_BC_MATCH:
+                    lo, hi, edge = self._layers[side]
+                    gr.MatchBoundaries_kernel(F, self.target, self.shape, self.metric, edge,
+                                              self.ds[0][side], tags, (lo, hi), self.comps,
+                                              self._bcs())(stream)
+                elif bc == _abi.FLDS_BC_AXIS:
+                    gr.AxisBoundaries_kernel(F, self.shape, side == 1, tags)(stream)
+                elif bc == _abi.FLDS_BC_HORIZON:
+                    gr.HorizonBoundaries_kernel(F, self.shape, tags, self.nfilter)(stream)
+
+    def _aux_e(self, D, B, st):
+        gr.ComputeAuxE_kernel(D, B, self.aux, self.shape, self.metric, self._bcs())(st)
+
+    def _aux_h(self, D, B, st):
+        gr.ComputeAuxH_kernel(D, B, self.aux, self.shape, self.metric, self._bcs())(st)
+
+    def _faraday(self, Bin, Bout, coeff, st):
+        gr.Faraday_kernel(Bin, Bout, self.aux, self.shape, self.metric, coeff, self._bcs())(st)
+
+    def _ampere(self, Din, Dout, coeff, st):
+        gr.Ampere_kernel(Din, Dout, self.aux, self.shape, self.metric, coeff, self._bcs())(st)
+
+    def _currents(self, J, st):
+        gr.CurrentsAmpere_kernel(self.em0, J, self.shape, self.metric, self.coeff_j,
+                                 self._bcs())(st)
+
+    def _swap(self):
+        self.em, self.em0 = self.em0, self.em
+        self.cur, self.cur0 = self.cur0, self.cur
+
+    def _init_fields(self, st) -> None:
+        """Step 0 with the field solver on (grpic.hpp:74-280)."""
+        B, D, dT = _abi.BC_B, _abi.BC_D, self.dT
+        self.boundaries(B | D, stream=st)
+        self.em0.copy_(self.em)
+        self._aux_e(self.em, self.em0, st)
+        self._aux_h(self.em, self.em0, st)
+        self._faraday(self.em0, self.em0, 0.5 * dT, st)
+        self.boundaries(B, stream=st)
+        self._ampere(self.em, self.em, 0.5 * dT, st)
+        self.boundaries(D, stream=st)
+        self._aux_e(self.em, self.em0, st)
+        self._aux_h(self.em, self.em0, st)
+        self._faraday(self.em, self.em0, dT, st)
+        self.boundaries(B, stream=st)
+        self._ampere(self.em0, self.em0, dT, st)
+        self.boundaries(D, stream=st)
+        self._aux_h(self.em0, self.em0, st)
+        self._ampere(self.em, self.em0, dT, st)
+        self.boundaries(D, stream=st)
+        self._swap()
+
+    def deposit_currents(self, stream=None) -> None:
+        """grpic::CurrentsDeposit (currents.h:34-77) into cur0, then the
+        absorbing layer and the filter passes (currents.h:80-110)."""
+        self.cur0.zero_()
+        for sp in self.species:
+            if sp.npart == 0 or sp.charge == 0.0:
+                continue
+            gr.DepositCurrents_kernel(self.cur0, self.shape, sp, self.metric, sp.charge,
+                                      self.dt, self.precision)(stream)
+        self.boundaries(0x1C0, "curr", stream)
+        for _ in range(self.nfilter):
+            self.buff.copy_(self.cur0)
+            gr.DigitalFilter_kernel(self.cur0, self.buff, self.shape, self._bcs())(stream)
+
+    def step_forward(self, stream=None) -> None:
+        st, dT = stream, self.dT
+        B, D = _abi.BC_B, _abi.BC_D
+        if self.step == 0:
+            if self.fieldsolver:
+                self._init_fields(st)
+            else:
+                self.em0.copy_(self.em)
+        if self.fieldsolver:
+            gr.TimeAverageDB_kernel(self.em, self.em0, self.shape)(st)
+            self._aux_e(self.em0, self.em, st)
+            self._faraday(self.em0, self.em0, dT, st)
+            self.boundaries(B, stream=st)
+            self._aux_h(self.em, self.em0, st)
+        self.push(st)
+        if self.deposit:
+            self.deposit_currents(st)
+        if self.fieldsolver:
+            if self.deposit:
+                gr.TimeAverageJ_kernel(self.cur, self.cur0, self.shape)(st)
+            self._aux_e(self.em, self.em0, st)
+            self._faraday(self.em, self.em0, dT, st)
+            self.boundaries(B, stream=st)
+            self._ampere(self.em0, self.em0, dT, st)
+            if self.deposit:
+                self._currents(self.cur, st)
+            self.boundaries(D, stream=st)
+            self._aux_h(self.em0, self.em0, st)
+            self._ampere(self.em, self.em0, dT, st)
+            if self.deposit:
+                self._currents(self.cur0, st)
+            self._swap()
+            self.boundaries(D, stream=st)
+        if self.clear_due():
+            self.remove_dead(st)
+        self.step += 1

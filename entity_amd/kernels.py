@@ -279,6 +279,99 @@ class gr:  # namespace kernel::gr
                        "ntt_gr_time_average_j")
 
 
+    # ---- boundaries, current filter and deposit of the GRPIC step
+    # (src/kernels/fields_bcs.hpp, digital_filter.hpp, currents_deposit.hpp)
+    class MatchBoundaries_kernel(_Field):
+        """kernel::bc::MatchBoundaries_kernel<GRPIC, M, FS, in::x1>(Fld, fset,
+        metric, xg_edge, ds, tags, boundaries) over x1 cells i1_range: fset is
+        `target`, the field setter's values at every staggered point, comps the
+        components it defines (bit c for em component c)."""
+
+        def __init__(self, Fld, target, shape: GridShape, metric, r_edge: float, ds: float,
+                     tags: int, i1_range, comps: int = 0x3F, boundaries=None):
+            self.F, self.T, self.shape, self.metric = Fld, target, shape, metric
+            self.m = _abi.NttGrMatch(r_edge, ds, int(i1_range[0]), int(i1_range[1]), tags, comps)
+            self.boundaries = boundaries
+
+        def __call__(self, stream=None) -> None:
+            f, t = field_abi(self.F, self.shape), field_abi(self.T, self.shape)
+            bc = self._bc(self.boundaries)
+            _abi.check(self.lib.ntt_gr_match_boundaries(
+                C.byref(f), C.byref(t), C.byref(self.metric.abi()),
+                C.byref(bc) if bc is not None else None, C.byref(self.m), _stream(stream)),
+                "ntt_gr_match_boundaries")
+
+    class HorizonBoundaries_kernel(_Launch):
+        """kernel::bc::gr::HorizonBoundaries_kernel(Fld, i1_min, tags, nfilter)."""
+
+        def __init__(self, Fld, shape: GridShape, tags: int, nfilter: int):
+            self.F, self.shape, self.tags, self.nfilter = Fld, shape, tags, nfilter
+
+        def __call__(self, stream=None) -> None:
+            f = field_abi(self.F, self.shape)
+            _abi.check(self.lib.ntt_gr_horizon_boundaries(C.byref(f), self.tags, self.nfilter,
+                                                          _stream(stream)),
+                       "ntt_gr_horizon_boundaries")
+
+    class AxisBoundaries_kernel(_Launch):
+        """kernel::bc::AxisBoundaries_kernel<2D, P>(Fld, i_edge, tags); upper = P."""
+
+        def __init__(self, Fld, shape: GridShape, upper: bool, tags: int):
+            self.F, self.shape, self.upper, self.tags = Fld, shape, upper, tags
+
+        def __call__(self, stream=None) -> None:
+            f = field_abi(self.F, self.shape)
+            _abi.check(self.lib.ntt_gr_axis_boundaries(C.byref(f), int(self.upper), self.tags,
+                                                       _stream(stream)),
+                       "ntt_gr_axis_boundaries")
+
+    class AbsorbCurrents_kernel(_Launch):
+        """kernel::bc::gr::AbsorbCurrents_kernel<M, 1>(J, metric, xg_edge, ds)
+        over x1 cells i1_range."""
+
+        def __init__(self, J, shape: GridShape, metric, r_edge: float, ds: float, i1_range):
+            self.J, self.shape, self.metric = J, shape, metric
+            self.r_edge, self.ds, self.range = r_edge, ds, (int(i1_range[0]), int(i1_range[1]))
+
+        def __call__(self, stream=None) -> None:
+            j = field_abi(self.J, self.shape)
+            _abi.check(self.lib.ntt_gr_absorb_currents(C.byref(j), C.byref(self.metric.abi()),
+                                                       self.r_edge, self.ds, self.range[0],
+                                                       self.range[1], _stream(stream)),
+                       "ntt_gr_absorb_currents")
+
+    class DigitalFilter_kernel(_Field):
+        """kernel::DigitalFilter_kernel<2D, spherical>(array, buffer, size, boundaries)."""
+
+        def __init__(self, array, buffer, shape: GridShape, boundaries=None):
+            self.A, self.B, self.shape, self.boundaries = array, buffer, shape, boundaries
+
+        def __call__(self, stream=None) -> None:
+            a, b = field_abi(self.A, self.shape), field_abi(self.B, self.shape)
+            bc = self._bc(self.boundaries)
+            _abi.check(self.lib.ntt_gr_digital_filter(C.byref(a), C.byref(b),
+                                                      C.byref(bc) if bc is not None else None,
+                                                      _stream(stream)), "ntt_gr_digital_filter")
+
+    class DepositCurrents_kernel(_Launch):
+        """kernel::DepositCurrents_kernel<GRPIC, M>(J, particles..., metric,
+        charge, dt): order 1; precision 64 evaluates the metric in FP64."""
+
+        def __init__(self, J, shape: GridShape, particles: Particles, metric, charge: float,
+                     dt: float, precision: int = 64):
+            self.J, self.shape, self.particles, self.metric = J, shape, particles, metric
+            self.charge, self.dt, self.precision = charge, dt, precision
+
+        def __call__(self, stream=None) -> None:
+            p = self.particles.abi()
+            j = field_abi(self.J, self.shape)
+            _abi.check(self.lib.ntt_gr_deposit_currents(C.byref(p), C.byref(j),
+                                                        C.byref(self.metric.abi()), self.charge,
+                                                        self.dt, self.precision,
+                                                        _stream(stream)),
+                       "ntt_gr_deposit_currents")
+
+
 class DepositCurrents_kernel(_Launch):
     def __init__(self, J: torch.Tensor, shape: GridShape, particles: Particles,
                  metric: Minkowski, charge: float, dt: float, shape_order: int = 0,

@@ -319,6 +319,71 @@ NTT_API ntt_status ntt_gr_currents_ampere(ntt_field_t* D, const ntt_field_t* J,
 NTT_API ntt_status ntt_gr_time_average_db(const ntt_field_t* BD, ntt_field_t* BD0, void* stream);
 NTT_API ntt_status ntt_gr_time_average_j(ntt_field_t* J, const ntt_field_t* J0, void* stream);
 
+/* ---- GR field boundaries, current filter and deposit (2D r-theta) -------- */
+
+/* BC tags (src/global/global.h:306-329): D shares the bits of E, H those of B. */
+#define NTT_BC_D 0x7u
+#define NTT_BC_B 0x38u
+
+/* One match layer at an r boundary: its cell range, as
+ * grpic::MatchFieldsIn computes it with Mesh::ExtentToRange
+ * (src/engines/grpic/fields_bcs.h:41-94, src/framework/domain/mesh.h:133-203),
+ * the edge radius and the layer width (grid.boundaries.match.ds). */
+typedef struct ntt_gr_match {
+  double   r_edge, ds;
+  int32_t  i1_lo, i1_hi;  /* x1 cells [lo, hi), ghost-extended indices          */
+  uint32_t tags;          /* NTT_BC_D and/or NTT_BC_B                           */
+  uint32_t comps;         /* components `target` defines: bit c for em component c */
+} ntt_gr_match_t;
+
+/* Blends em towards `target` over the layer, every x2 row (ghosts
+ * included): F = s F + (1 - s) T, s = tanh(4 |r - r_edge| / ds) at each
+ * component's own x1 position; D^3 on and past an axis row is only damped.
+ * `target` holds the problem's field at every component's staggered point
+ * (the values the reference's field setter returns there).  Replaces
+ * kernel::bc::MatchBoundaries_kernel<GRPIC, M, FS, in::x1>
+ * (src/kernels/fields_bcs.hpp:176-345). */
+NTT_API ntt_status ntt_gr_match_boundaries(ntt_field_t* em, const ntt_field_t* target,
+                                           const ntt_ks_metric_t* metric, const ntt_flds_bc_t* bc,
+                                           const ntt_gr_match_t* match, void* stream);
+
+/* Inner (horizon) boundary: x1 rows 0 .. 2 + nfilter take the values of row
+ * 3 + nfilter, for x2 rows [2, 2 + n2] and the tagged components.  Replaces
+ * kernel::bc::gr::HorizonBoundaries_kernel (src/kernels/fields_bcs.hpp:
+ * 1188-1240). */
+NTT_API ntt_status ntt_gr_horizon_boundaries(ntt_field_t* em, uint32_t tags, int32_t nfilter,
+                                             void* stream);
+
+/* Axis rows and their ghosts for every x1 column (upper = 0: theta = 0,
+ * 1: theta = pi).  Replaces kernel::bc::AxisBoundaries_kernel
+ * (src/kernels/fields_bcs.hpp:825-867). */
+NTT_API ntt_status ntt_gr_axis_boundaries(ntt_field_t* em, int32_t upper, uint32_t tags,
+                                          void* stream);
+
+/* J *= tanh(|r(x1) - r_edge| / (ds / 4)) over x1 cells [i1_lo, i1_hi), every
+ * x2 row.  Replaces kernel::bc::gr::AbsorbCurrents_kernel
+ * (src/kernels/fields_bcs.hpp:1243-1280). */
+NTT_API ntt_status ntt_gr_absorb_currents(ntt_field_t* cur, const ntt_ks_metric_t* metric,
+                                          double r_edge, double ds, int32_t i1_lo, int32_t i1_hi,
+                                          void* stream);
+
+/* One pass of the current filter of a spherical grid, buff -> cur, over x1
+ * [2, 2 + n1) and x2 [2, 2 + n2] (axis rows per bc).  Replaces
+ * kernel::DigitalFilter_kernel<2D, non-Cartesian> (src/kernels/
+ * digital_filter.hpp:196-266) as grpic::CurrentsFilter launches it
+ * (src/engines/grpic/currents.h:80-110). */
+NTT_API ntt_status ntt_gr_digital_filter(ntt_field_t* cur, const ntt_field_t* buff,
+                                         const ntt_flds_bc_t* bc, void* stream);
+
+/* The GR current deposit (order 1, whatever the pusher's shape order), added
+ * atomically into `cur` with its ghosts; precision 64 (or 0) evaluates the
+ * metric in FP64, 32 in FP32.  Replaces kernel::DepositCurrents_kernel<GRPIC,
+ * M> (src/kernels/currents_deposit.hpp:108-555) as grpic::CurrentsDeposit
+ * launches it (src/engines/grpic/currents.h:34-77). */
+NTT_API ntt_status ntt_gr_deposit_currents(const ntt_particles_t* prtls, ntt_field_t* cur,
+                                           const ntt_ks_metric_t* metric, float charge, float dt,
+                                           int32_t precision, void* stream);
+
 /* Charge-conserving current deposit, accumulated (atomically) into `cur`
  * including its ghost cells, exactly where the reference writes.
  * Replaces kernel::DepositCurrents_kernel<SRPIC, Minkowski<D>, O>

@@ -640,6 +640,31 @@ int deposit(const ort_particles_t* prtls, ort_field_t* cur, double charge_d, dou
   return acc.err;
 }
 
+// The 2D deposit with the out-of-plane velocity given per particle: the
+// GRPIC instantiation of DepositCurrents_kernel (currents_deposit.hpp:
+// 131-166 compute it from the metric; the rest is the same scheme).
+template <typename R>
+int deposit_vz(const ort_particles_t* prtls, ort_field_t* cur, double charge_d, double dt_d,
+               int order, const double* vz) {
+  using C = K<R>;
+  Fld<R> J(cur);
+  Prt<R> P(prtls);
+  JAcc<R> acc{ &J };
+  if (order < 0 || order > 2 || J.dim != 2) return 8;
+  const R charge = static_cast<R>(charge_d);
+  const R inv_dt = C::ONE / static_cast<R>(dt_d);
+  const int G = J.ngh;
+  for (int64_t p = 0; p < P.n; ++p) {
+    if (P.tag[p] == TAG_DEAD) continue;
+    const R vp[3] = { C::ZERO, C::ZERO, static_cast<R>(vz[p]) };
+    const R coeff = P.w[p] * charge;
+    if (order == 0) deposit_zigzag<R>(acc, 2, G, P, p, vp, coeff, inv_dt);
+    else if (order == 1) deposit_esirkepov_2d<R, 1>(acc, G, P, p, vp, coeff, inv_dt);
+    else deposit_esirkepov_2d<R, 2>(acc, G, P, p, vp, coeff, inv_dt);
+  }
+  return acc.err;
+}
+
 /* ---- Faraday (Minkowski): src/kernels/faraday_mink.hpp:71-166 ---------- */
 template <typename R>
 int faraday(ort_field_t* em, double c1d, double c2d, const ort_fd_coeffs_t* fd) {
@@ -991,6 +1016,11 @@ int sr_step(ort_field_t* em, ort_field_t* cur, ort_field_t* buff, ort_species_t*
                                           double charge, double dt, double dx,              \
                                           int32_t order) {                                  \
     return deposit<R>(prtls, cur, charge, dt, dx, order);                                   \
+  }                                                                                         \
+  extern "C" int ntt_oracle_deposit_vz_##SUF(const ort_particles_t* prtls, ort_field_t* cur,\
+                                             double charge, double dt, int32_t order,       \
+                                             const double* vz) {                            \
+    return deposit_vz<R>(prtls, cur, charge, dt, order, vz);                                \
   }                                                                                         \
   extern "C" int ntt_oracle_faraday_##SUF(ort_field_t* em, double c1, double c2,            \
                                           const ort_fd_coeffs_t* fd) {                      \

@@ -91,6 +91,20 @@ typedef struct ort_gr_params {
   int32_t nx1, nx2;
 } ort_gr_params_t;
 
+/* GR field boundaries and the spherical current filter
+ * (ntt_oracle_gr_bc_*): arguments of one boundary operation */
+typedef struct ort_gr_bc_args {
+  double   xg_edge, ds;     /* match / absorb: edge radius and layer width   */
+  int32_t  i1_lo, i1_hi;    /* match / absorb: x1 cell range [lo, hi), ghost-
+                               extended indices                              */
+  uint32_t tags;            /* bits 0-2: D (E), bits 3-5: B (H)              */
+  uint32_t comps;           /* match: components the target defines, bit c
+                               for em component c                            */
+  int32_t  nfilter;         /* horizon: number of current filter passes      */
+  int32_t  upper;           /* axis: 1 for the theta = pi side               */
+  int32_t  axis_min, axis_max;
+} ort_gr_bc_args_t;
+
 #define ORT_DECL(SUF, R)                                                        \
   int ntt_oracle_sr_push_##SUF(const ort_sr_params_t* prm,                       \
                                ort_particles_t* prtls, const ort_field_t* em);   \

@@ -11,6 +11,7 @@ Arrays are numpy; particle dicts use the reference's SoA names
 from __future__ import annotations
 
 import ctypes as C
+import math
 from pathlib import Path
 
 import numpy as np
@@ -330,3 +331,219 @@ def gr_field(op: str, prm: OrtGrParams, a, b, c=None, *, coeff=0.0, axis=(True, 
     return fn(GR_FIELD_OPS[op], C.addressof(prm), C.addressof(fa), C.addressof(fb),
               C.addressof(fc) if fc is not None else None, float(coeff), int(axis[0]),
               int(axis[1]), int(mutate))
+
+
+# ---- GR field boundaries, the spherical filter, the GR deposit and the GRPIC
+# step (src/engines/grpic/grpic.hpp:66-634) -----------------------------------
+GR_BC_OPS = {"match": 0, "horizon": 1, "axis": 2, "absorb_currents": 3, "filter": 4}
+# the reference's BCTags bits (src/global/global.h:306-329): D shares the bits
+# of E, H those of B
+BC_D, BC_B = 0x7, 0x38
+FLDS_MATCH, FLDS_HORIZON, FLDS_AXIS = 2, 6, 7
+PRTL_ABSORB, PRTL_HORIZON, PRTL_AXIS = 2, 6, 7
+
+
+class OrtGrBcArgs(C.Structure):
+    _fields_ = [("xg_edge", C.c_double), ("ds", C.c_double), ("i1_lo", C.c_int32),
+                ("i1_hi", C.c_int32), ("tags", C.c_uint32), ("comps", C.c_uint32),
+                ("nfilter", C.c_int32), ("upper", C.c_int32), ("axis_min", C.c_int32),
+                ("axis_max", C.c_int32)]
+
+
+def gr_bc(op: str, prm: OrtGrParams, a, b=None, *, xg_edge=0.0, ds=1.0, i1_range=(0, 0),
+          tags=BC_D | BC_B, comps=0x3F, nfilter=0, upper=False, axis=(True, True), mutate=0,
+          prec="f64") -> int:
+    """One GR boundary operation on 2D fields (arrays (ncomp, N2, N1)):
+    match(F=a, target=b), horizon(F=a), axis(F=a), absorb_currents(J=a),
+    filter(array=a, buffer=b)."""
+    g = OrtGrBcArgs(float(xg_edge), float(ds), int(i1_range[0]), int(i1_range[1]), int(tags),
+                    int(comps), int(nfilter), int(upper), int(axis[0]), int(axis[1]))
+    fa = field_struct(a, 2)
+    fb = field_struct(b, 2) if b is not None else None
+    fn = getattr(lib(), f"ntt_oracle_gr_bc_{prec}")
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
+    return fn(GR_BC_OPS[op], C.addressof(prm), C.addressof(fa),
+              C.addressof(fb) if fb is not None else None, C.addressof(g), int(mutate))
+
+
+def gr_x1_to_code(prm: OrtGrParams, r: float) -> float:
+    """convert<1, Ph, Cd> (kerr_schild.h, qkerr_schild.h): radius to code x1."""
+    if prm.metric_kind == METRIC_KIND["qkerr_schild"]:
+        chi_min = math.log(prm.x1_min - prm.r0)
+        dchi = (math.log(prm.x1_max - prm.r0) - chi_min) / prm.nx1
+        return (math.log(r - prm.r0) - chi_min) / dchi
+    return (r - prm.x1_min) / ((prm.x1_max - prm.x1_min) / prm.nx1)
+
+
+def gr_match_range(prm: OrtGrParams, upper: bool, ds: float, G: int = 2):
+    """The x1 cell range of a match layer of width ds at one r boundary:
+    Mesh::ExtentToRange (src/framework/domain/mesh.h:133-203) of the box that
+    MatchFieldsIn builds (src/engines/grpic/fields_bcs.h:41-94), with ghosts
+    excluded below and included above.  Returns (lo, hi), ghost-extended
+    indices, and the edge radius."""
+    n1 = prm.nx1
+    if upper:
+        lo_r, hi_r, edge = prm.x1_max - ds, prm.x1_max, prm.x1_max
+    else:
+        lo_r, hi_r, edge = prm.x1_min, prm.x1_min + ds, prm.x1_min
+    lo_r = min(max(prm.x1_min, lo_r), prm.x1_max)
+    hi_r = max(min(prm.x1_max, hi_r), prm.x1_min)
+    lo = max(math.floor(gr_x1_to_code(prm, lo_r)), 0)
+    hi = math.ceil(gr_x1_to_code(prm, hi_r))
+    return lo + G, min(hi + 2 * G, n1 + 2 * G), edge
+
+
+def gr_deposit(prm: OrtGrParams, p: dict, J, *, charge, dt, prec="f64") -> int:
+    """DepositCurrents_kernel<GRPIC, M> (always order 1,
+    src/engines/grpic/currents.h:34-77): the 2D Esirkepov deposit with the
+    out-of-plane velocity of the metric."""
+    q = p if "_n" in p else normalize_particles(p, prec)
+    s = particles_struct(q)
+    vz = np.zeros(max(q["_n"], 1))
+    fn = getattr(lib(), f"ntt_oracle_gr_deposit_velocity_{prec}")
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    assert fn(C.addressof(prm), C.addressof(s), _ptr(vz)) == 0
+    f = field_struct(J, 2)
+    return _call("deposit_vz", prec, C.byref(s), C.byref(f), C.c_double(charge),
+                 C.c_double(dt), C.c_int32(1), C.c_void_p(_ptr(vz)))
+
+
+class GRStep:
+    """The GRPIC step of one 2D (r, theta) domain in FP64
+    (GRPICEngine::step_forward, src/engines/grpic/grpic.hpp:66-634): fields
+    em, em0, aux (D/E in 0..2, B/H in 3..5), cur, cur0, buff; species as
+    (prm, particle dict) pairs; boundaries as in grpic/fields_bcs.h:234-266.
+    flds_bc / prtl_bc: ((x1 min, x1 max), (x2 min, x2 max)) codes; ds: match
+    layer widths ((x1 min, x1 max), ...); target: the fields the match layer
+    relaxes to (the problem's initial fields), comps: which it defines."""
+
+    def __init__(self, prm: OrtGrParams, em, species, *, flds_bc, prtl_bc, ds, target,
+                 comps=0x3F, nfilter=0, q0=1.0, B0=1.0, order=1, fieldsolver=True, deposit=True,
+                 clear_interval=100, correction=1.0):
+        self.prm = prm
+        self.em = np.array(em, dtype=np.float64)
+        self.em0 = np.zeros_like(self.em)
+        self.aux = np.zeros_like(self.em)
+        shp = (3,) + self.em.shape[1:]
+        self.cur, self.cur0, self.buff = (np.zeros(shp) for _ in range(3))
+        self.species = [(sp, normalize_particles(p, "f64")) for sp, p in species]
+        self.flds_bc, self.prtl_bc, self.ds = flds_bc, prtl_bc, ds
+        self.target = np.array(target, dtype=np.float64)
+        self.comps, self.nfilter, self.order = comps, nfilter, order
+        self.fieldsolver, self.deposit = fieldsolver, deposit
+        self.clear_interval = clear_interval
+        self.dt = prm.dt
+        self.dT = correction * prm.dt
+        self.coeff_j = -prm.dt * q0 / B0
+        self.axis = (flds_bc[1][0] == FLDS_AXIS, flds_bc[1][1] == FLDS_AXIS)
+        self.step = 0
+
+    # ---- grpic/fields_bcs.h:234-266 ----
+    def boundaries(self, tags, kind="main"):
+        if kind == "aux":      # only horizon sides act on aux, and only for main
+            return
+        prm = self.prm
+        if kind == "curr":
+            for d, side in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                if self.prtl_bc[d][side] == PRTL_ABSORB:
+                    if d != 0 or side == 0:
+                        raise ValueError("absorption of currents only in +x1")
+                    lo, hi, edge = gr_match_range(prm, True, self.ds[0][1])
+                    assert gr_bc("absorb_currents", prm, self.cur0, xg_edge=edge,
+                                 ds=self.ds[0][1], i1_range=(lo, hi)) == 0
+            return
+        for d, side in ((0, 0), (1, 0), (1, 1), (0, 1)):   # Directions<2D>::orth
+            bc = self.flds_bc[d][side]
+            for F in (self.em, self.em0):
+                if bc == FLDS_MATCH:
+                    if d != 0:
+                        raise ValueError("match boundaries only in x1")
+                    lo, hi, edge = gr_match_range(prm, side == 1, self.ds[0][side])
+                    assert gr_bc("match", prm, F, self.target, xg_edge=edge, ds=self.ds[0][side],
+                                 i1_range=(lo, hi), tags=tags, comps=self.comps,
+                                 axis=self.axis) == 0
+                elif bc == FLDS_AXIS:
+                    assert gr_bc("axis", prm, F, tags=tags, upper=side == 1) == 0
+                elif bc == FLDS_HORIZON:
+                    assert gr_bc("horizon", prm, F, tags=tags, nfilter=self.nfilter) == 0
+
+    def _f(self, op, a, b, c=None, coeff=0.0):
+        assert gr_field(op, self.prm, a, b, c, coeff=coeff, axis=self.axis) == 0
+
+    def _swap(self):
+        self.em, self.em0 = self.em0, self.em
+        self.cur, self.cur0 = self.cur0, self.cur
+
+    def _init_fields(self):
+        """Step 0 with the field solver on (grpic.hpp:74-280)."""
+        dT = self.dT
+        self.boundaries(BC_B | BC_D)
+        self.em0[...] = self.em
+        self._f("aux_e", self.em, self.em0, self.aux)
+        self._f("aux_h", self.em, self.em0, self.aux)
+        self._f("faraday", self.em0, self.em0, self.aux, 0.5 * dT)
+        self.boundaries(BC_B)
+        self._f("ampere", self.em, self.em, self.aux, 0.5 * dT)
+        self.boundaries(BC_D)
+        self._f("aux_e", self.em, self.em0, self.aux)
+        self._f("aux_h", self.em, self.em0, self.aux)
+        self._f("faraday", self.em, self.em0, self.aux, dT)
+        self.boundaries(BC_B)
+        self._f("ampere", self.em0, self.em0, self.aux, dT)
+        self.boundaries(BC_D)
+        self._f("aux_h", self.em0, self.em0, self.aux)
+        self._f("ampere", self.em, self.em0, self.aux, dT)
+        self.boundaries(BC_D)
+        self._swap()
+
+    def step_forward(self):
+        dT = self.dT
+        if self.step == 0:
+            if self.fieldsolver:
+                self._init_fields()
+            else:
+                self.em0[...] = self.em
+        if self.fieldsolver:
+            self._f("time_average_db", self.em, self.em0)
+            self._f("aux_e", self.em0, self.em, self.aux)
+            self._f("faraday", self.em0, self.em0, self.aux, dT)
+            self.boundaries(BC_B)
+            self._f("aux_h", self.em, self.em0, self.aux)
+        for sp, p in self.species:
+            if p["_n"]:
+                assert gr_push(sp, p, self.em, self.em0, order=self.order, prec="f64") == 0
+        if self.deposit:
+            self.cur0[...] = 0.0
+            for sp, p in self.species:
+                if p["_n"] and sp.charge != 0.0:
+                    assert gr_deposit(sp, p, self.cur0, charge=sp.charge, dt=self.dt) == 0
+            self.boundaries(0x1C0, "curr")
+            for _ in range(self.nfilter):
+                self.buff[...] = self.cur0
+                assert gr_bc("filter", self.prm, self.cur0, self.buff, axis=self.axis) == 0
+        if self.fieldsolver:
+            if self.deposit:
+                self._f("time_average_j", self.cur0, self.cur)
+            self._f("aux_e", self.em, self.em0, self.aux)
+            self._f("faraday", self.em, self.em0, self.aux, dT)
+            self.boundaries(BC_B)
+            self._f("ampere", self.em0, self.em0, self.aux, dT)
+            if self.deposit:
+                self._f("currents_ampere", self.em0, self.cur, coeff=self.coeff_j)
+            self.boundaries(BC_D)
+            self._f("aux_h", self.em0, self.em0, self.aux)
+            self._f("ampere", self.em, self.em0, self.aux, dT)
+            if self.deposit:
+                self._f("currents_ampere", self.em0, self.cur0, coeff=self.coeff_j)
+            self._swap()
+            self.boundaries(BC_D)
+        if self.clear_interval and self.step > 0 and self.step % self.clear_interval == 0:
+            for k, (sp, p) in enumerate(self.species):
+                live = p["tag"][: p["_n"]] == 1
+                q = {key: (v[: p["_n"]][live].copy() if key != "_n" else v)
+                     for key, v in p.items()}
+                q["_n"] = int(live.sum())
+                self.species[k] = (sp, q)
+        self.step += 1

@@ -933,6 +933,190 @@ int gr_field(int op, const ort_gr_params_t* prm, ort_field_t* fa, ort_field_t* f
   return 1;
 }
 
+/* ---- GR field boundaries and the spherical current filter ----------------
+ * op 0: MatchBoundaries_kernel<GRPIC, M, FS, in::x1>, 2D
+ *       (src/kernels/fields_bcs.hpp:176-345): over x1 cells [i1_lo, i1_hi)
+ *       and every x2 row (ghosts included), each component the target
+ *       defines and the tags select is blended towards the target,
+ *       F = s F + (1 - s) T with s = tanh(4 |r - r_edge| / ds), r taken at
+ *       the component's own x1 position; D^3 on and beyond an axis row is
+ *       only damped (s F).
+ * op 1: gr::HorizonBoundaries_kernel (fields_bcs.hpp:1188-1240): rows
+ *       x1 = 0 .. 2 + nfilter copy row G + 1 + nfilter, x2 rows [G, G + n2].
+ * op 2: AxisBoundaries_kernel (fields_bcs.hpp:825-867), every x1 column.
+ * op 3: gr::AbsorbCurrents_kernel<M, 1> (fields_bcs.hpp:1243-1280): J *=
+ *       tanh(|r(x1) - r_edge| / (ds / 4)) over [i1_lo, i1_hi) x every row.
+ * op 4: DigitalFilter_kernel<2D, non-Cartesian> (digital_filter.hpp:196-
+ *       266) over x1 [G, G + n1), x2 [G, G + n2] (currents.h:80-110):
+ *       fa = array, fb = buffer.
+ * Tags: the reference's BC bits, D and E share bits 0-2, B and H bits 3-5
+ * (src/global/global.h:306-329). */
+template <typename R>
+int gr_bc(int op, const ort_gr_params_t* prm, ort_field_t* fa, ort_field_t* fb,
+          const ort_gr_bc_args_t* g, int mutate) {
+  GRMetric<R> m(prm);
+  Fld<R>      F(fa);
+  const int   G = F.ngh, n1 = F.n[0], n2 = F.n[1];
+  const int   N1 = n1 + 2 * G, N2 = n2 + 2 * G;
+  const R     ONE = K<R>::ONE, ZERO = K<R>::ZERO, H = K<R>::HALF;
+  const bool  setD = (g->tags & 0x7u) != 0, setB = (g->tags & 0x38u) != 0;
+  auto r_at = [&](R x1) { const R x[2] = { x1, ZERO }; return m.r_of(x); };
+  if (op == 0) {
+    Fld<R> T(fb);
+    const R four = static_cast<R>(4.0);
+    auto shape = [&](R x1) {
+      return std::tanh(std::fabs(r_at(x1) - static_cast<R>(g->xg_edge)) * four /
+                       static_cast<R>(g->ds));
+    };
+    auto has = [&](int c) { return (g->comps >> c) & 1u; };
+    for (int i2 = 0; i2 < N2; ++i2)
+      for (int i1 = g->i1_lo; i1 < g->i1_hi; ++i1) {
+        const R i1_ = static_cast<R>(i1 - G);
+        auto blend = [&](int c, R s_) { F(i1, i2, c) = s_ * F(i1, i2, c) + (ONE - s_) * T(i1, i2, c); };
+        if (has(0) || has(4)) {   // (i1 + 1/2, i2): D^1, B^2
+          const R s_ = shape(i1_ + H);
+          if (has(0) && setD) blend(0, s_);
+          if (has(4) && setB) blend(4, s_);
+        }
+        if (has(1) || has(3)) {   // (i1, i2 + 1/2): D^2, B^1
+          const R s_ = shape(i1_);
+          if (has(1) && setD) blend(1, s_);
+          if (has(3) && setB) blend(3, s_);
+        }
+        if (has(2) && setD) {     // (i1, i2): D^3
+          const R s_ = shape(i1_);
+          F(i1, i2, 2) = s_ * F(i1, i2, 2);
+          if ((!g->axis_min || i2 > G) && (!g->axis_max || i2 < N2 - G))
+            F(i1, i2, 2) += (ONE - s_) * T(i1, i2, 2);
+        }
+        if (has(5) && setB) {     // (i1 + 1/2, i2 + 1/2): B^3
+          const R s_ = shape(mutate == 1 ? i1_ : i1_ + H);
+          blend(5, s_);
+        }
+      }
+    return 0;
+  }
+  if (op == 1) {
+    const int src = G + 1 + g->nfilter;
+    for (int i2 = G; i2 <= G + n2; ++i2)
+      for (int i = 0; i <= 2 + g->nfilter; ++i) {
+        if (setD)
+          for (int c = 0; c < 3; ++c) F(i, i2, c) = F(src, i2, c);
+        if (setB)
+          for (int c = 3; c < 6; ++c) F(i, i2, c) = F(src, i2, c);
+      }
+    return 0;
+  }
+  if (op == 2) {
+    const R sgn = mutate == 2 ? ONE : -ONE;
+    for (int i1 = 0; i1 < N1; ++i1) {
+      if (!g->upper) {
+        const int e = G;
+        if (setD) {
+          F(i1, e - 1, 1) = sgn * F(i1, e, 1);
+          F(i1, e, 2)     = ZERO;
+          F(i1, e - 1, 2) = F(i1, e + 1, 2);
+        }
+        if (setB) {
+          F(i1, e - 1, 3) = F(i1, e, 3);
+          F(i1, e, 4)     = ZERO;
+          F(i1, e - 1, 4) = -F(i1, e + 1, 4);
+          F(i1, e - 1, 5) = F(i1, e, 5);
+        }
+      } else {
+        const int e = G + n2;
+        if (setD) {
+          F(i1, e, 1)     = sgn * F(i1, e - 1, 1);
+          F(i1, e, 2)     = ZERO;
+          F(i1, e + 1, 2) = F(i1, e - 1, 2);
+        }
+        if (setB) {
+          F(i1, e, 3)     = F(i1, e - 1, 3);
+          F(i1, e, 4)     = ZERO;
+          F(i1, e + 1, 4) = -F(i1, e - 1, 4);
+          F(i1, e, 5)     = F(i1, e - 1, 5);
+        }
+      }
+    }
+    return 0;
+  }
+  if (op == 3) {
+    const R q = static_cast<R>(0.25) * static_cast<R>(g->ds);
+    for (int i2 = 0; i2 < N2; ++i2)
+      for (int i1 = g->i1_lo; i1 < g->i1_hi; ++i1) {
+        const R f = std::tanh(std::fabs(r_at(static_cast<R>(i1 - G)) - static_cast<R>(g->xg_edge)) / q);
+        for (int c = 0; c < 3; ++c) F(i1, i2, c) *= f;
+      }
+    return 0;
+  }
+  if (op == 4) {
+    Fld<R> B(fb);
+    const R Q = K<R>::HALF * K<R>::HALF, E8 = Q * H, S16 = E8 * H;
+    // binomial filter along x1 of buffer component c at (i, j)
+    auto f1 = [&](int c, int i, int j) { return H * B(i, j, c) + Q * (B(i - 1, j, c) + B(i + 1, j, c)); };
+    const int i2_min = G, i2_max = G + n2;
+    for (int i2 = G; i2 <= G + n2; ++i2)
+      for (int i1 = G; i1 < G + n1; ++i1) {
+        if (g->axis_min && i2 == i2_min) {
+          F(i1, i2, 0) = H * f1(0, i1, i2) + H * f1(0, i1, i2 + 1);
+          F(i1, i2, 2) = ZERO;
+          F(i1, i2, 1) = Q * (f1(1, i1, i2) + f1(1, i1, i2 + 1));
+        } else if (g->axis_min && i2 == i2_min + 1) {
+          F(i1, i2, 0) = H * f1(0, i1, i2) + Q * (f1(0, i1, i2 + 1) + f1(0, i1, i2 - 1));
+          F(i1, i2, 2) = H * f1(2, i1, i2) + Q * f1(2, i1, i2 + 1);
+          F(i1, i2, 1) = H * f1(1, i1, i2) + Q * (f1(1, i1, i2 - 1) + f1(1, i1, i2 + 1));
+        } else if (g->axis_max && i2 == i2_max - 1) {
+          F(i1, i2, 0) = H * f1(0, i1, i2) + Q * (f1(0, i1, i2 - 1) + f1(0, i1, i2 + 1));
+          F(i1, i2, 2) = H * f1(2, i1, i2) + Q * f1(2, i1, i2 - 1);
+          F(i1, i2, 1) = Q * (f1(1, i1, i2) + f1(1, i1, i2 - 1));
+        } else if (g->axis_max && i2 == i2_max) {
+          F(i1, i2, 0) = H * f1(0, i1, i2) + H * f1(0, i1, i2 - 1);
+          F(i1, i2, 2) = ZERO;
+        } else {
+          for (int c = 0; c < 3; ++c)
+            F(i1, i2, c) = Q * B(i1, i2, c) +
+                           E8 * (B(i1 - 1, i2, c) + B(i1 + 1, i2, c) + B(i1, i2 - 1, c) +
+                                 B(i1, i2 + 1, c)) +
+                           S16 * (B(i1 - 1, i2 - 1, c) + B(i1 + 1, i2 + 1, c) +
+                                  B(i1 - 1, i2 + 1, c) + B(i1 + 1, i2 - 1, c));
+        }
+      }
+    return 0;
+  }
+  return 1;
+}
+
+/* ---- velocity of the GR deposit (currents_deposit.hpp:112-166, GRPIC) ------
+ * Per particle: theta clamped to [SMALL_ANGLE_GR, pi - SMALL_ANGLE_GR] as
+ * the pusher does, v^i = h^ij u_j (kerr_schild.h:638-642), and
+ * v^i * alpha / sqrt(1 + u_i v^i); the out-of-plane component (set to zero
+ * if not finite) is what the 2D deposit uses.  Dead particles get 0. */
+template <typename R>
+int gr_deposit_velocity(const ort_gr_params_t* prm, const ort_particles_t* prtls, double* vz) {
+  GRMetric<R> m(prm);
+  Prt<R>      P(prtls);
+  const R small = static_cast<R>(SMALL_ANGLE_GR_D);
+  const R large = static_cast<R>(PI_D - SMALL_ANGLE_GR_D);
+  for (int64_t p = 0; p < P.n; ++p) {
+    vz[p] = 0.0;
+    if (P.tag[p] != TAG_DEAD) {
+      R x[2] = { static_cast<R>(P.i1[p]) + static_cast<R>(P.dx1[p]),
+                 static_cast<R>(P.i2[p]) + static_cast<R>(P.dx2[p]) };
+      const R th = m.theta_cd2ph(x[1]);
+      if (th < small) x[1] = m.theta_ph2cd(small);
+      else if (th >= large) x[1] = m.theta_ph2cd(large);
+      const R u[3] = { P.ux1[p], P.ux2[p], P.ux3[p] };
+      const R v[3] = { u[0] * m.h11(x) + u[2] * m.h13(x), u[1] * m.h22(x),
+                       u[0] * m.h13(x) + u[2] * m.h33(x) };
+      const R inv_energy = m.alpha(x) / std::sqrt(K<R>::ONE + u[0] * v[0] + u[1] * v[1] + u[2] * v[2]);
+      R v3 = v[2];
+      if (std::isnan(v3) || std::isinf(v3)) v3 = K<R>::ZERO;
+      vz[p] = static_cast<double>(v3 * inv_energy);
+    }
+  }
+  return 0;
+}
+
 // Metric quantities at code coordinates, for the tests: per point, NQ values
 constexpr int NQ = 27;
 template <typename R>
@@ -997,6 +1181,16 @@ int gr_transform(const ort_gr_params_t* prm, int32_t which, int64_t n, const dou
                                            ort_field_t* fc, double coeff, int32_t axis_min, \
                                            int32_t axis_max, int32_t mutate) {               \
     return gr_field<R>(op, prm, fa, fb, fc, coeff, axis_min, axis_max, mutate);             \
+  }                                                                                         \
+  extern "C" int ntt_oracle_gr_bc_##SUF(int32_t op, const ort_gr_params_t* prm,             \
+                                        ort_field_t* fa, ort_field_t* fb,                   \
+                                        const ort_gr_bc_args_t* args, int32_t mutate) {    \
+    return gr_bc<R>(op, prm, fa, fb, args, mutate);                                         \
+  }                                                                                         \
+  extern "C" int ntt_oracle_gr_deposit_velocity_##SUF(const ort_gr_params_t* prm,           \
+                                                      const ort_particles_t* prtls,         \
+                                                      double* vz) {                         \
+    return gr_deposit_velocity<R>(prm, prtls, vz);                                          \
   }                                                                                         \
   extern "C" int ntt_oracle_gr_transform_##SUF(const ort_gr_params_t* prm, int32_t which,   \
                                                int64_t n, const double* x1,                 \
