@@ -23,9 +23,11 @@
 // reference evaluates its field setter there (fields_bcs.hpp:183-330); a
 // caller fills the array once from the same function.  The deposit's
 // out-of-plane velocity uses the metric in the pusher's precision (FP64 by
-// default) and the in-plane Esirkepov current of the SR deposit, accumulated
-// with FP32 global atomics as ntt_deposit_currents does.
+// default) and the in-plane Esirkepov current of the SR deposit, summed in an
+// LDS window of doubles per workgroup and added to J with FP32 atomics.
 #include "gr_metric.hpp"
+
+#include <climits>
 
 namespace ntt {
 namespace {
@@ -207,6 +209,22 @@ struct AtomicJ {
   }
 };
 
+// J sink into an LDS window of doubles (ds_add_f64, about 20x the throughput
+// of the FP32 LDS atomic on gfx950), origin (w1, w2) in ghost-extended cells
+constexpr int kGW1 = 64, kGW2 = 16, kGWC = kGW1 * kGW2;
+struct LdsJ2 {
+  double* w;
+  int     w1, w2;
+  int     base = 0;
+  __device__ __forceinline__ void set_origin(int i, int j, int) { base = (i - w1) + kGW1 * (j - w2); }
+  __device__ __forceinline__ void add_rel(int a, int b, int, int c, float v) {
+    atomicAdd(w + c * kGWC + base + a + kGW1 * b, static_cast<double>(v));
+  }
+  __device__ __forceinline__ void add(int i, int j, int, int c, float v) {
+    atomicAdd(w + c * kGWC + (i - w1) + kGW1 * (j - w2), static_cast<double>(v));
+  }
+};
+
 struct GRDepArgs {
   ParticleView P;
   BFld         J;
@@ -215,41 +233,94 @@ struct GRDepArgs {
   uint32_t*    err;
 };
 
+// workgroup-wide min (MAX = false) or max of v; every thread calls it
+template <bool MAX>
+__device__ __forceinline__ int blk_reduce(int v, int* red) {
+  auto op = [](int x, int y) { return MAX ? max(x, y) : min(x, y); };
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int r = red[0];
+#pragma unroll
+  for (int w = 1; w < kB / 64; ++w) r = op(r, red[w]);
+  __syncthreads();
+  return r;
+}
+
 // DepositCurrents_kernel<GRPIC, M> at order 1: theta clamped as the pusher
 // does, v^i = h^ij u_j, v^3 alpha / sqrt(1 + u_i v^i) as the out-of-plane
-// velocity (zero if not finite), then the 2D Esirkepov deposit.
+// velocity (zero if not finite), then the 2D Esirkepov deposit.  A workgroup
+// takes kB consecutive particles; when their deposit stencils fit a
+// kGW1 x kGW2 window (particles kept in cell order, as a fresh or re-sorted
+// plasma is, mostly do), the deposit goes to an LDS window of doubles and
+// is added to J with one FP32 atomic per non-zero entry, else straight to J:
+// the FP32 atomics of neighbouring particles no longer queue on the same
+// addresses.  Rounding: the window sums in FP64, so only the order of the
+// final FP32 additions is free, as it is for the direct atomics.
 template <int KIND, typename RR>
 __global__ __launch_bounds__(kB) void k_gr_deposit(GRDepArgs a, KSMetric<RR, KIND> M) {
+  __shared__ double jw[3 * kGWC];
+  __shared__ int    red[kB / 64];
   const int64_t p = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
-  if (p >= a.P.n || a.P.tag[p] == NTT_TAG_DEAD) return;
+  bool live = p < a.P.n && a.P.tag[p] != NTT_TAG_DEAD;
   PState s;
-  s.i[0] = a.P.i1[p];   s.ip[0] = a.P.i1p[p];
-  s.dx[0] = a.P.dx1[p]; s.dxp[0] = a.P.dx1p[p];
-  s.i[1] = a.P.i2[p];   s.ip[1] = a.P.i2p[p];
-  s.dx[1] = a.P.dx2[p]; s.dxp[1] = a.P.dx2p[p];
-  for (int d = 0; d < 2; ++d) {
-    int lo, hi;
-    deposit_extent<1>(s.i[d], s.dx[d], s.ip[d], s.dxp[d], lo, hi);
-    if (lo + a.G < 0 || hi + a.G >= (d == 0 ? a.J.N1 : a.J.N2)) {
-      raise_device_error(a.err, NTT_DEVERR_OUT_OF_BOUNDS);
-      return;
+  int lo[2] = { INT_MAX, INT_MAX }, hi[2] = { INT_MIN, INT_MIN };
+  if (live) {
+    s.i[0] = a.P.i1[p];   s.ip[0] = a.P.i1p[p];
+    s.dx[0] = a.P.dx1[p]; s.dxp[0] = a.P.dx1p[p];
+    s.i[1] = a.P.i2[p];   s.ip[1] = a.P.i2p[p];
+    s.dx[1] = a.P.dx2[p]; s.dxp[1] = a.P.dx2p[p];
+    for (int d = 0; d < 2; ++d) {
+      deposit_extent<1>(s.i[d], s.dx[d], s.ip[d], s.dxp[d], lo[d], hi[d]);
+      if (lo[d] + a.G < 0 || hi[d] + a.G >= (d == 0 ? a.J.N1 : a.J.N2)) {
+        raise_device_error(a.err, NTT_DEVERR_OUT_OF_BOUNDS);
+        live = false;
+      }
+    }
+    if (!live) lo[0] = lo[1] = INT_MAX, hi[0] = hi[1] = INT_MIN;
+  }
+  // the workgroup's box (ghost-extended cells) and whether it fits the window
+  const int b1 = blk_reduce<false>(lo[0], red), b2 = blk_reduce<false>(lo[1], red);
+  const int e1 = blk_reduce<true>(hi[0], red), e2 = blk_reduce<true>(hi[1], red);
+  if (b1 > e1) return;   // no live particle (uniform across the workgroup)
+  const bool fits = e1 - b1 < kGW1 && e2 - b2 < kGW2;
+  const int w1 = b1 + a.G, w2 = b2 + a.G;
+  if (fits) {
+    for (int e = threadIdx.x; e < 3 * kGWC; e += kB) jw[e] = 0.0;
+    __syncthreads();
+  }
+  if (live) {
+    const RR u1 = a.P.ux1[p], u2 = a.P.ux2[p], u3 = a.P.ux3[p];
+    const RR x1 = static_cast<RR>(s.i[0]) + static_cast<RR>(s.dx[0]);
+    RR x2 = static_cast<RR>(s.i[1]) + static_cast<RR>(s.dx[1]);
+    const RR th = M.theta_ph(x2);
+    if (th < M.k.small_angle) x2 = M.k.th_small_cd;
+    else if (static_cast<double>(th) >= M.k.large_angle) x2 = M.k.th_large_cd;
+    MPt<RR> m;
+    M.eval(x1, x2, m);
+    const RR v1 = u1 * m.h11 + u3 * m.h13, v2 = u2 * m.h22, v3 = u1 * m.h13 + u3 * m.h33;
+    const RR inv_energy = m.alpha / sqrt(RR(1) + u1 * v1 + u2 * v2 + u3 * v3);
+    RR vz = v3;
+    if (isnan(vz) || isinf(vz)) vz = RR(0);
+    const float vzf = static_cast<float>(vz * inv_energy), coeff = a.P.w[p] * a.charge;
+    if (fits) {
+      LdsJ2 J{ jw, w1, w2 };
+      deposit_esirkepov_2d<1>(J, a.G, s, vzf, coeff, a.inv_dt);
+    } else {
+      AtomicJ J{ a.J };
+      deposit_esirkepov_2d<1>(J, a.G, s, vzf, coeff, a.inv_dt);
     }
   }
-  const RR u1 = a.P.ux1[p], u2 = a.P.ux2[p], u3 = a.P.ux3[p];
-  const RR x1 = static_cast<RR>(s.i[0]) + static_cast<RR>(s.dx[0]);
-  RR x2 = static_cast<RR>(s.i[1]) + static_cast<RR>(s.dx[1]);
-  const RR th = M.theta_ph(x2);
-  if (th < M.k.small_angle) x2 = M.k.th_small_cd;
-  else if (static_cast<double>(th) >= M.k.large_angle) x2 = M.k.th_large_cd;
-  MPt<RR> m;
-  M.eval(x1, x2, m);
-  const RR v1 = u1 * m.h11 + u3 * m.h13, v2 = u2 * m.h22, v3 = u1 * m.h13 + u3 * m.h33;
-  const RR inv_energy = m.alpha / sqrt(RR(1) + u1 * v1 + u2 * v2 + u3 * v3);
-  RR vz = v3;
-  if (isnan(vz) || isinf(vz)) vz = RR(0);
-  AtomicJ J{ a.J };
-  deposit_esirkepov_2d<1>(J, a.G, s, static_cast<float>(vz * inv_energy), a.P.w[p] * a.charge,
-                          a.inv_dt);
+  if (!fits) return;
+  __syncthreads();
+  const int n1 = e1 - b1 + 1, n2 = e2 - b2 + 1;
+  for (int e = threadIdx.x; e < 3 * n1 * n2; e += kB) {
+    const int c = e / (n1 * n2), r = e - c * (n1 * n2);
+    const int y = r / n1, x = r - y * n1;
+    const double v = jw[c * kGWC + x + kGW1 * y];
+    if (v != 0.0) unsafeAtomicAdd(&a.J.at(w1 + x, w2 + y, c), static_cast<float>(v));
+  }
 }
 
 template <typename RR>

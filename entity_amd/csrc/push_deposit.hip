@@ -500,7 +500,18 @@ __device__ __forceinline__ void push_deposit_chunk(const FusedArgs& a, int64_t c
 template <int D, int O, int ALG>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NTT_PD_WAVES_PER_EU)))
 void k_push_deposit(FusedArgs a) {
-  const int64_t chunk = xcd_chunk(a.nchunks);
+  // With tile offsets the last chunk is the tail past the last tile (dead or
+  // appended particles, usually none).  It takes the last block: the tiles
+  // keep their XCD-aware mapping, and a box whose tiles fill the GPU exactly
+  // once (two workgroups per CU) does not send a real tile to a second round
+  // behind them.
+  int64_t chunk;
+  if (a.toff != nullptr) {
+    const int64_t ntiles = a.nchunks - 1;
+    chunk = (static_cast<int64_t>(blockIdx.x) == ntiles) ? ntiles : xcd_chunk(ntiles);
+  } else {
+    chunk = xcd_chunk(a.nchunks);
+  }
   if (chunk < 0) return;
   push_deposit_chunk<D, O, ALG>(a, chunk);
 }

@@ -376,8 +376,9 @@ NTT_API ntt_status ntt_gr_digital_filter(ntt_field_t* cur, const ntt_field_t* bu
                                          const ntt_flds_bc_t* bc, void* stream);
 
 /* The GR current deposit (order 1, whatever the pusher's shape order), added
- * atomically into `cur` with its ghosts; precision 64 (or 0) evaluates the
- * metric in FP64, 32 in FP32.  Replaces kernel::DepositCurrents_kernel<GRPIC,
+ * atomically into `cur` with its ghosts (summed per workgroup in an LDS
+ * window where its particles' stencils fit one); precision 64 (or 0)
+ * evaluates the metric in FP64, 32 in FP32.  Replaces kernel::DepositCurrents_kernel<GRPIC,
  * M> (src/kernels/currents_deposit.hpp:108-555) as grpic::CurrentsDeposit
  * launches it (src/engines/grpic/currents.h:34-77). */
 NTT_API ntt_status ntt_gr_deposit_currents(const ntt_particles_t* prtls, ntt_field_t* cur,

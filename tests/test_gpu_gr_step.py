@@ -153,13 +153,19 @@ def _load(sp, h):
 
 @pytest.mark.parametrize("kind", list(KINDS))
 @pytest.mark.parametrize("precision", [64, 32])
-def test_gr_deposit(hip, oracle, kind, precision):
+@pytest.mark.parametrize("ordered", [False, True])
+def test_gr_deposit(hip, oracle, kind, precision, ordered):
+    """Random order: each workgroup's particles span the grid and deposit
+    with global atomics; cell order: they fit the LDS window."""
     from entity_amd.containers import Particles
     from entity_amd.kernels import device_errors, gr
     n = (32, 24)
     prm, met, shape = _setup(oracle, kind, n, a=0.9 if kind != "kerr_schild_0" else 0.0)
     rng = np.random.default_rng(24)
     h = _gr_particles(rng, n, 4000)
+    if ordered:
+        order = np.lexsort((h["i1"], h["i2"]))
+        h = {k: v[order] for k, v in h.items()}
     sp = Particles(2, 4096, device="cuda", with_phi=True)
     _load(sp, h)
     J = torch.zeros(shape.tensor_shape(3), device="cuda")
