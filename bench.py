@@ -134,6 +134,26 @@ def measured_traffic(cells, ppc, tile):
     return None
 
 
+def measured_valu_per_64(order, pusher):
+    """VALU wavefront instructions per 64 particles of the fused 3D launch,
+    from the committed rocprofv3 SQ_INSTS_VALU summary (order 2, Vay)."""
+    if order != 2 or pusher != "vay":
+        return None
+    path = ROOT / "profiles" / "k_push_deposit_3d_o2_vay_128x64x64_counters.md"
+    try:
+        for line in path.read_text().splitlines():
+            if line.startswith("valu_per_64_particles:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
+# CU count and peak engine clock of the MI355X: one 64-wide FP32 VALU
+# wavefront instruction per CU-cycle at most (four 16-lane SIMDs)
+MI355X_CUS, MI355X_CLOCK_HZ = 256, 2.4e9
+
+
 def parity_leg(args):
     """The C1 step's physics (order 2, Vay, 2 filter passes, B0 z plus the
     random-phase modes, theta = 0.3) on a 32 x 16 x 16 box, one ntt_sr_step
@@ -871,6 +891,9 @@ def main():
     bytes_launch = npart * b_p + ncell * 36
     avg_ms = ms_pd.value / max(nl.value, 1)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    v64 = measured_valu_per_64(args.order, args.pusher) if dim == 3 else None
+    valu_frac = (npart / 64 * v64 / (MI355X_CUS * MI355X_CLOCK_HZ * avg_ms * 1e-3)
+                 if v64 else None)
     out = {
         "metric": METRIC,
         "value": value,
@@ -912,6 +935,10 @@ def main():
             "avg_launch_ms": avg_ms,
             "launches": int(nl.value),
             "algorithmic_bytes_per_launch": bytes_launch,
+            "valu_frac": valu_frac,
+            "valu_source": "profiles/k_push_deposit_3d_o2_vay_128x64x64_counters.md "
+                           "(SQ_INSTS_VALU per 64 particles) x particles per launch / "
+                           "(256 CUs x 2.4 GHz x the launch time measured here)",
         },
         "device_errors": errs,
     }
