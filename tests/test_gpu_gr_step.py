@@ -320,3 +320,75 @@ def test_engine_gauss_law(hip, oracle):
     I1 = invariant()[sel]
     scale = np.abs(I0).max()
     np.testing.assert_allclose(I1, I0, rtol=0, atol=2e-5 * scale)
+
+
+@pytest.mark.parametrize("kind", ["kerr_schild", "kerr_schild_0"])
+def test_engine_other_metrics_and_horizon(hip, oracle, kind):
+    """The engine against the oracle's step on the uniform-r grids (KS at
+    a = 0.6 with a horizon boundary at r_min inside the horizon; KS0, flat
+    space), Wald's field built from its vector potential, a random pair
+    plasma kept off the poles: 3 steps, fields and particles as in
+    test_engine_matches_oracle_step."""
+    from entity_amd import _abi
+    from entity_amd.grpic import GRPICEngine, GRSpeciesSpec
+    from entity_amd.kernels import device_errors
+    n, a = (40, 32), (0.6 if kind == "kerr_schild" else 0.0)
+    prm, met, shape = _setup(oracle, kind, n, a=a, rmax=10.0)
+    em = wald_fields(oracle, prm, n, a).astype(np.float32)
+    dr = (prm.x1_max - prm.x1_min) / n[0]
+    dt = float(np.float32(0.25 * min(dr, prm.x1_min * math.pi / n[1])))
+    prm.dt = dt
+    lo_bc = _abi.FLDS_BC_HORIZON if kind == "kerr_schild" else _abi.FLDS_BC_MATCH
+    flds = ((lo_bc, _abi.FLDS_BC_MATCH), (_abi.FLDS_BC_AXIS, _abi.FLDS_BC_AXIS))
+    prtl = ((_abi.PRTL_BC_HORIZON, _abi.PRTL_BC_ABSORB), (_abi.PRTL_BC_AXIS, _abi.PRTL_BC_AXIS))
+    specs = [GRSpeciesSpec("e-", 1.0, -1.0), GRSpeciesSpec("e+", 1.0, 1.0)]
+    ds = ((0.5, 1.5),)
+    eng = GRPICEngine(met, specs, 4096, dt=dt, flds_bc=flds, prtl_bc=prtl, ds=ds,
+                      target=_dev(em), nfilter=1, q0=0.05, B0=1.0, clear_interval=2)
+    eng.em.copy_(_dev(em))
+    rng = np.random.default_rng(41)
+    hosts, oprm = [], []
+    for sp in eng.species:
+        m = 3000
+        cell = np.sort(rng.integers(0, n[0] * (n[1] - 6), m))
+        h = dict(i1=(cell % n[0]).astype(np.int32), i2=(3 + cell // n[0]).astype(np.int32),
+                 dx1=rng.random(m).astype(np.float32), dx2=rng.random(m).astype(np.float32),
+                 ux1=(0.3 * rng.standard_normal(m)).astype(np.float32),
+                 ux2=(0.3 * rng.standard_normal(m)).astype(np.float32),
+                 ux3=(0.3 * rng.standard_normal(m)).astype(np.float32),
+                 weight=np.ones(m, np.float32), tag=np.ones(m, np.int16))
+        for k in ("i1", "i2", "dx1", "dx2"):
+            h[k + "_prev"] = h[k].copy()
+        _load(sp, h)
+        hosts.append(h)
+        oprm.append(oracle.gr_params(kind, n, ((prm.x1_min, prm.x1_max), (0.0, math.pi)), a=a,
+                                     dt=dt, mass=sp.mass, charge=sp.charge, omegaB0=1.0,
+                                     niter=10, absorb=(True, True), axis=(True, True)))
+    bc_o = {_abi.FLDS_BC_HORIZON: oracle.FLDS_HORIZON, _abi.FLDS_BC_MATCH: oracle.FLDS_MATCH}
+    st = oracle.GRStep(prm, em.astype(np.float64), list(zip(oprm, hosts)),
+                       flds_bc=((bc_o[lo_bc], oracle.FLDS_MATCH),
+                                (oracle.FLDS_AXIS, oracle.FLDS_AXIS)),
+                       prtl_bc=((oracle.PRTL_HORIZON, oracle.PRTL_ABSORB),
+                                (oracle.PRTL_AXIS, oracle.PRTL_AXIS)),
+                       ds=ds, target=em.astype(np.float64), nfilter=1, q0=0.05, B0=1.0,
+                       clear_interval=2)
+    for _ in range(3):
+        eng.step_forward()
+        st.step_forward()
+    torch.cuda.synchronize()
+    assert device_errors() == 0
+    for name in ("em", "em0"):
+        got = getattr(eng, name).cpu().numpy().astype(np.float64)
+        ref = getattr(st, name)
+        floor = 2e-6 * np.abs(ref).max()
+        for c in range(6):
+            sc = max(np.abs(ref[c]).max(), 1e-30)
+            np.testing.assert_allclose(got[c], ref[c], rtol=0, atol=1e-4 * sc + floor,
+                                       err_msg=f"{kind} {name}[{c}]")
+    for sp, (_, q) in zip(eng.species, st.species):
+        h = sp.to_host()
+        m = q["_n"]
+        assert sp.npart == m
+        for k in ("ux1", "ux2", "ux3"):
+            np.testing.assert_allclose(np.sort(h[k][:m].astype(np.float64)), np.sort(q[k][:m]),
+                                       rtol=0, atol=1e-4 * max(np.abs(q[k][:m]).max(), 1))
