@@ -254,8 +254,8 @@ def functor_leg(eng, frac=16):
     torch.cuda.synchronize()
     t_push, t_dep = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
     npart = int(abi.npart)
-    return {"workload": f"ntt_sr_push + ntt_deposit_currents (one thread per particle, fields "
-                        f"from HBM, float atomics into cur) on {npart} particles (1/{frac} of "
+    return {"workload": f"ntt_sr_push + ntt_deposit_currents (fields "
+                        f"from HBM, J per workgroup in LDS, then atomics into cur) on {npart} particles (1/{frac} of "
                         f"one species of the benchmark's box)",
             "particles": npart, "push_ms": t_push, "deposit_ms": t_dep,
             "value": npart / ((t_push + t_dep) * 1e-3), "unit": "particle-steps/s (push + deposit)"}

@@ -1,11 +1,15 @@
 // sr_kernels.hip -- kernel-level SR pusher and current deposit over an
 // arbitrary (unsorted) particle range, reading E/B from HBM and accumulating J
-// with hardware float atomics at the reference's ghost-extended indices.
+// at the reference's ghost-extended indices: per workgroup in an LDS window of
+// doubles when its particles are spatially close (then one FP32 atomic per
+// touched entry), else with FP32 atomics straight into J.
 //
 // These are the drop-ins for kernel::sr::Pusher_kernel and
 // kernel::DepositCurrents_kernel; the fused, tile-windowed push+deposit of
 // the engine step lives in push_deposit.hip.
 #include "sr_particle.hpp"
+
+#include <climits>
 
 namespace ntt {
 namespace {
@@ -107,12 +111,55 @@ struct DepositArgs {
   uint32_t*    err;
 };
 
+// LDS window of the deposit (cells per dimension): a workgroup's 256
+// consecutive particles, when kept in spatial order (the reference sorts
+// them periodically), deposit into it with FP64 LDS atomics; the window is
+// then added to J with one FP32 atomic per non-zero entry.
+template <int D> struct DepWin;
+template <> struct DepWin<2> { static constexpr int W1 = 64, W2 = 16, W3 = 1; };
+template <> struct DepWin<3> { static constexpr int W1 = 14, W2 = 10, W3 = 10; };
+
+template <int D>
+struct LdsJd {
+  static constexpr int W1 = DepWin<D>::W1, W2 = DepWin<D>::W2, WC = W1 * W2 * DepWin<D>::W3;
+  double* w;
+  int     o1, o2, o3;
+  int     base = 0;
+  __device__ __forceinline__ int index(int i, int j, int k) const {
+    return (i - o1) + W1 * ((j - o2) + W2 * (k - o3));
+  }
+  __device__ __forceinline__ void set_origin(int i, int j, int k) { base = index(i, j, k); }
+  __device__ __forceinline__ void add_rel(int a, int b, int g, int c, float v) {
+    atomicAdd(w + c * WC + base + a + W1 * (b + W2 * g), static_cast<double>(v));
+  }
+  __device__ __forceinline__ void add(int i, int j, int k, int c, float v) {
+    atomicAdd(w + c * WC + index(i, j, k), static_cast<double>(v));
+  }
+};
+
+template <bool MAX>
+__device__ __forceinline__ int block_reduce(int v, int* red) {
+  auto op = [](int x, int y) { return MAX ? max(x, y) : min(x, y); };
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int r = red[0];
+#pragma unroll
+  for (int w = 1; w < kBlock / 64; ++w) r = op(r, red[w]);
+  __syncthreads();
+  return r;
+}
+
+// particles per thread of the deposit: a workgroup takes kBlock * kDepPPT
+// consecutive particles, so the window's clearing and flush are paid once
+// per 1024 particles
+constexpr int kDepPPT = 4;
+
 template <int D, int O>
-__global__ __launch_bounds__(kBlock) void k_deposit(DepositArgs a) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (p >= a.P.n) return;
-  if (a.P.tag[p] == NTT_TAG_DEAD) return;
-  PState s;
+__device__ __forceinline__ bool deposit_state(const DepositArgs& a, int64_t p, PState& s,
+                                              int* lo, int* hi) {
+  if (p >= a.P.n || a.P.tag[p] == NTT_TAG_DEAD) return false;
   s.i[0] = a.P.i1[p];   s.ip[0] = a.P.i1p[p];
   s.dx[0] = a.P.dx1[p]; s.dxp[0] = a.P.dx1p[p];
   s.i[1] = a.P.i2[p];   s.ip[1] = a.P.i2p[p];
@@ -121,23 +168,84 @@ __global__ __launch_bounds__(kBlock) void k_deposit(DepositArgs a) {
     s.i[2] = a.P.i3[p];   s.ip[2] = a.P.i3p[p];
     s.dx[2] = a.P.dx3[p]; s.dxp[2] = a.P.dx3p[p];
   }
-  if constexpr (D == 2) {
-    s.u[0] = a.P.ux1[p];
-    s.u[1] = a.P.ux2[p];
-    s.u[2] = a.P.ux3[p];
-  }
-  // refuse particles whose stencil leaves the ghost-extended array
+  // particles whose stencil leaves the ghost-extended array are refused
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    int lo, hi;
-    deposit_extent<O>(s.i[d], s.dx[d], s.ip[d], s.dxp[d], lo, hi);
-    if (lo + a.ngh < 0 || hi + a.ngh >= a.N[d]) {
+    deposit_extent<O>(s.i[d], s.dx[d], s.ip[d], s.dxp[d], lo[d], hi[d]);
+    if (lo[d] + a.ngh < 0 || hi[d] + a.ngh >= a.N[d]) return false;
+  }
+  return true;
+}
+
+template <int D, int O>
+__global__ __launch_bounds__(kBlock) void k_deposit(DepositArgs a) {
+  constexpr int W1 = DepWin<D>::W1, W2 = DepWin<D>::W2, W3 = DepWin<D>::W3;
+  __shared__ double jw[3 * W1 * W2 * W3];
+  __shared__ int    red[kBlock / 64];
+  const int64_t p0 = static_cast<int64_t>(blockIdx.x) * (kBlock * kDepPPT) + threadIdx.x;
+  // the workgroup's box of deposit cells, and whether it fits the window
+  int blo[3] = { INT_MAX, INT_MAX, INT_MAX }, bhi[3] = { INT_MIN, INT_MIN, INT_MIN };
+#pragma unroll 1
+  for (int q = 0; q < kDepPPT; ++q) {
+    const int64_t p = p0 + static_cast<int64_t>(q) * kBlock;
+    PState s;
+    int lo[3], hi[3];
+    if (deposit_state<D, O>(a, p, s, lo, hi)) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        blo[d] = min(blo[d], lo[d]);
+        bhi[d] = max(bhi[d], hi[d]);
+      }
+    } else if (p < a.P.n && a.P.tag[p] != NTT_TAG_DEAD) {
       raise_device_error(a.err, NTT_DEVERR_OUT_OF_BOUNDS);
-      return;
     }
   }
-  const float coeff = a.P.w[p] * a.charge;
-  deposit_particle<D, O>(a.J, a.ngh, s, coeff, a.inv_dt);
+  bool fits = true;
+  int  b[3] = { 0, 0, 0 }, e[3] = { 0, 0, 0 };
+  const int Wd[3] = { W1, W2, W3 };
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    b[d] = block_reduce<false>(blo[d], red);
+    e[d] = block_reduce<true>(bhi[d], red);
+    if (b[d] > e[d]) return;   // no live particle (uniform across the workgroup)
+    if (e[d] - b[d] >= Wd[d]) fits = false;
+  }
+  const int o1 = b[0] + a.ngh, o2 = b[1] + a.ngh, o3 = (D == 3) ? b[2] + a.ngh : 0;
+  if (fits) {
+    for (int k = threadIdx.x; k < 3 * W1 * W2 * W3; k += kBlock) jw[k] = 0.0;
+    __syncthreads();
+  }
+#pragma unroll 1
+  for (int q = 0; q < kDepPPT; ++q) {
+    const int64_t p = p0 + static_cast<int64_t>(q) * kBlock;
+    PState s;
+    int lo[3], hi[3];
+    if (!deposit_state<D, O>(a, p, s, lo, hi)) continue;
+    if constexpr (D == 2) {
+      s.u[0] = a.P.ux1[p];
+      s.u[1] = a.P.ux2[p];
+      s.u[2] = a.P.ux3[p];
+    }
+    const float coeff = a.P.w[p] * a.charge;
+    if (fits) {
+      LdsJd<D> J{ jw, o1, o2, o3 };
+      deposit_particle<D, O>(J, a.ngh, s, coeff, a.inv_dt);
+    } else {
+      GlobalJ J = a.J;
+      deposit_particle<D, O>(J, a.ngh, s, coeff, a.inv_dt);
+    }
+  }
+  if (!fits) return;
+  __syncthreads();
+  const int n1 = e[0] - b[0] + 1, n2 = e[1] - b[1] + 1, n3 = (D == 3) ? e[2] - b[2] + 1 : 1;
+  const int nb = n1 * n2 * n3;
+  for (int k = threadIdx.x; k < 3 * nb; k += kBlock) {
+    const int c = k / nb, r = k - c * nb;
+    const int z = r / (n1 * n2), r2 = r - z * (n1 * n2);
+    const int y = r2 / n1, x = r2 - y * n1;
+    const double v = jw[c * (W1 * W2 * W3) + x + W1 * (y + W2 * z)];
+    if (v != 0.0) a.J.add(o1 + x, o2 + y, o3 + z, c, static_cast<float>(v));
+  }
 }
 
 template <int D, int O, int ALG>
@@ -241,7 +349,7 @@ ntt_status launch_deposit(const ParticleView& pv, const FieldView& J, int D, int
   a.charge = charge;
   a.inv_dt = ONE / dt;
   a.err = err;
-  const unsigned g = grid_for(pv.n, kBlock);
+  const unsigned g = grid_for(pv.n, kBlock * kDepPPT);
   if (D == 2) {
     if (order == 0) k_deposit<2, 0><<<g, kBlock, 0, st>>>(a);
     else if (order == 1) k_deposit<2, 1><<<g, kBlock, 0, st>>>(a);

@@ -386,7 +386,9 @@ NTT_API ntt_status ntt_gr_deposit_currents(const ntt_particles_t* prtls, ntt_fie
                                            int32_t precision, void* stream);
 
 /* Charge-conserving current deposit, accumulated (atomically) into `cur`
- * including its ghost cells, exactly where the reference writes.
+ * including its ghost cells, exactly where the reference writes (summed per
+ * workgroup of 1,024 consecutive particles in an LDS window when their
+ * stencils fit one, as spatially sorted particles do).
  * Replaces kernel::DepositCurrents_kernel<SRPIC, Minkowski<D>, O>
  * (src/kernels/currents_deposit.hpp:33-762): O = 0 zig-zag, 1..2 Esirkepov.
  * The caller zeroes `cur` first, as srpic::CurrentsDeposit does

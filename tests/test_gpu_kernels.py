@@ -122,12 +122,20 @@ def test_push_parity(hip, oracle, dim, order, alg, strict):
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("order", [0, 1, 2])
 @pytest.mark.parametrize("strict", [True, False])
-def test_deposit_parity(hip, oracle, dim, order, strict):
+@pytest.mark.parametrize("ordered", [False, True])
+def test_deposit_parity(hip, oracle, dim, order, strict, ordered):
+    """Random order, and cell order (each workgroup's particles then fit the
+    deposit's LDS window in 3D too; in 2D the whole grid fits it)."""
     from entity_amd.containers import GridShape
     from entity_amd.kernels import DepositCurrents_kernel, Minkowski, device_errors
     n = (13, 10, 9)[:dim]
     rng = np.random.default_rng(100 + order + dim)
     p = make_particles(dim, n, 8192 if dim == 2 else 4096, seed=3 * order + dim)
+    if ordered:
+        keys = [p[f"i{d + 1}"] for d in range(dim)]
+        order_ = np.lexsort(keys)
+        p = {k: (v[order_] if hasattr(v, "__len__") and len(v) == len(order_) else v)
+             for k, v in p.items()}
     # previous state: one move of at most 0.45 cells, with periodic index shift
     for d in range(dim):
         disp = rng.uniform(-0.45, 0.45, len(p["tag"])).astype(np.float32)
