@@ -2,8 +2,11 @@
 
 Two builds of the same HIP sources are produced:
 
-* ``lib/libntt_hip.so`` -- the product build.  Floating-point contraction is
-  left to the compiler (``-ffp-contract=fast``), so multiply-adds become FMAs.
+* ``lib/libntt_hip.so`` -- the product build.  Each source expression's
+  multiply-adds become FMAs (``-ffp-contract=on``): contraction is decided per
+  expression by the front end, not per basic block by the back end, so an
+  expression rounds the same in every kernel that inlines it (the fused, split
+  and batched push kernels give bit-identical particles).
 * ``lib/libntt_hip_strict.so`` -- identical sources compiled with
   ``-ffp-contract=off``.  Its single-precision arithmetic follows the
   reference's operation order exactly, which is what lets the parity tests
@@ -46,7 +49,7 @@ def _common_flags(strict: bool) -> list[str]:
         "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
         f"--offload-arch={ARCH}",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-ffp-contract=off" if strict else "-ffp-contract=fast",
+        "-ffp-contract=off" if strict else "-ffp-contract=on",
         "-Wall", "-Wno-unused-function", "-Wno-unused-result",
         f"-I{INCLUDE}", f"-I{CSRC}",
     ] + (["-DNTT_STRICT_FP=1"] if strict else [])

@@ -58,12 +58,39 @@ bool fd_nonzero(const ntt_fd_coeffs_t& f) {
 
 extern "C" ntt_status ntt_faraday_mink(ntt_field_t*, float, float, const ntt_fd_coeffs_t*, void*);
 
+namespace {
+
+// Workspace of ntt_sr_step: the fixed-point current accumulator (the layout
+// of `cur`, 64-bit entries), then, in 3D, the chunk flags of the split push
+// and deposit launches for up to one chunk per cell plus the tail (a sort
+// tile holds at least one cell), then the push's per-chunk box records for
+// tiles of 16 cells or more (smaller tiles run without them).
+struct StepWorkspace {
+  uint64_t acc_bytes, flag_bytes, rec_bytes;
+  int64_t  flag_chunks, rec_chunks;
+};
+StepWorkspace step_workspace(const FieldView& J, int dim) {
+  StepWorkspace w{};
+  w.acc_bytes = static_cast<uint64_t>(J.cstride) * 3 * sizeof(unsigned long long);
+  if (dim == 3) {
+    const int64_t ncells = static_cast<int64_t>(J.n1) * J.n2 * J.n3;
+    w.flag_chunks = ncells + 1;
+    w.flag_bytes  = static_cast<uint64_t>(split_flag_bytes(w.flag_chunks));
+    w.rec_chunks  = (ncells / 16 + 1 + 15) / 16 * 16;
+    w.rec_bytes   = static_cast<uint64_t>(split_record_bytes(w.rec_chunks));
+  }
+  return w;
+}
+
+}  // namespace
+
 extern "C" ntt_status ntt_sr_step_workspace_size(const ntt_field_t* cur, uint64_t* bytes) {
   if (bytes == nullptr) return NTT_ERR_ARG;
   FieldView J;
   ntt_status s = make_field_view(cur, 3, &J);
   if (s != NTT_OK) return s;
-  *bytes = static_cast<uint64_t>(J.cstride) * 3 * sizeof(unsigned long long);
+  const StepWorkspace w = step_workspace(J, cur->dim);
+  *bytes = w.acc_bytes + w.flag_bytes + w.rec_bytes;
   return NTT_OK;
 }
 
@@ -82,6 +109,10 @@ struct Prep {
 struct StepPlan {
   FieldView           E, J, Bf;
   unsigned long long* acc;
+  uint32_t*           flags;             // split-path chunk flags (null: fused kernel)
+  int64_t             flag_chunks;
+  void*               recs;              // split-path box records (null: none)
+  int64_t             rec_chunks;
   int                 D;
   float               dt, dxh;
   bool                ext;               // extended Faraday stencil
@@ -105,11 +136,26 @@ ntt_status plan_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t* buff,
   if (s != NTT_OK) return s;
   const FieldView &E = P->E, &J = P->J, &Bf = P->Bf;
   if (!same_grid(E, J) || !same_grid(E, Bf) || J.d == Bf.d) return NTT_ERR_ARG;
-  const uint64_t ws_need = static_cast<uint64_t>(J.cstride) * 3 * sizeof(unsigned long long);
-  if (prm->deposit_enable && (workspace == nullptr || workspace_bytes < ws_need))
+  const int D = em->dim;
+  // the accumulator is required; the flags after it are used when the
+  // workspace holds them (ntt_sr_step_workspace_size counts them)
+  const StepWorkspace wsz = step_workspace(J, D);
+  if (prm->deposit_enable && (workspace == nullptr || workspace_bytes < wsz.acc_bytes))
     return NTT_ERR_CAPACITY;
   P->acc = static_cast<unsigned long long*>(workspace);
-  const int D = em->dim;
+  P->flags = nullptr;
+  P->flag_chunks = 0;
+  P->recs = nullptr;
+  P->rec_chunks = 0;
+  if (workspace != nullptr && wsz.flag_bytes > 0 &&
+      workspace_bytes >= wsz.acc_bytes + wsz.flag_bytes) {
+    P->flags = reinterpret_cast<uint32_t*>(static_cast<char*>(workspace) + wsz.acc_bytes);
+    P->flag_chunks = wsz.flag_chunks;
+    if (workspace_bytes >= wsz.acc_bytes + wsz.flag_bytes + wsz.rec_bytes) {
+      P->recs = static_cast<char*>(workspace) + wsz.acc_bytes + wsz.flag_bytes;
+      P->rec_chunks = wsz.rec_chunks;
+    }
+  }
   P->D = D;
   if (metric->dim != D || !(metric->dx > 0.0f) || !(prm->dt > 0.0f)) return NTT_ERR_ARG;
   if (E.ngh != 2) return NTT_ERR_UNSUPPORTED;
@@ -206,7 +252,8 @@ ntt_status sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t* buff, ntt_spe
     const bool timed = timed_ok && prof_begin(st, &ev);
     s = launch_push_deposit_periodic(p.pv, E, J, P.acc, P.D, prm->shape_order, p.alg, p.ndth,
                                      P.dt, metric->dx, species[sp].charge, p.dep, P.fscale,
-                                     species[sp].tile_offsets, species[sp].ntiles, ppt, st);
+                                     species[sp].tile_offsets, species[sp].ntiles, ppt,
+                                     P.flags, P.flag_chunks, P.recs, P.rec_chunks, st);
     if (timed) prof_end(st, ev);
     if (s != NTT_OK) return s;
   }
@@ -277,10 +324,12 @@ extern "C" ntt_status ntt_sr_step_static(ntt_field_t* em, ntt_field_t* cur, ntt_
                                          ntt_species_t* species, int32_t nspecies,
                                          const ntt_sr_step_params_t* prm,
                                          const ntt_minkowski_t* metric, void* stream) {
-  uint64_t need = 0;
-  ntt_status s = ntt_sr_step_workspace_size(cur, &need);
+  // the accumulator must fit; the split path's flags and records are used
+  // when they fit after it
+  FieldView J;
+  ntt_status s = make_field_view(cur, 3, &J);
   if (s != NTT_OK) return s;
-  if (need > kStaticScratchBytes) return NTT_ERR_CAPACITY;
+  if (step_workspace(J, cur->dim).acc_bytes > kStaticScratchBytes) return NTT_ERR_CAPACITY;
   void* ws = nullptr;
   s = static_scratch(&ws);
   if (s != NTT_OK) return s;

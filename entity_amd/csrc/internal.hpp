@@ -22,7 +22,18 @@ ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView&
                                         float ndth, float dt, float dxm, float charge,
                                         bool deposit, float fscale,
                                         const uint32_t* tile_offsets,
-                                        int64_t ntiles, int ppt, hipStream_t st);
+                                        int64_t ntiles, int ppt, uint32_t* flags,
+                                        int64_t flag_chunks, void* recs, int64_t rec_chunks,
+                                        hipStream_t st);
+// bytes of the chunk flags of the split (push, then deposit) launches for up
+// to max_chunks chunks; `flags` above holds flag_chunks of them (null: the
+// fused kernel runs)
+int64_t split_flag_bytes(int64_t max_chunks);
+// bytes of the per-chunk box records the split push leaves for its deposit
+// (16 B per chunk, a multiple of 256 B for a multiple of 16 chunks); `recs`
+// above holds rec_chunks of them (null, or too few: the deposit finds each
+// chunk's box itself)
+int64_t split_record_bytes(int64_t max_chunks);
 // the same, as a kernel argument block written to host memory (for the
 // batched launch over the boxes of an ensemble)
 size_t     fused_args_size();

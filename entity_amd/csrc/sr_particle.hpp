@@ -595,6 +595,83 @@ __device__ __forceinline__ void deposit_esirkepov_3d(Sink& J, int G, const PStat
   }
 }
 
+// Whether a particle's merged deposit stencil keeps its O + 1 cells along
+// direction d (for_deposit's i_min of the initial and final shapes agree),
+// i.e. the particle does not need the extra slot of a stencil that shifted.
+template <int O>
+__device__ __forceinline__ bool deposit_stays(int i, float dx, int ip, float dxp) {
+  if constexpr (O == 2) {
+    const int a = (dxp < HALF) ? ip - 1 : ip;
+    const int b = (dx < HALF) ? i - 1 : i;
+    return a == b;
+  } else {
+    return i == ip;
+  }
+}
+
+// deposit_esirkepov_3d for a particle whose stencil stays in every direction
+// (deposit_stays true for d = 1, 2, 3): the same values, slot for slot, with
+// the loop bounds of that case known at compile time (3 (O + 1)^2 O slots, no
+// per-slot conditions).
+template <int O, class Sink>
+__device__ __forceinline__ void deposit_esirkepov_3d_stay(Sink& J, int G, const PState& s,
+                                                          float coeff, float inv_dt) {
+  constexpr int M = O + 1;
+  float iS1[M], fS1[M], iS2[M], fS2[M], iS3[M], fS3[M];
+  int   i1_min, i2_min, i3_min, unused;
+  shape_order<false, O>(s.ip[0], s.dxp[0], i1_min, iS1);
+  shape_order<false, O>(s.i[0], s.dx[0], unused, fS1);
+  shape_order<false, O>(s.ip[1], s.dxp[1], i2_min, iS2);
+  shape_order<false, O>(s.i[1], s.dx[1], unused, fS2);
+  shape_order<false, O>(s.ip[2], s.dxp[2], i3_min, iS3);
+  shape_order<false, O>(s.i[2], s.dx[2], unused, fS3);
+  const float Qdt = coeff * inv_dt;
+  J.set_origin(i1_min + G, i2_min + G, i3_min + G);
+  const float q3 = -Qdt * THIRD;
+  float C1[O], C2[O], C3[O];
+  C1[0] = q3 * (fS1[0] - iS1[0]);
+  C2[0] = q3 * (fS2[0] - iS2[0]);
+  C3[0] = q3 * (fS3[0] - iS3[0]);
+#pragma unroll
+  for (int a = 1; a < O; ++a) {
+    C1[a] = fmaf(q3, fS1[a] - iS1[a], C1[a - 1]);
+    C2[a] = fmaf(q3, fS2[a] - iS2[a], C2[a - 1]);
+    C3[a] = fmaf(q3, fS3[a] - iS3[a], C3[a - 1]);
+  }
+  float P1[M], Q1[M], P2[M], Q2[M];
+#pragma unroll
+  for (int a = 0; a < M; ++a) {
+    P1[a] = fmaf(HALF, fS1[a], iS1[a]);
+    Q1[a] = fmaf(HALF, iS1[a], fS1[a]);
+    P2[a] = fmaf(HALF, fS2[a], iS2[a]);
+    Q2[a] = fmaf(HALF, iS2[a], fS2[a]);
+  }
+#pragma unroll
+  for (int b = 0; b < M; ++b)
+#pragma unroll
+    for (int g = 0; g < M; ++g) {
+      const float t = fmaf(P2[b], iS3[g], Q2[b] * fS3[g]);
+#pragma unroll
+      for (int a = 0; a < O; ++a) J.add_rel(a, b, g, 0, C1[a] * t);
+    }
+#pragma unroll
+  for (int a = 0; a < M; ++a)
+#pragma unroll
+    for (int g = 0; g < M; ++g) {
+      const float t = fmaf(P1[a], iS3[g], Q1[a] * fS3[g]);
+#pragma unroll
+      for (int b = 0; b < O; ++b) J.add_rel(a, b, g, 1, C2[b] * t);
+    }
+#pragma unroll
+  for (int a = 0; a < M; ++a)
+#pragma unroll
+    for (int b = 0; b < M; ++b) {
+      const float t = fmaf(P1[a], iS2[b], Q1[a] * fS2[b]);
+#pragma unroll
+      for (int g = 0; g < O; ++g) J.add_rel(a, b, g, 2, C3[g] * t);
+    }
+}
+
 template <int D, int O, class Sink>
 __device__ __forceinline__ void deposit_particle(Sink& J, int G, const PState& s, float coeff,
                                                  float inv_dt) {

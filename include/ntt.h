@@ -492,15 +492,19 @@ typedef struct ntt_sr_step_params {
  * filter passes -> Faraday(1/2) -> Ampere -> current source (E += coeff J,
  * J /= ppc0), leaving E, B and J with consistent periodic ghosts.
  * Replaces SRPICEngine::step_forward (src/engines/srpic/srpic.hpp:65-188) for
- * a single periodic domain.  Push and deposit run fused and LDS-windowed
- * (the species arrays should be sorted by spatial tile, see ntt_sort_particles;
- * unsorted input is handled, only slower).  `buff` is scratch (its contents
- * afterwards are unspecified).  The deposit accumulates the current as 64-bit
- * fixed point in `workspace` (ntt_sr_step_workspace_size bytes), in units of
+ * a single periodic domain.  Push and deposit run LDS-windowed (the species
+ * arrays should be sorted by spatial tile, see ntt_sort_particles; unsorted
+ * input is handled, only slower): fused in one kernel in 2D, and in 3D (shape
+ * orders 1 and 2, Boris and Vay, sorted species) as a push launch and a
+ * deposit launch when the workspace holds their chunk flags and box records.
+ * `buff` is scratch (its contents afterwards are unspecified).  The deposit
+ * accumulates the current as 64-bit fixed point at the start of `workspace`
+ * (ntt_sr_step_workspace_size bytes; the 3D accumulator alone, 24 bytes per
+ * ghost-extended cell, is enough, the fused kernel then runs), in units of
  * 2^-k with 2^k the largest power of two for which every contribution of a
  * particle with |charge * weight| <= max|charge| * weight_max stays below
  * 2^30 units (each contribution is truncated to a unit), so J does not depend
- * on the order of the atomic additions. */
+ * on the order of the atomic additions, nor on the path. */
 NTT_API ntt_status ntt_sr_step(ntt_field_t*                em,
                                ntt_field_t*                cur,
                                ntt_field_t*                buff,
@@ -512,12 +516,17 @@ NTT_API ntt_status ntt_sr_step(ntt_field_t*                em,
                                uint64_t                    workspace_bytes,
                                void*                       stream);
 
-/* bytes of the workspace ntt_sr_step needs for a current field of this shape */
+/* bytes of the workspace ntt_sr_step uses for a current field of this shape:
+ * the accumulator (24 bytes per ghost-extended cell), and in 3D the split
+ * launches' chunk flags (one bit pair per cell) and box records (16 bytes per
+ * 16 cells) */
 NTT_API ntt_status ntt_sr_step_workspace_size(const ntt_field_t* cur, uint64_t* bytes);
 
-/* Largest workspace (ntt_sr_step_workspace_size) ntt_sr_step_static serves:
- * 32 MiB, i.e. 1,398,101 ghost-extended cells (2D up to 1178 x 1178 active
- * cells, 3D up to 107^3). */
+/* Size of the device array ntt_sr_step_static works in: 32 MiB.  The
+ * current accumulator must fit (1,398,101 ghost-extended cells: 2D up to
+ * 1178 x 1178 active cells, 3D up to 107^3); the 3D split launches' flags and
+ * records are used when they fit after it (3D up to 106^3), else the fused
+ * kernel runs. */
 #define NTT_SR_STEP_STATIC_MAX_BYTES (32ull << 20)
 
 /* ntt_sr_step without the workspace argument, for callers that cannot

@@ -1064,4 +1064,15 @@ extern "C" int ntt_oracle_sr_step_omp_f32(ort_field_t* em, ort_field_t* cur, ort
   return r;
 }
 
+// The same in double: the finite-difference reference of gradient checks.
+extern "C" int ntt_oracle_sr_step_omp_f64(ort_field_t* em, ort_field_t* cur, ort_field_t* buff,
+                                          ort_species_t* species, int32_t nspecies,
+                                          const ort_step_params_t* prm,
+                                          const uint8_t prtl_bc[3][2]) {
+  t_par = true;
+  const int r = sr_step<double>(em, cur, buff, species, nspecies, prm, prtl_bc);
+  t_par = false;
+  return r;
+}
+
 extern "C" int ntt_oracle_omp_threads(void) { return omp_get_max_threads(); }

@@ -203,9 +203,9 @@ def sr_step(em, cur, buff, species: list[dict], *, dim, prm: dict, bc, prec="f32
             threads=False):
     """One SRPIC step (srpic.hpp:65-188) over normalized species dicts.
     species entries: {"p": normalized particle dict, "mass", "charge", "pusher"}.
-    threads=True (FP32 only) runs the step's loops on OpenMP threads with
-    per-thread current copies (the CPU baseline); the default is the
-    sequential checker."""
+    threads=True runs the step's loops on OpenMP threads with per-thread
+    current copies (the CPU baseline in FP32, finite-difference references in
+    FP64); the default is the sequential checker."""
     arr = (OrtSpecies * len(species))()
     for k, sp in enumerate(species):
         arr[k].prtls = particles_struct(sp["p"])
@@ -222,9 +222,7 @@ def sr_step(em, cur, buff, species: list[dict], *, dim, prm: dict, bc, prec="f32
     args = (C.byref(field_struct(em, dim)), C.byref(field_struct(cur, dim)),
             C.byref(field_struct(buff, dim)), arr, C.c_int32(len(species)), C.byref(sp_), bcs)
     if threads:
-        if prec != "f32":
-            raise ValueError("the threaded step is FP32")
-        return lib().ntt_oracle_sr_step_omp_f32(*args)
+        return getattr(lib(), f"ntt_oracle_sr_step_omp_{prec}")(*args)
     return _call("sr_step", prec, *args)
 
 

@@ -125,3 +125,32 @@ def divergence_J(J, dim):
         div[1:, :, :] += J[2][1:, :, :] - J[2][:-1, :, :]
         div[0, :, :] += J[2][0, :, :]
     return div
+
+
+def oracle_field_energy(eng, nsteps, *, dux=(), dbz=0.0):
+    """L = sum |B|^2 / 2 over the active cells after `nsteps` steps of the
+    oracle in FP64 (OpenMP threads), started from the 2D engine's current
+    state with species s's u_x shifted by dux[s] and the active B_z shifted
+    by dbz: the finite-difference reference of the adjoint checks, with
+    momenta and fields in double (offsets in float, as in the reference's
+    FP64 build)."""
+    import ntt_oracle
+    p = eng.params
+    n1, n2 = p.n
+    em = eng.fields.em.detach().cpu().numpy().astype(np.float64)
+    em[5, G:G + n2, G:G + n1] += dbz
+    cur = np.zeros((3,) + em.shape[1:], np.float64)
+    sp = []
+    for k, s in enumerate(eng.species):
+        q = ntt_oracle.normalize_particles(s.to_host(), "f64")
+        if k < len(dux):
+            q["ux1"] += dux[k]
+        sp.append({"p": q, "mass": s.mass, "charge": s.charge, "pusher": s.pusher})
+    for k in range(nsteps):
+        prm = dict(dt=p.dt, correction=p.correction, omegaB0=p.omegaB0, q0=p.q0, B0=p.B0,
+                   V0=p.V0, ppc0=p.ppc0, dx=p.dx, shape_order=p.shape_order,
+                   n_filters=p.n_filters, fieldsolver_enable=1, deposit_enable=1,
+                   step=eng.step + k)
+        assert ntt_oracle.sr_step(em, cur, np.zeros_like(cur), sp, dim=2, prm=prm,
+                                  bc=((1, 1),) * 3, prec="f64", threads=True) == 0
+    return 0.5 * float((em[3:6, G:G + n2, G:G + n1] ** 2).sum())

@@ -120,10 +120,11 @@ def test_batched_scan_of_64_boxes(hip, oracle):
             assert torch.equal(eng.tile_offsets[k].to(torch.int64), off)
 
 
-def test_adjoint_workload_gradients(hip):
+def test_adjoint_workload_gradients(hip, oracle):
     from bench import (ADJ_B, ADJ_CHECKPOINT, ADJ_SIGN, ADJ_STEPS, ADJ_UD, adjoint_box)
     from entity_amd.adjoint import SRAdjoint
     from entity_amd.kernels import device_errors
+    from _helpers import oracle_field_energy
     h = 1e-3
 
     def loss(ud, b):
@@ -140,11 +141,28 @@ def test_adjoint_workload_gradients(hip):
     assert device_errors() == 0
     L = loss(ADJ_UD, ADJ_B)
     assert L64 == pytest.approx(L, rel=1e-12) and L32 == pytest.approx(L, rel=1e-12)
+    # central differences of the forward, h = 1e-3.  In FP32, L(+h) - L(-h)
+    # is about 1e-6 of L, a few float roundings of the forward, so that
+    # quotient carries ~1e-3 of rounding noise (changing only how the build
+    # contracts multiply-adds moved it by 1.6e-3); the 1e-3 bar is held
+    # against the same differences of the oracle's FP64 forward from the same
+    # initial state
     fd_u = (loss(ADJ_UD + h, ADJ_B) - loss(ADJ_UD - h, ADJ_B)) / (2 * h)
     fd_b = (loss(ADJ_UD, ADJ_B + h) - loss(ADJ_UD, ADJ_B - h)) / (2 * h)
-    print(f"L={L64:.8e} dL/dud adj64={gu64:.8e} adj32={gu32:.8e} fd={fd_u:.8e}; "
-          f"dL/db adj64={gb64:.8e} adj32={gb32:.8e} fd={fd_b:.8e}", flush=True)
-    assert gu64 == pytest.approx(fd_u, rel=1e-3)
-    assert gb64 == pytest.approx(fd_b, rel=1e-3)
+    base = adjoint_box(ADJ_UD, ADJ_B)
+    torch.cuda.synchronize()
+    L_or = oracle_field_energy(base, ADJ_STEPS)
+    fd64_u = (oracle_field_energy(base, ADJ_STEPS, dux=[s * h for s in ADJ_SIGN]) -
+              oracle_field_energy(base, ADJ_STEPS, dux=[-s * h for s in ADJ_SIGN])) / (2 * h)
+    fd64_b = (oracle_field_energy(base, ADJ_STEPS, dbz=h) -
+              oracle_field_energy(base, ADJ_STEPS, dbz=-h)) / (2 * h)
+    print(f"L={L64:.8e} (oracle FP64 {L_or:.8e}) dL/dud adj64={gu64:.8e} adj32={gu32:.8e} "
+          f"fd32={fd_u:.8e} fd64={fd64_u:.8e}; dL/db adj64={gb64:.8e} adj32={gb32:.8e} "
+          f"fd32={fd_b:.8e} fd64={fd64_b:.8e}", flush=True)
+    assert L_or == pytest.approx(L64, rel=1e-4)
+    assert gu64 == pytest.approx(fd64_u, rel=1e-3)
+    assert gb64 == pytest.approx(fd64_b, rel=1e-3)
+    assert gu64 == pytest.approx(fd_u, rel=5e-3)
+    assert gb64 == pytest.approx(fd_b, rel=5e-3)
     assert gu32 == pytest.approx(gu64, rel=1e-3)
     assert gb32 == pytest.approx(gb64, rel=1e-3)

@@ -266,34 +266,39 @@ def test_fold_conserves_and_wraps(oracle, dim):
     np.testing.assert_allclose(act.sum(axis=tuple(range(1, dim + 1))), total, atol=1e-10)
 
 
-def test_threaded_step_matches_sequential_step(oracle):
-    """The OpenMP step of the CPU baseline pushes the same particles and,
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_threaded_step_matches_sequential_step(oracle, prec):
+    """The OpenMP step (the CPU baseline in FP32, the finite-difference
+    reference of the gradient checks in FP64) pushes the same particles and,
     up to the current's summation order, gives the same fields."""
     from _helpers import field_array, random_particles
+    dt_ = np.float32 if prec == "f32" else np.float64
+    tol = 1e-5 if prec == "f32" else 1e-12
     rng = np.random.default_rng(4)
     dim, n = 3, (12, 10, 8)
-    em = (rng.standard_normal(field_array(dim, n, 6).shape) * 0.05).astype(np.float32)
+    em = (rng.standard_normal(field_array(dim, n, 6).shape) * 0.05).astype(dt_)
     out = []
     for threads in (False, True):
         sp = []
         for q in (-1.0, 1.0):
             p = random_particles(np.random.default_rng(5), dim, n, 4000, umax=0.5)
-            sp.append({"p": oracle.normalize_particles(p, "f32"), "mass": 1.0, "charge": q,
+            sp.append({"p": oracle.normalize_particles(p, prec), "mass": 1.0, "charge": q,
                        "pusher": 4})
         e = em.copy()
-        cur = field_array(dim, n, 3)
+        cur = field_array(dim, n, 3).astype(dt_)
         prm = dict(dt=0.28, correction=1.0, omegaB0=1.0, q0=0.01, B0=1.0, V0=1.0, ppc0=8.0,
                    dx=1.0, shape_order=2, n_filters=2, fieldsolver_enable=1, deposit_enable=1,
                    step=0)
         for s in range(3):
             prm["step"] = s
             assert oracle.sr_step(e, cur, np.zeros_like(cur), sp, dim=dim, prm=prm,
-                                  bc=((1, 1),) * 3, threads=threads) == 0
+                                  bc=((1, 1),) * 3, prec=prec, threads=threads) == 0
         out.append((sp, e, cur))
     (sa, ea, ja), (sb, eb, jb) = out
     for a, b in zip(sa, sb):
         for k in ("i1", "i2", "i3", "dx1", "dx2", "dx3", "ux1", "ux2", "ux3"):
-            np.testing.assert_allclose(a["p"][k], b["p"][k], rtol=1e-5, atol=1e-6, err_msg=k)
-    np.testing.assert_allclose(ja, jb, rtol=0, atol=1e-5 * np.abs(ja).max())
-    np.testing.assert_allclose(ea, eb, rtol=0, atol=1e-5 * np.abs(ea).max())
+            np.testing.assert_allclose(a["p"][k], b["p"][k], rtol=tol, atol=tol / 10,
+                                       err_msg=k)
+    np.testing.assert_allclose(ja, jb, rtol=0, atol=tol * np.abs(ja).max())
+    np.testing.assert_allclose(ea, eb, rtol=0, atol=tol * np.abs(ea).max())
     assert oracle.omp_threads() >= 1
