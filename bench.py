@@ -116,10 +116,19 @@ def init_fields(eng, seed=12345, nmodes=16, kmax=4, dB=0.5, B0=1.0):
     torch.cuda.synchronize()
 
 
+# the 3D step's push and deposit of one species (shape orders 1, 2, Boris or
+# Vay, sorted species): two launches, k_push_window then k_deposit_window,
+# timed together (HIP events around the species' launches)
+SPLIT_KERNELS = "k_push_window + k_deposit_window"
+TRAFFIC_PROFILE = "k_push_window_k_deposit_window_3d_o2_vay_512x256x256_traffic.md"
+COUNTERS_PROFILE = "k_push_window_k_deposit_window_3d_o2_vay_128x64x64_counters.md"
+
+
 def measured_traffic(cells, ppc, tile):
-    """HBM bytes per fused launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (None for another workload or tile)."""
-    path = ROOT / "profiles" / "k_push_deposit_3d_o2_vay_512x256x256_traffic.md"
+    """HBM bytes of one species' push and deposit launches from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (None for
+    another workload or tile)."""
+    path = ROOT / "profiles" / TRAFFIC_PROFILE
     want = (f"workload: 3D periodic {'x'.join(map(str, cells))} cells, e-/e+ {ppc} ppc per "
             f"species, {'x'.join(map(str, tile))} tiles")
     try:
@@ -135,11 +144,12 @@ def measured_traffic(cells, ppc, tile):
 
 
 def measured_valu_per_64(order, pusher):
-    """VALU wavefront instructions per 64 particles of the fused 3D launch,
-    from the committed rocprofv3 SQ_INSTS_VALU summary (order 2, Vay)."""
+    """VALU wavefront instructions per 64 particles of one species' push and
+    deposit launches in 3D, from the committed rocprofv3 SQ_INSTS_VALU
+    summary (order 2, Vay)."""
     if order != 2 or pusher != "vay":
         return None
-    path = ROOT / "profiles" / "k_push_deposit_3d_o2_vay_128x64x64_counters.md"
+    path = ROOT / "profiles" / COUNTERS_PROFILE
     try:
         for line in path.read_text().splitlines():
             if line.startswith("valu_per_64_particles:"):
@@ -885,8 +895,8 @@ def main():
     ms_step = 1000.0 * el / args.steps
     total_particles = 2 * npart
     value = total_particles * args.steps / el
-    # algorithmic bytes of one fused push+deposit launch (one species):
-    # per particle 102 B (3D) / 78 B (2D), per cell 36 B (E/B read + J write)
+    # algorithmic bytes of one species' push and deposit: per particle 102 B
+    # (3D) / 78 B (2D), per cell 36 B (E/B read + J write)
     b_p = 102 if dim == 3 else 78
     bytes_launch = npart * b_p + ncell * 36
     avg_ms = ms_pd.value / max(nl.value, 1)
@@ -922,23 +932,26 @@ def main():
         "cell_updates_per_s": ncell * args.steps / el,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_push_deposit (fused push + Esirkepov deposit, one species per launch)",
+            "kernel": (SPLIT_KERNELS + " (push with E/B from LDS, then Esirkepov deposit into "
+                       "LDS windows; one species per pair, timed together)")
+                      if dim == 3 and args.order in (1, 2) else
+                      "k_push_deposit (fused push + Esirkepov deposit, one species per launch)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(cells, args.ppc, tile) if args.order == 2 and
                        args.pusher == "vay" else None,
-            "traffic_source": "profiles/k_push_deposit_3d_o2_vay_512x256x256_traffic.md "
-                              "(rocprofv3 FETCH_SIZE and WRITE_SIZE of this kernel on this "
-                              "workload, copy-kernel calibrated)",
+            "traffic_source": f"profiles/{TRAFFIC_PROFILE} (rocprofv3 FETCH_SIZE and "
+                              "WRITE_SIZE of these kernels on this workload, copy-kernel "
+                              "calibrated)",
             "avg_launch_ms": avg_ms,
             "launches": int(nl.value),
             "algorithmic_bytes_per_launch": bytes_launch,
             "valu_frac": valu_frac,
-            "valu_source": "profiles/k_push_deposit_3d_o2_vay_128x64x64_counters.md "
-                           "(SQ_INSTS_VALU per 64 particles) x particles per launch / "
-                           "(256 CUs x 2.4 GHz x the launch time measured here)",
+            "valu_source": f"profiles/{COUNTERS_PROFILE} (SQ_INSTS_VALU per 64 particles, "
+                           "both launches) x particles per species / (256 CUs x 2.4 GHz x the "
+                           "time of the pair measured here)",
         },
         "device_errors": errs,
     }

@@ -326,11 +326,14 @@ __device__ __forceinline__ void sr_move(const A& F, int ngh, float ndth, float d
     float e0[3], b0[3];
     interpolate<D, O>(F, s.i[0] + ngh, s.i[1] + ngh, D == 3 ? s.i[2] + ngh : 0, s.dx[0], s.dx[1],
                       D == 3 ? s.dx[2] : 0.0f, e0, b0);
-    // transform_xyz<U, XYZ> (minkowski.h:282-289): in-plane components times dx
+    // transform_xyz<U, XYZ> (minkowski.h:282-289): in-plane components times
+    // dx (the identity for dx = 1, skipped by a branch uniform over the launch)
+    if (dxm != ONE) {
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      e0[d] *= dxm;
-      b0[d] *= dxm;
+      for (int d = 0; d < D; ++d) {
+        e0[d] *= dxm;
+        b0[d] *= dxm;
+      }
     }
     if constexpr (ALG == ALG_BORIS) boris(ndth, s.u, e0, b0);
     else vay(ndth, s.u, e0, b0);
@@ -339,12 +342,18 @@ __device__ __forceinline__ void sr_move(const A& F, int ngh, float ndth, float d
   const float dt_inv_energy = (ALG != ALG_PHOTON)
                                 ? (dt / sqrtf(ONE + ux1 * ux1 + ux2 * ux2 + ux3 * ux3))
                                 : (dt / sqrtf(ux1 * ux1 + ux2 * ux2 + ux3 * ux3));
+  // transform<d, XYZ, U>: u / dx (minkowski.h:277-281), exact without the
+  // division for dx = 1
+  float ud[3] = { s.u[0], s.u[1], s.u[2] };
+  if (dxm != ONE) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) ud[d] = s.u[d] / dxm;
+  }
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     s.ip[d]  = s.i[d];
     s.dxp[d] = s.dx[d];
-    // transform<d, XYZ, U>: u / dx (minkowski.h:277-281)
-    push_coord(s.i[d], s.dx[d], (s.u[d] / dxm) * dt_inv_energy);
+    push_coord(s.i[d], s.dx[d], ud[d] * dt_inv_energy);
   }
 }
 
