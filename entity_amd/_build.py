@@ -2,11 +2,13 @@
 
 Two builds of the same HIP sources are produced:
 
-* ``lib/libntt_hip.so`` -- the product build.  Each source expression's
-  multiply-adds become FMAs (``-ffp-contract=on``): contraction is decided per
-  expression by the front end, not per basic block by the back end, so an
-  expression rounds the same in every kernel that inlines it (the fused, split
-  and batched push kernels give bit-identical particles).
+* ``lib/libntt_hip.so`` -- the product build.  In the SR sources each source
+  expression's multiply-adds become FMAs (``-ffp-contract=on``): contraction
+  is decided per expression by the front end, not per basic block by the back
+  end, so an expression rounds the same in every kernel that inlines it (the
+  fused, split and batched push kernels give bit-identical particles).  The GR
+  sources, bound by FP64 issue, keep the back end's contraction across
+  statements (``-ffp-contract=fast``: 7% fewer instructions in the pusher).
 * ``lib/libntt_hip_strict.so`` -- identical sources compiled with
   ``-ffp-contract=off``.  Its single-precision arithmetic follows the
   reference's operation order exactly, which is what lets the parity tests
@@ -44,12 +46,17 @@ def _hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm toolchain required to build libntt_hip)")
 
 
-def _common_flags(strict: bool) -> list[str]:
+# sources whose product build contracts across statements (see above)
+FAST_CONTRACT_SOURCES = {"gr_pusher.hip", "gr_fields.hip", "gr_bcs.hip"}
+
+
+def _common_flags(strict: bool, source: str = "") -> list[str]:
+    contract = "off" if strict else ("fast" if source in FAST_CONTRACT_SOURCES else "on")
     return [
         "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
         f"--offload-arch={ARCH}",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-ffp-contract=off" if strict else "-ffp-contract=on",
+        f"-ffp-contract={contract}",
         "-Wall", "-Wno-unused-function", "-Wno-unused-result",
         f"-I{INCLUDE}", f"-I{CSRC}",
     ] + (["-DNTT_STRICT_FP=1"] if strict else [])
@@ -69,12 +76,12 @@ def build_hip(strict: bool = False, jobs: int | None = None) -> Path:
     objdir = PKG / "_build" / tag
     objdir.mkdir(parents=True, exist_ok=True)
     LIBDIR.mkdir(parents=True, exist_ok=True)
-    flags = _common_flags(strict)
     srcs = [CSRC / s for s in HIP_SOURCES if (CSRC / s).exists()]
     objs = [objdir / (s.stem + ".o") for s in srcs]
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = [ex.submit(_run, [hipcc, *flags, "-c", str(s), "-o", str(o)])
+        futs = [ex.submit(_run, [hipcc, *_common_flags(strict, s.name), "-c", str(s), "-o",
+                                 str(o)])
                 for s, o in zip(srcs, objs)]
         for f in futs:
             f.result()

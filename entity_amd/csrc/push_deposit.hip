@@ -933,13 +933,15 @@ __device__ __forceinline__ void deposit_chunk(const FusedArgs& a, int64_t chunk)
       if (tid == 63) nall_s = incl;
     }
     __syncthreads();
-    // list order: group by group.  A rank group holds at most one particle
-    // per bank class, so a wavefront over two full groups puts two lanes on
-    // each bank pair, as consecutive addresses do (a few lanes may repeat an
-    // address).  Measured on 512 x 256 x 256, order 2: 18.2 ms per launch
-    // against 23.8 ms in particle order; ordering by slot first as well (no
-    // repeated addresses) took 19.1 ms, its extra atomic costing more than it
-    // saved
+    // list order: group by group, each group in the order its particles
+    // arrived.  A rank group holds at most one particle per bank class, so a
+    // wavefront over two full groups puts two lanes on each bank pair, as
+    // consecutive addresses do (a few lanes may repeat an address).  Measured
+    // on 512 x 256 x 256, order 2, per launch: with a box pass, 18.2 ms
+    // against 23.8 ms in particle order, and 19.1 ms when ranked by stencil
+    // slot first (no repeated addresses, one more atomic); with the box from
+    // the push's record, 16.1 ms, and 17.2 ms with each group placed in
+    // bank-class order
     if (live) list[atomicAdd(&gpos[grp], 1u)] = r;
     __syncthreads();
     const int ns = static_cast<int>(nstay_s), nall = static_cast<int>(nall_s);
