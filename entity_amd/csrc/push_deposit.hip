@@ -444,47 +444,58 @@ __device__ __forceinline__ WinGeom<D> window_geometry(const FusedArgs& a, const 
 }
 
 // Stages E, B of the window (periodic images of the active cells) into LDS;
-// rows of constant width W1 keep the index math to constant divisions.
+// rows of constant width W1 keep the index math to constant divisions, and
+// each window cell's source offset is computed once for its six components,
+// whose loads are then in flight together.
 template <int D>
 __device__ __forceinline__ void stage_em(const FusedArgs& a, const WinGeom<D>& g,
                                          const int* gtab, int WM, float* em_w) {
   constexpr int W1 = Win<D>::W1, W2 = Win<D>::W2, WC = W1 * W2 * Win<D>::W3;
-  const int G = a.ngh;
-  const int rows = g.e[2] * W2 * W1;
+  const int     G    = a.ngh;
+  const int     rows = g.e[2] * W2 * W1;
+  const int64_t cs   = a.F.cstride;
+  for (int idx = threadIdx.x; idx < rows; idx += kThreads) {
+    const int l3 = idx / (W1 * W2);
+    const int r  = idx - l3 * (W1 * W2);
+    const int l2 = r / W1, l1 = r - l2 * W1;
+    if (l1 < g.e[0] && l2 < g.e[1]) {
+      const int    g3  = (D == 3) ? gtab[2 * WM + l3] + G : 0;
+      const float* src = a.F.d + (static_cast<int64_t>(gtab[l1] + G) +
+                                  static_cast<int64_t>(a.F.N1) *
+                                    (static_cast<int64_t>(gtab[WM + l2] + G) +
+                                     static_cast<int64_t>(a.F.N2) * g3));
+      float v[6];
 #pragma unroll
-  for (int c = 0; c < 6; ++c) {
-    for (int idx = threadIdx.x; idx < rows; idx += kThreads) {
-      const int l3 = idx / (W1 * W2);
-      const int r  = idx - l3 * (W1 * W2);
-      const int l2 = r / W1, l1 = r - l2 * W1;
-      if (l1 < g.e[0] && l2 < g.e[1]) {
-        const int g3 = (D == 3) ? gtab[2 * WM + l3] + G : 0;
-        em_w[c * WC + idx] = a.F(c, gtab[l1] + G, gtab[WM + l2] + G, g3);
-      }
+      for (int c = 0; c < 6; ++c) v[c] = src[c * cs];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) em_w[c * WC + idx] = v[c];
     }
   }
 }
 
 // Adds the current window to the periodic image of the HBM accumulator, one
-// 64-bit integer atomic per non-zero entry.
+// 64-bit integer atomic per non-zero entry (the target offset computed once
+// per window cell for its three components).
 template <int D>
 __device__ __forceinline__ void flush_j(const FusedArgs& a, const WinGeom<D>& g, const int* gtab,
                                         int WM, const unsigned long long* j_w) {
   constexpr int W1 = Win<D>::W1, W2 = Win<D>::W2, WC = W1 * W2 * Win<D>::W3;
   const int G = a.ngh;
   const int rows = g.e[2] * W2 * W1;
+  for (int idx = threadIdx.x; idx < rows; idx += kThreads) {
+    const int l3 = idx / (W1 * W2);
+    const int r  = idx - l3 * (W1 * W2);
+    const int l2 = r / W1, l1 = r - l2 * W1;
+    if (l1 < g.e[0] && l2 < g.e[1]) {
+      unsigned long long v[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    for (int idx = threadIdx.x; idx < rows; idx += kThreads) {
-      const int l3 = idx / (W1 * W2);
-      const int r  = idx - l3 * (W1 * W2);
-      const int l2 = r / W1, l1 = r - l2 * W1;
-      if (l1 < g.e[0] && l2 < g.e[1]) {
-        const unsigned long long v = j_w[c * WC + idx];
-        if (v != 0ull) {
-          const int g3 = (D == 3) ? gtab[2 * WM + l3] + G : 0;
-          a.J.add_fixed_at(a.J.linear(gtab[l1] + G, gtab[WM + l2] + G, g3, c), v);
-        }
+      for (int c = 0; c < 3; ++c) v[c] = j_w[c * WC + idx];
+      if ((v[0] | v[1] | v[2]) != 0ull) {
+        const int     g3  = (D == 3) ? gtab[2 * WM + l3] + G : 0;
+        const int64_t lin = a.J.linear(gtab[l1] + G, gtab[WM + l2] + G, g3, 0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          if (v[c] != 0ull) a.J.add_fixed_at(lin + a.J.cstride * c, v[c]);
       }
     }
   }
