@@ -338,33 +338,49 @@ __device__ __forceinline__ ChunkBox chunk_box(const FusedArgs& a, int64_t pbeg, 
   int lo[3] = { INT_MAX, INT_MAX, INT_MAX }, hi[3] = { INT_MIN, INT_MIN, INT_MIN };
   int rlo[3] = { INT_MAX, INT_MAX, INT_MAX }, rhi[3] = { INT_MIN, INT_MIN, INT_MIN };
   int bad = 0;
-  for (int64_t p = pbeg + tid; p < pend; p += kThreads) {
-    const int16_t tag = a.P.tag[p];
-    if (tag == NTT_TAG_ALIVE) {
-      int c[3] = { a.P.i1[p], a.P.i2[p], 0 };
-      if constexpr (D == 3) c[2] = a.P.i3[p];
-      int dl[3] = { 0, 0, 0 }, dh[3] = { 0, 0, 0 };
-      if constexpr (PREV) {
-        const int q[3] = { a.P.i1p[p], a.P.i2p[p], (D == 3) ? a.P.i3p[p] : 0 };
+  // two particles per iteration, branch-free: all their loads are in flight
+  // together, and a particle that is not alive (or past the end: it reads
+  // the last particle again) leaves the box unchanged
+  for (int64_t p0 = pbeg + tid; p0 < pend; p0 += 2 * kThreads) {
+    int16_t tag[2];
+    bool    in[2];
+    int     c[2][3], q[2][3];
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const int dd = c[d] - q[d];  // the move
-          c[d]  = wrap_prev(q[d], a.n[d]);
-          dl[d] = min(dd, 0);
-          dh[d] = max(dd, 0);
-        }
+    for (int u = 0; u < 2; ++u) {
+      const int64_t p = min(p0 + u * kThreads, pend - 1);
+      in[u]   = p0 + u * kThreads < pend;
+      tag[u]  = a.P.tag[p];
+      c[u][0] = a.P.i1[p];
+      c[u][1] = a.P.i2[p];
+      c[u][2] = (D == 3) ? a.P.i3[p] : 0;
+      q[u][0] = q[u][1] = q[u][2] = 0;
+      if constexpr (PREV) {
+        q[u][0] = a.P.i1p[p];
+        q[u][1] = a.P.i2p[p];
+        if constexpr (D == 3) q[u][2] = a.P.i3p[p];
       }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const bool alive = in[u] && tag[u] == NTT_TAG_ALIVE;
+      if (!PREV && in[u] && !alive && tag[u] != NTT_TAG_DEAD)
+        raise_device_error(a.err, NTT_DEVERR_INVALID_TAG);
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        if (c[d] < 0 || c[d] >= a.n[d]) bad = 1;
-        const int r = rotated(c[d], a.n[d]);
-        lo[d]  = min(lo[d], c[d] + dl[d]);
-        hi[d]  = max(hi[d], c[d] + dh[d]);
-        rlo[d] = min(rlo[d], r + dl[d]);
-        rhi[d] = max(rhi[d], r + dh[d]);
+        int cd = c[u][d], dl = 0, dh = 0;
+        if constexpr (PREV) {
+          const int dd = cd - q[u][d];  // the move
+          cd = wrap_prev(q[u][d], a.n[d]);
+          dl = min(dd, 0);
+          dh = max(dd, 0);
+        }
+        const int r = rotated(cd, a.n[d]);
+        bad |= (alive && static_cast<uint32_t>(cd) >= static_cast<uint32_t>(a.n[d])) ? 1 : 0;
+        lo[d]  = min(lo[d], alive ? cd + dl : INT_MAX);
+        hi[d]  = max(hi[d], alive ? cd + dh : INT_MIN);
+        rlo[d] = min(rlo[d], alive ? r + dl : INT_MAX);
+        rhi[d] = max(rhi[d], alive ? r + dh : INT_MIN);
       }
-    } else if (!PREV && tag != NTT_TAG_DEAD) {
-      raise_device_error(a.err, NTT_DEVERR_INVALID_TAG);
     }
   }
   const int lane = tid & 63, wv = tid >> 6;
