@@ -265,6 +265,61 @@ __device__ __forceinline__ void store_state(const ParticleView& P, int64_t p, co
   P.ux3[p] = s.u[2];
 }
 
+// A chunk's particle arrays from its first particle on, addressed by a 32-bit
+// byte offset from a base that is uniform over the workgroup: each access is
+// then the scalar base plus one vector offset shared by every array, with no
+// 64-bit address arithmetic per array.  Chunks of more than kMaxRel
+// particles are left to the 64-bit paths.
+constexpr int64_t kMaxRel = int64_t{ 1 } << 28;
+template <class T>
+__device__ __forceinline__ T* rel(T* base, uint32_t k) {
+  return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + k * static_cast<uint32_t>(sizeof(T)));
+}
+__device__ __forceinline__ ParticleView rebase(const ParticleView& P, int64_t p0) {
+  ParticleView R = P;
+  R.i1 += p0;  R.i2 += p0;  R.i3 += p0;
+  R.i1p += p0; R.i2p += p0; R.i3p += p0;
+  R.dx1 += p0;  R.dx2 += p0;  R.dx3 += p0;
+  R.dx1p += p0; R.dx2p += p0; R.dx3p += p0;
+  R.ux1 += p0; R.ux2 += p0; R.ux3 += p0;
+  R.w += p0;
+  R.tag += p0;
+  return R;
+}
+template <int D>
+__device__ __forceinline__ void load_state_rel(const ParticleView& R, uint32_t k, PState& s) {
+  s.i[0]  = *rel(R.i1, k);
+  s.i[1]  = *rel(R.i2, k);
+  s.i[2]  = (D == 3) ? *rel(R.i3, k) : 0;
+  s.dx[0] = *rel(R.dx1, k);
+  s.dx[1] = *rel(R.dx2, k);
+  if constexpr (D == 3) s.dx[2] = *rel(R.dx3, k);
+  s.u[0] = *rel(R.ux1, k);
+  s.u[1] = *rel(R.ux2, k);
+  s.u[2] = *rel(R.ux3, k);
+}
+template <int D>
+__device__ __forceinline__ void store_state_rel(const ParticleView& R, uint32_t k,
+                                                const PState& s) {
+  *rel(R.i1p, k)  = s.ip[0];
+  *rel(R.dx1p, k) = s.dxp[0];
+  *rel(R.i1, k)   = s.i[0];
+  *rel(R.dx1, k)  = s.dx[0];
+  *rel(R.i2p, k)  = s.ip[1];
+  *rel(R.dx2p, k) = s.dxp[1];
+  *rel(R.i2, k)   = s.i[1];
+  *rel(R.dx2, k)  = s.dx[1];
+  if constexpr (D == 3) {
+    *rel(R.i3p, k)  = s.ip[2];
+    *rel(R.dx3p, k) = s.dxp[2];
+    *rel(R.i3, k)   = s.i[2];
+    *rel(R.dx3, k)  = s.dx[2];
+  }
+  *rel(R.ux1, k) = s.u[0];
+  *rel(R.ux2, k) = s.u[1];
+  *rel(R.ux3, k) = s.u[2];
+}
+
 // Periodic frame of one dimension for a chunk.  A chunk whose particles sit
 // on both sides of the periodic boundary (e.g. cells n-1 and 0) is compact
 // in the frame rotated by h = n/2: cells c >= n - h are taken as c - n.  The
@@ -743,11 +798,12 @@ __device__ __forceinline__ void push_chunk(const FusedArgs& a, int64_t chunk) {
   const int G = a.ngh;
   int64_t   pbeg, pend;
   chunk_range(a, chunk, pbeg, pend);
-  const ChunkBox bx = chunk_box<D, false>(a, pbeg, pend, red);
-  if (!bx.any || !bx.fits) {
+  const ChunkBox bx   = chunk_box<D, false>(a, pbeg, pend, red);
+  const bool     fits = bx.fits && pend - pbeg <= kMaxRel;
+  if (!bx.any || !fits) {
     if (threadIdx.x == 0) {
       if (a.recs != nullptr) a.recs[chunk] = pack_box(bx.lo, bx.hi, bx.rot, false);
-      if (!bx.fits) set_chunk_flag(a.flags, chunk, kFlagPush);
+      if (!fits) set_chunk_flag(a.flags, chunk, kFlagPush);
     }
     return;
   }
@@ -756,25 +812,24 @@ __device__ __forceinline__ void push_chunk(const FusedArgs& a, int64_t chunk) {
   __syncthreads();
   stage_em<D>(a, g, gtab, WM, em_w);
   __syncthreads();
-  const LdsEM<D> F{ em_w, g.o[0], g.o[1], g.o[2] };
+  const LdsEM<D>     F{ em_w, g.o[0], g.o[1], g.o[2] };
+  const ParticleView R   = rebase(a.P, pbeg);
+  const uint32_t     cnt = static_cast<uint32_t>(pend - pbeg);
   int16_t ntag = NTT_TAG_DEAD;
   PState  nx;
-  {
-    const int64_t q = pbeg + threadIdx.x;
-    if (q < pend) {
-      ntag = a.P.tag[q];
-      load_state<D>(a.P, q, nx);
-    }
+  if (threadIdx.x < cnt) {
+    ntag = *rel(R.tag, threadIdx.x);
+    load_state_rel<D>(R, threadIdx.x, nx);
   }
 #pragma unroll 1
-  for (int64_t p = pbeg + threadIdx.x; p < pend; p += kThreads) {
+  for (uint32_t k = threadIdx.x; k < cnt; k += kThreads) {
     const int16_t tag = ntag;
     PState        s   = nx;
     {
-      const int64_t q = p + kThreads;
-      if (q < pend) {
-        ntag = a.P.tag[q];
-        load_state<D>(a.P, q, nx);
+      const uint32_t q = k + kThreads;
+      if (q < cnt) {
+        ntag = *rel(R.tag, q);
+        load_state_rel<D>(R, q, nx);
       }
     }
     if (tag != NTT_TAG_ALIVE) continue;
@@ -798,7 +853,7 @@ __device__ __forceinline__ void push_chunk(const FusedArgs& a, int64_t chunk) {
       ext |= ((s.i[d] < bx.lo[d]) ? 1u : 0u) << (2 * d) | ((s.i[d] > bx.hi[d]) ? 2u : 0u) << (2 * d);
     if (ext != 0u) atomicOr(&moved, ext);
     sr_wrap_periodic<D>(a.n, s);
-    store_state<D>(a.P, p, s);
+    store_state_rel<D>(R, k, s);
   }
   if (a.recs == nullptr) return;
   // the box of the cells before and after the push (a particle moves at most
@@ -882,16 +937,18 @@ __device__ __forceinline__ void deposit_chunk(const FusedArgs& a, int64_t chunk)
   }
   if (!have) bx = chunk_box<D, true>(a, pbeg, pend, red);
   if (!bx.any) return;
-  if (!bx.fits) {
+  if (!bx.fits || pend - pbeg > kMaxRel) {
     if (tid == 0) set_chunk_flag(a.flags, chunk, kFlagDeposit);
     return;
   }
   const WinGeom<D> g = window_geometry<D>(a, bx, gtab, WM);
   for (int idx = tid; idx < 3 * WC; idx += kThreads) j_w[idx] = 0ull;
   if (tid == 0) heavy = 0u;
-  LdsJ<D> jw{ j_w, g.o[0], g.o[1], g.o[2] };
+  LdsJ<D>            jw{ j_w, g.o[0], g.o[1], g.o[2] };
+  const ParticleView R   = rebase(a.P, pbeg);
+  const uint32_t     cnt = static_cast<uint32_t>(pend - pbeg);
 #pragma unroll 1
-  for (int64_t sb = pbeg; sb < pend; sb += kThreads) {
+  for (uint32_t sb = 0; sb < cnt; sb += kThreads) {
     // (the previous segment's list and counts are read before the barrier
     // after the next classification, and the group counters are not read
     // by its deposits)
@@ -902,16 +959,16 @@ __device__ __forceinline__ void deposit_chunk(const FusedArgs& a, int64_t chunk)
     __syncthreads();
     // load one particle per thread; its record, class and the bank class of
     // its stencil base (the window's 64-bit slot of the lowest cell, mod 32)
-    const int64_t p    = sb + tid;
-    bool          live = false, stay = true;
-    DepRecord     r;
-    int           grp = 0;
-    if (p < pend && a.P.tag[p] == NTT_TAG_ALIVE) {
-      const int   c[3]  = { a.P.i1[p], a.P.i2[p], a.P.i3[p] };
-      const int   cp[3] = { a.P.i1p[p], a.P.i2p[p], a.P.i3p[p] };
-      const float x[3]  = { a.P.dx1[p], a.P.dx2[p], a.P.dx3[p] };
-      const float xp[3] = { a.P.dx1p[p], a.P.dx2p[p], a.P.dx3p[p] };
-      const float coeff = a.P.w[p] * a.charge * a.fscale;
+    const uint32_t k    = sb + tid;
+    bool           live = false, stay = true;
+    DepRecord      r;
+    int            grp = 0;
+    if (k < cnt && *rel(R.tag, k) == NTT_TAG_ALIVE) {
+      const int   c[3]  = { *rel(R.i1, k), *rel(R.i2, k), *rel(R.i3, k) };
+      const int   cp[3] = { *rel(R.i1p, k), *rel(R.i2p, k), *rel(R.i3p, k) };
+      const float x[3]  = { *rel(R.dx1, k), *rel(R.dx2, k), *rel(R.dx3, k) };
+      const float xp[3] = { *rel(R.dx1p, k), *rel(R.dx2p, k), *rel(R.dx3p, k) };
+      const float coeff = *rel(R.w, k) * a.charge * a.fscale;
       live = true;
       uint32_t cells = 0u;
       int      slot = 0, stride = 1;
