@@ -15,13 +15,23 @@ namespace ntt {
 enum PusherAlg : int { ALG_BORIS = 0, ALG_VAY = 1, ALG_PHOTON = 2 };
 
 // ---- shapes: particle_shapes.hpp:544-588 --------------------------------
-template <bool STAG, int O>
+// SEL: the two branches on di < 1/2 taken as selects of their operands (the
+// same operations on the same values, so the same results) rather than as
+// divergent branches.  Measured at 512 x 256 x 256, order 2: the deposit's
+// stencils run faster with selects (15.4 against 15.6 ms per launch), the
+// push's field gather with branches (15.9 against 16.7 ms).
+template <bool STAG, int O, bool SEL = false>
 __device__ __forceinline__ void shape_order(int i, float di, int& i_min, float* S) {
   if constexpr (O == 1) {
     if constexpr (!STAG) {
       i_min = i;
       S[0]  = ONE - di;
       S[1]  = di;
+    } else if constexpr (SEL) {
+      const bool lo = di < HALF;
+      i_min = lo ? i - 1 : i;
+      S[0]  = (lo ? HALF : THREE_HALFS) - di;
+      S[1]  = ONE - S[0];
     } else {
       if (di < HALF) {
         i_min = i - 1;
@@ -34,7 +44,14 @@ __device__ __forceinline__ void shape_order(int i, float di, int& i_min, float* 
     }
   } else if constexpr (O == 2) {
     if constexpr (!STAG) {
-      if (di < HALF) {
+      if constexpr (SEL) {
+        const bool  lo = di < HALF;
+        const float t  = (lo ? HALF : THREE_HALFS) - di;  // 1/2 - di or 3/2 - di
+        const float v  = lo ? di : ONE - di;               // di or 1 - di
+        i_min = lo ? i - 1 : i;
+        S[0]  = HALF * (t * t);
+        S[1]  = THREE_FOURTHS - v * v;
+      } else if (di < HALF) {
         i_min = i - 1;
         S[0]  = HALF * ((HALF - di) * (HALF - di));
         S[1]  = THREE_FOURTHS - di * di;
@@ -60,8 +77,8 @@ __device__ __forceinline__ void for_deposit(int i_init, float di_init, int i_fin
                                             int& i_min, int& di, float* iS, float* fS) {
   int   a, b;
   float iS_[O + 1], fS_[O + 1];
-  shape_order<false, O>(i_init, di_init, a, iS_);
-  shape_order<false, O>(i_fin, di_fin, b, fS_);
+  shape_order<false, O, true>(i_init, di_init, a, iS_);
+  shape_order<false, O, true>(i_fin, di_fin, b, fS_);
   if (a < b) {
     i_min = a;
     di    = O + 1;
@@ -628,12 +645,12 @@ __device__ __forceinline__ void deposit_esirkepov_3d_stay(Sink& J, int G, const 
   constexpr int M = O + 1;
   float iS1[M], fS1[M], iS2[M], fS2[M], iS3[M], fS3[M];
   int   i1_min, i2_min, i3_min, unused;
-  shape_order<false, O>(s.ip[0], s.dxp[0], i1_min, iS1);
-  shape_order<false, O>(s.i[0], s.dx[0], unused, fS1);
-  shape_order<false, O>(s.ip[1], s.dxp[1], i2_min, iS2);
-  shape_order<false, O>(s.i[1], s.dx[1], unused, fS2);
-  shape_order<false, O>(s.ip[2], s.dxp[2], i3_min, iS3);
-  shape_order<false, O>(s.i[2], s.dx[2], unused, fS3);
+  shape_order<false, O, true>(s.ip[0], s.dxp[0], i1_min, iS1);
+  shape_order<false, O, true>(s.i[0], s.dx[0], unused, fS1);
+  shape_order<false, O, true>(s.ip[1], s.dxp[1], i2_min, iS2);
+  shape_order<false, O, true>(s.i[1], s.dx[1], unused, fS2);
+  shape_order<false, O, true>(s.ip[2], s.dxp[2], i3_min, iS3);
+  shape_order<false, O, true>(s.i[2], s.dx[2], unused, fS3);
   const float Qdt = coeff * inv_dt;
   J.set_origin(i1_min + G, i2_min + G, i3_min + G);
   const float q3 = -Qdt * THIRD;
