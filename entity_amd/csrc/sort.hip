@@ -28,16 +28,23 @@ constexpr int kScanTile  = kB * kScanItems;   // elements per scan block
 
 struct TileKey {
   int32_t n[3], t[3], nt[3];
+  int32_t sh[3];  // log2 t[d] when t[d] is a power of two, else -1
   int32_t dim;
   uint32_t dead_key;
+  // cell / tile edge, unsigned (the cell is clamped to [0, n) first): a
+  // shift for power-of-two edges (a branch uniform over the launch)
+  __device__ __forceinline__ uint32_t tdiv(int c, int d) const {
+    const uint32_t x = static_cast<uint32_t>(c);
+    return sh[d] >= 0 ? x >> sh[d] : x / static_cast<uint32_t>(t[d]);
+  }
   __device__ __forceinline__ uint32_t operator()(int i1, int i2, int i3) const {
     // particles outside [0, n) are clamped to the edge tile
     i1 = min(max(i1, 0), n[0] - 1);
     i2 = min(max(i2, 0), n[1] - 1);
-    const uint32_t a = i1 / t[0], b = i2 / t[1];
+    const uint32_t a = tdiv(i1, 0), b = tdiv(i2, 1);
     if (dim == 2) return a + static_cast<uint32_t>(nt[0]) * b;
     i3 = min(max(i3, 0), n[2] - 1);
-    const uint32_t c = i3 / t[2];
+    const uint32_t c = tdiv(i3, 2);
     return a + static_cast<uint32_t>(nt[0]) * (b + static_cast<uint32_t>(nt[1]) * c);
   }
 };
@@ -285,19 +292,40 @@ struct ArrayPairs {
   int32_t     count;
 };
 
-// the same by destination: dst[k][inv[p]] = src[k][p] (coalesced reads)
+// the same by destination: dst[k][inv[p]] = src[k][p] (coalesced reads),
+// kPermItems particles per thread, so that each array's loads of a thread are
+// in flight together
+constexpr int kPermItems = 4;
+template <typename T>
+__device__ __forceinline__ void scatter_items(void* dst, const void* src, const int64_t* p,
+                                              const uint32_t* q, const bool* act) {
+  T v[kPermItems];
+#pragma unroll
+  for (int r = 0; r < kPermItems; ++r)
+    if (act[r]) v[r] = static_cast<const T*>(src)[p[r]];
+#pragma unroll
+  for (int r = 0; r < kPermItems; ++r)
+    if (act[r]) static_cast<T*>(dst)[q[r]] = v[r];
+}
 __global__ __launch_bounds__(kB) void k_permute_scatter(ArrayPairs b,
                                                         const uint32_t* __restrict__ inv,
                                                         int64_t n) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
-  if (p >= n) return;
-  const int64_t q = inv[p];
+  const int64_t p0 = static_cast<int64_t>(blockIdx.x) * (kB * kPermItems) + threadIdx.x;
+  int64_t       p[kPermItems];
+  uint32_t      q[kPermItems];
+  bool          act[kPermItems];
+#pragma unroll
+  for (int r = 0; r < kPermItems; ++r) {
+    p[r]   = p0 + r * kB;
+    act[r] = p[r] < n;
+    q[r]   = act[r] ? inv[p[r]] : 0u;
+  }
 #pragma unroll
   for (int k = 0; k < kDirect; ++k) {
     if (k >= b.count) break;
-    if (b.bytes[k] == 2) gather_one<int16_t>(b.dst[k], b.src[k], q, p);
-    else if (b.bytes[k] == 4) gather_one<int32_t>(b.dst[k], b.src[k], q, p);
-    else gather_one<int64_t>(b.dst[k], b.src[k], q, p);
+    if (b.bytes[k] == 2) scatter_items<int16_t>(b.dst[k], b.src[k], p, q, act);
+    else if (b.bytes[k] == 4) scatter_items<int32_t>(b.dst[k], b.src[k], p, q, act);
+    else scatter_items<int64_t>(b.dst[k], b.src[k], p, q, act);
   }
 }
 
@@ -357,6 +385,10 @@ ntt_status make_layout(int64_t npart, int32_t dim, const int32_t* n, const int32
     ntiles *= K->nt[d];
   }
   for (int d = dim; d < 3; ++d) { K->n[d] = 1; K->t[d] = 1; K->nt[d] = 1; }
+  for (int d = 0; d < 3; ++d) {
+    const int32_t t = K->t[d];
+    K->sh[d] = (t & (t - 1)) == 0 ? __builtin_ctz(static_cast<unsigned>(t)) : -1;
+  }
   if (ntiles >= 0xFFFFFFFFll) return NTT_ERR_ARG;
   K->dim      = dim;
   K->dead_key = static_cast<uint32_t>(ntiles);
@@ -763,7 +795,9 @@ ntt_status sort_species(ntt_particles_t* src, ntt_particles_t* dst, int32_t dim,
         ++b.count;
       }
       if (b.count == kDirect || (k == kSortArrays && b.count > 0)) {
-        if (NTT_SORT_SCATTER) k_permute_scatter<<<blocks_for(np), kB, 0, st>>>(b, keys, np);
+        if (NTT_SORT_SCATTER)
+          k_permute_scatter<<<blocks_for((np + kPermItems - 1) / kPermItems), kB, 0, st>>>(
+            b, keys, np);
         else k_permute_direct<<<blocks_for(np), kB, 0, st>>>(b, perm, np);
         b = ArrayPairs{};
       }
