@@ -115,7 +115,7 @@ __device__ __forceinline__ void ranked_keys(int64_t base, int64_t n, int64_t gof
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     win[r]   = act[r] && k[r] - kref < static_cast<uint32_t>(kHist);
-    loc[r]   = aggregated_increment(hist, win[r] ? k[r] - kref : 0u, win[r]);
+    loc[r]   = win[r] ? atomicAdd(hist + (k[r] - kref), 1u) : 0u;
     const uint32_t g = aggregated_increment(counts, k[r], act[r] && !win[r]);
     if (act[r] && !win[r]) rank[gofs + base + r * kB + threadIdx.x] = g;
   }
