@@ -229,6 +229,7 @@ SIGNATURES: dict[str, tuple] = {
                               _P(NttSrStepParams), _P(NttMinkowski), C.c_void_p, C.c_uint64,
                               C.c_void_p]),
     "ntt_sr_step_workspace_size": (C.c_int, [_P(NttField), _P(C.c_uint64)]),
+    "ntt_sr_step_workspace_size_species": (C.c_int, [_P(NttField), C.c_int32, _P(C.c_uint64)]),
     "ntt_sr_step_static": (C.c_int, [_P(NttField), _P(NttField), _P(NttField), _P(NttSpecies),
                                      C.c_int32, _P(NttSrStepParams), _P(NttMinkowski),
                                      C.c_void_p]),

@@ -2,6 +2,7 @@
 // counterpart of SRPICEngine::step_forward, src/engines/srpic/srpic.hpp:65-188,
 // with the drivers of src/engines/srpic/{fieldsolvers.h, currents.h,
 // particle_pusher.h}).  All launches are enqueued on one stream.
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <vector>
@@ -64,7 +65,10 @@ namespace {
 // of `cur`, 64-bit entries), then, in 3D, the chunk flags of the split push
 // and deposit launches for up to one chunk per cell plus the tail (a sort
 // tile holds at least one cell), then the push's per-chunk box records for
-// tiles of 16 cells or more (smaller tiles run without them).
+// tiles of 16 cells or more (smaller tiles run without them): one set shared
+// by the species, or one set per species when the workspace holds them
+// (ntt_sr_step_workspace_size_species), which lets each species' push start
+// from the boxes its previous push recorded.
 struct StepWorkspace {
   uint64_t acc_bytes, flag_bytes, rec_bytes;
   int64_t  flag_chunks, rec_chunks;
@@ -84,14 +88,19 @@ StepWorkspace step_workspace(const FieldView& J, int dim) {
 
 }  // namespace
 
-extern "C" ntt_status ntt_sr_step_workspace_size(const ntt_field_t* cur, uint64_t* bytes) {
-  if (bytes == nullptr) return NTT_ERR_ARG;
+extern "C" ntt_status ntt_sr_step_workspace_size_species(const ntt_field_t* cur,
+                                                         int32_t nspecies, uint64_t* bytes) {
+  if (bytes == nullptr || nspecies < 0) return NTT_ERR_ARG;
   FieldView J;
   ntt_status s = make_field_view(cur, 3, &J);
   if (s != NTT_OK) return s;
   const StepWorkspace w = step_workspace(J, cur->dim);
-  *bytes = w.acc_bytes + w.flag_bytes + w.rec_bytes;
+  *bytes = w.acc_bytes + w.flag_bytes + w.rec_bytes * static_cast<uint64_t>(std::max(nspecies, 1));
   return NTT_OK;
+}
+
+extern "C" ntt_status ntt_sr_step_workspace_size(const ntt_field_t* cur, uint64_t* bytes) {
+  return ntt_sr_step_workspace_size_species(cur, 1, bytes);
 }
 
 namespace {
@@ -113,6 +122,7 @@ struct StepPlan {
   int64_t             flag_chunks;
   void*               recs;              // split-path box records (null: none)
   int64_t             rec_chunks;
+  uint64_t            rec_stride;        // bytes between species' record sets (0: shared)
   int                 D;
   float               dt, dxh;
   bool                ext;               // extended Faraday stencil
@@ -147,13 +157,20 @@ ntt_status plan_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t* buff,
   P->flag_chunks = 0;
   P->recs = nullptr;
   P->rec_chunks = 0;
+  P->rec_stride = 0;
   if (workspace != nullptr && wsz.flag_bytes > 0 &&
       workspace_bytes >= wsz.acc_bytes + wsz.flag_bytes) {
     P->flags = reinterpret_cast<uint32_t*>(static_cast<char*>(workspace) + wsz.acc_bytes);
     P->flag_chunks = wsz.flag_chunks;
-    if (workspace_bytes >= wsz.acc_bytes + wsz.flag_bytes + wsz.rec_bytes) {
-      P->recs = static_cast<char*>(workspace) + wsz.acc_bytes + wsz.flag_bytes;
+    const uint64_t base = wsz.acc_bytes + wsz.flag_bytes;
+    if (workspace_bytes >= base + wsz.rec_bytes) {
+      P->recs = static_cast<char*>(workspace) + base;
       P->rec_chunks = wsz.rec_chunks;
+      // one record set per species (the push reuses its own boxes), or one
+      // set shared by the species (every push runs its box pass)
+      if (nspecies == 1 ||
+          workspace_bytes >= base + wsz.rec_bytes * static_cast<uint64_t>(nspecies))
+        P->rec_stride = wsz.rec_bytes;
     }
   }
   P->D = D;
@@ -250,10 +267,14 @@ ntt_status sr_step(ntt_field_t* em, ntt_field_t* cur, ntt_field_t* buff, ntt_spe
     const int ppt = prm->particles_per_thread > 0 ? prm->particles_per_thread : 4;
     std::pair<hipEvent_t, hipEvent_t> ev;
     const bool timed = timed_ok && prof_begin(st, &ev);
+    void* recs = (P.recs != nullptr && P.rec_stride > 0)
+                   ? static_cast<char*>(P.recs) + P.rec_stride * static_cast<uint64_t>(sp)
+                   : P.recs;
     s = launch_push_deposit_periodic(p.pv, E, J, P.acc, P.D, prm->shape_order, p.alg, p.ndth,
                                      P.dt, metric->dx, species[sp].charge, p.dep, P.fscale,
                                      species[sp].tile_offsets, species[sp].ntiles, ppt,
-                                     P.flags, P.flag_chunks, P.recs, P.rec_chunks, st);
+                                     P.flags, P.flag_chunks, recs, P.rec_chunks,
+                                     P.rec_stride > 0, st);
     if (timed) prof_end(st, ev);
     if (s != NTT_OK) return s;
   }

@@ -24,7 +24,7 @@ ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView&
                                         const uint32_t* tile_offsets,
                                         int64_t ntiles, int ppt, uint32_t* flags,
                                         int64_t flag_chunks, void* recs, int64_t rec_chunks,
-                                        hipStream_t st);
+                                        bool reuse_boxes, hipStream_t st);
 // bytes of the chunk flags of the split (push, then deposit) launches for up
 // to max_chunks chunks; `flags` above holds flag_chunks of them (null: the
 // fused kernel runs)

@@ -191,12 +191,15 @@ struct FusedArgs {
   uint32_t*       err;
   uint32_t*       flags; // split path: 2 bits per chunk (kFlagPush, kFlagDeposit), or null
   uint4*          recs;  // split path: per chunk, the box the push leaves (or null)
+  int32_t         reuse; // split path: recs holds the records of this species' last push
 };
 
 // A chunk's box record, written by the split path's push for its deposit:
 // the box of every live particle's cells before and after the push, in the
 // push's frame (x, y, z: lo | hi << 16 as 16-bit integers; w: bit d set when
-// dimension d is rotated, bit 31 when the record is valid).
+// dimension d is rotated, bit 30 when the box is in a frame where each cell
+// has one representative (the next push can reuse it), bit 31 when the record
+// is valid).
 __device__ __forceinline__ uint4 pack_box(const int* lo, const int* hi, const bool* rot,
                                           bool valid) {
   uint4 r;
@@ -212,6 +215,10 @@ __device__ __forceinline__ uint4 pack_box(const int* lo, const int* hi, const bo
 // whole (kFlagPush); one pushed from the window whose deposit cannot run from
 // the current window is left to it for the deposit only (kFlagDeposit).
 constexpr uint32_t kFlagPush = 1u, kFlagDeposit = 2u;
+// i1_prev of a particle that the window push left for the fallback launch
+// (its cell lay outside a reused box): the deposit launch skips it, the
+// fallback pushes and deposits it
+constexpr int32_t kUnpushed = INT32_MIN;
 constexpr int64_t split_flag_words(int64_t nchunks) { return (nchunks + 15) / 16; }
 __device__ __forceinline__ uint32_t chunk_flags(const uint32_t* f, int64_t c) {
   return (f[c >> 4] >> (static_cast<uint32_t>(c & 15) * 2u)) & 3u;
@@ -342,6 +349,28 @@ __device__ __forceinline__ int wrap_prev(int q, int n) {
   return q < 0 ? q + n : (q >= n ? q - n : q);
 }
 
+// Moves a box [lo, hi] of unwrapped cells (one dimension) into a frame in
+// which each of its cells has one representative: the plain frame [0, n) or
+// the rotated one [-(n/2), n - n/2), shifted by a multiple of n.  A cell c in
+// [0, n) then maps into the box as the frame maps it (rotated: c - n for c >=
+// n - n/2).  False when the box fits neither (wider than the frames allow).
+__device__ __forceinline__ bool canonical_box(int& lo, int& hi, bool& rot, int n) {
+  const int h = n >> 1;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const bool r = (k < 3) ? rot : !rot;
+    const int  s = (k % 3 == 0) ? 0 : ((k % 3 == 1) ? -n : n);
+    const int  l = lo + s, u = hi + s;
+    if (r ? (l >= -h && u < n - h) : (l >= 0 && u < n)) {
+      lo  = l;
+      hi  = u;
+      rot = r;
+      return true;
+    }
+  }
+  return false;
+}
+
 // XCD-aware chunk order: blockIdx.x % 8 is the XCD a workgroup runs on; each
 // XCD gets a contiguous run of chunks, whose windows overlap in its L2.
 // Returns -1 for the padding blocks of the last run.
@@ -417,7 +446,7 @@ __device__ __forceinline__ ChunkBox chunk_box(const FusedArgs& a, int64_t pbeg, 
     }
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      const bool alive = in[u] && tag[u] == NTT_TAG_ALIVE;
+      const bool alive = in[u] && tag[u] == NTT_TAG_ALIVE && (!PREV || q[u][0] != kUnpushed);
       if (!PREV && in[u] && !alive && tag[u] != NTT_TAG_DEAD)
         raise_device_error(a.err, NTT_DEVERR_INVALID_TAG);
 #pragma unroll
@@ -574,14 +603,16 @@ __device__ __forceinline__ void flush_j(const FusedArgs& a, const WinGeom<D>& g,
 
 // Push (and deposit) of a chunk's particles straight from HBM: gather from
 // the ghost-extended em array, deposit with global atomics (periodic fold per
-// index).  For chunks whose box does not fit the window.
-template <int D, int O, int ALG>
+// index).  For chunks whose box does not fit the window; MARKED: only the
+// particles the window push left (i1_prev == kUnpushed).
+template <int D, int O, int ALG, bool MARKED = false>
 __device__ __forceinline__ void hbm_push_deposit(const FusedArgs& a, int64_t pbeg, int64_t pend) {
   const int  G      = a.ngh;
   const bool do_dep = a.charge != ZERO;
 #pragma unroll 1
   for (int64_t p = pbeg + threadIdx.x; p < pend; p += kThreads) {
     if (a.P.tag[p] != NTT_TAG_ALIVE) continue;
+    if (MARKED && a.P.i1p[p] != kUnpushed) continue;
     PState s;
     s.tag = NTT_TAG_ALIVE;
     load_state<D>(a.P, p, s);
@@ -612,13 +643,14 @@ __device__ __forceinline__ void hbm_push_deposit(const FusedArgs& a, int64_t pbe
 
 // Deposit of a chunk's pushed particles straight to HBM (global atomics,
 // periodic fold per index), from the state the push stored.  Particles whose
-// offset is outside [0, 1] were skipped (and reported) by the push.
+// offset is outside [0, 1] were skipped (and reported) by the push, and those
+// it left for the fallback (kUnpushed) are not pushed yet.
 template <int D, int O>
 __device__ __forceinline__ void hbm_deposit(const FusedArgs& a, int64_t pbeg, int64_t pend) {
   const int G = a.ngh;
 #pragma unroll 1
   for (int64_t p = pbeg + threadIdx.x; p < pend; p += kThreads) {
-    if (a.P.tag[p] != NTT_TAG_ALIVE) continue;
+    if (a.P.tag[p] != NTT_TAG_ALIVE || a.P.i1p[p] == kUnpushed) continue;
     PState s;
     s.tag   = NTT_TAG_ALIVE;
     s.i[0]  = a.P.i1[p];
@@ -794,12 +826,36 @@ __device__ __forceinline__ void push_chunk(const FusedArgs& a, int64_t chunk) {
   __shared__ float    em_w[6 * W1 * W2 * W3];
   __shared__ int      red[4 * 3 + 1][kWaves];
   __shared__ int      gtab[3 * WM];
-  __shared__ uint32_t moved;
+  __shared__ uint32_t moved, skipped;
   const int G = a.ngh;
   int64_t   pbeg, pend;
   chunk_range(a, chunk, pbeg, pend);
-  const ChunkBox bx   = chunk_box<D, false>(a, pbeg, pend, red);
-  const bool     fits = bx.fits && pend - pbeg <= kMaxRel;
+  // the box: this species' record of the previous push when it fits the
+  // window (a superset of the cells now: the cells before and after that
+  // push), else the box pass.  A particle outside a reused box is left to
+  // the fallback launch, so a stale record costs time, never correctness.
+  ChunkBox bx;
+  bool     reuse = false;
+  if (a.reuse && pend - pbeg <= kMaxRel) {
+    const uint4 r = a.recs[chunk];
+    if ((r.w >> 30) == 3u) {
+      const uint32_t xs[3] = { r.x, r.y, r.z };
+      bx.any  = true;
+      bx.fits = true;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        bx.lo[d]  = static_cast<int16_t>(xs[d] & 0xFFFFu);
+        bx.hi[d]  = static_cast<int16_t>(xs[d] >> 16);
+        bx.rot[d] = (r.w >> d) & 1u;
+        if (d < D && (bx.hi[d] - bx.lo[d] + 4 > Win<D>::W[d] || bx.lo[d] < -a.n[d] ||
+                      bx.hi[d] >= 2 * a.n[d]))
+          bx.fits = false;
+      }
+      reuse = bx.fits;
+    }
+  }
+  if (!reuse) bx = chunk_box<D, false>(a, pbeg, pend, red);
+  const bool fits = bx.fits && pend - pbeg <= kMaxRel;
   if (!bx.any || !fits) {
     if (threadIdx.x == 0) {
       if (a.recs != nullptr) a.recs[chunk] = pack_box(bx.lo, bx.hi, bx.rot, false);
@@ -807,7 +863,10 @@ __device__ __forceinline__ void push_chunk(const FusedArgs& a, int64_t chunk) {
     }
     return;
   }
-  if (threadIdx.x == 0) moved = 0u;
+  if (threadIdx.x == 0) {
+    moved   = 0u;
+    skipped = 0u;
+  }
   const WinGeom<D> g = window_geometry<D>(a, bx, gtab, WM);
   __syncthreads();
   stage_em<D>(a, g, gtab, WM, em_w);
@@ -832,7 +891,10 @@ __device__ __forceinline__ void push_chunk(const FusedArgs& a, int64_t chunk) {
         load_state_rel<D>(R, q, nx);
       }
     }
-    if (tag != NTT_TAG_ALIVE) continue;
+    if (tag != NTT_TAG_ALIVE) {
+      if (reuse && tag != NTT_TAG_DEAD) raise_device_error(a.err, NTT_DEVERR_INVALID_TAG);
+      continue;
+    }
     s.tag   = NTT_TAG_ALIVE;
     bool ok = true;
 #pragma unroll
@@ -843,6 +905,17 @@ __device__ __forceinline__ void push_chunk(const FusedArgs& a, int64_t chunk) {
     if (!ok) {
       raise_device_error(a.err, NTT_DEVERR_OUT_OF_BOUNDS);
       continue;
+    }
+    if (reuse) {
+      bool in = true;
+#pragma unroll
+      for (int d = 0; d < D; ++d)
+        if (s.i[d] < bx.lo[d] || s.i[d] > bx.hi[d]) in = false;
+      if (!in) {
+        *rel(R.i1p, k) = kUnpushed;
+        skipped        = 1u;
+        continue;
+      }
     }
     sr_move<D, O, ALG>(F, G, a.ndth, a.dt, a.dxm, s);
     // bit 2d: the particle left the box below in dimension d, 2d + 1: above
@@ -857,16 +930,23 @@ __device__ __forceinline__ void push_chunk(const FusedArgs& a, int64_t chunk) {
   }
   if (a.recs == nullptr) return;
   // the box of the cells before and after the push (a particle moves at most
-  // one cell), for the deposit launch
+  // one cell), for the deposit launch; a chunk with particles left outside
+  // the box goes to the fallback launch for those (kFlagPush, valid record)
   __syncthreads();
   if (threadIdx.x == 0) {
-    int lo[3], hi[3];
+    if (skipped != 0u) set_chunk_flag(a.flags, chunk, kFlagPush);
+    int  lo[3], hi[3];
+    bool rot[3], canon = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      lo[d] = bx.lo[d] - static_cast<int>((moved >> (2 * d)) & 1u);
-      hi[d] = bx.hi[d] + static_cast<int>((moved >> (2 * d + 1)) & 1u);
+      lo[d]  = bx.lo[d] - static_cast<int>((moved >> (2 * d)) & 1u);
+      hi[d]  = bx.hi[d] + static_cast<int>((moved >> (2 * d + 1)) & 1u);
+      rot[d] = bx.rot[d];
+      if (d < D && !canonical_box(lo[d], hi[d], rot[d], a.n[d])) canon = false;
     }
-    a.recs[chunk] = pack_box(lo, hi, bx.rot, true);
+    uint4 r = pack_box(lo, hi, rot, true);
+    if (canon) r.w |= 0x40000000u;
+    a.recs[chunk] = r;
   }
 }
 
@@ -911,16 +991,18 @@ __device__ __forceinline__ void deposit_chunk(const FusedArgs& a, int64_t chunk)
   __shared__ uint32_t           gsz[kGroups];
   __shared__ uint32_t           gpos[kGroups];
   __shared__ uint32_t           nstay_s, nall_s, heavy;
-  if (chunk_flags(a.flags, chunk) & kFlagPush) return;
   const int tid = threadIdx.x, lane = tid & 63;
   const int G   = a.ngh;
+  const uint4 r = (a.recs != nullptr) ? a.recs[chunk] : make_uint4(0u, 0u, 0u, 0u);
+  // a chunk the push left whole (no valid record) is the fallback's; one it
+  // left in part is deposited here without the particles it left
+  if ((chunk_flags(a.flags, chunk) & kFlagPush) && !(r.w >> 31)) return;
   int64_t   pbeg, pend;
   chunk_range(a, chunk, pbeg, pend);
   // the box from the push's record when it has one that fits, else pass 1
   ChunkBox bx;
   bool     have = false;
   if (a.recs != nullptr) {
-    const uint4 r = a.recs[chunk];
     if (r.w >> 31) {
       const uint32_t xs[3] = { r.x, r.y, r.z };
       bx.any  = true;
@@ -969,7 +1051,7 @@ __device__ __forceinline__ void deposit_chunk(const FusedArgs& a, int64_t chunk)
       const float x[3]  = { *rel(R.dx1, k), *rel(R.dx2, k), *rel(R.dx3, k) };
       const float xp[3] = { *rel(R.dx1p, k), *rel(R.dx2p, k), *rel(R.dx3p, k) };
       const float coeff = *rel(R.w, k) * a.charge * a.fscale;
-      live = true;
+      live = cp[0] != kUnpushed;
       uint32_t cells = 0u;
       int      slot = 0, stride = 1;
 #pragma unroll
@@ -1075,9 +1157,11 @@ void k_deposit_window(FusedArgs a) {
   deposit_chunk<D, O>(a, chunk);
 }
 
-// The chunks the window kernels left: push and deposit from HBM (kFlagPush),
-// or the deposit of the pushed particles (kFlagDeposit).  One workgroup per
-// flag word (16 chunks), which exits at once when the word is clear.
+// The chunks the window kernels left: push and deposit from HBM (kFlagPush:
+// the whole chunk, or with a valid record only the particles the push left
+// outside its reused box), or the deposit of the pushed particles
+// (kFlagDeposit).  One workgroup per flag word (16 chunks), which exits at
+// once when the word is clear.
 template <int D, int O, int ALG>
 __global__ __launch_bounds__(kThreads) void k_split_fallback(FusedArgs a) {
   const uint32_t word = a.flags[blockIdx.x];
@@ -1089,8 +1173,16 @@ __global__ __launch_bounds__(kThreads) void k_split_fallback(FusedArgs a) {
     const int64_t chunk = static_cast<int64_t>(blockIdx.x) * 16 + k;
     int64_t pbeg, pend;
     chunk_range(a, chunk, pbeg, pend);
-    if (f & kFlagPush) hbm_push_deposit<D, O, ALG>(a, pbeg, pend);
-    else if (a.charge != ZERO) hbm_deposit<D, O>(a, pbeg, pend);
+    if (!(f & kFlagPush)) {
+      if (a.charge != ZERO) hbm_deposit<D, O>(a, pbeg, pend);
+    } else if (a.recs == nullptr || !(a.recs[chunk].w >> 31)) {
+      hbm_push_deposit<D, O, ALG>(a, pbeg, pend);  // the whole chunk
+    } else {
+      // the particles the window push left outside a reused box; the others
+      // too when the deposit launch left the chunk (kFlagDeposit)
+      if ((f & kFlagDeposit) && a.charge != ZERO) hbm_deposit<D, O>(a, pbeg, pend);
+      hbm_push_deposit<D, O, ALG, true>(a, pbeg, pend);
+    }
   }
 }
 
@@ -1188,6 +1280,7 @@ ntt_status build_fused_args(void* out, int64_t* nchunks, const ParticleView& pv,
   a.toff = tile_offsets;
   a.flags = nullptr;
   a.recs = nullptr;
+  a.reuse = 0;
   a.ppt = ppt < 1 ? 1 : ppt;
   if (pv.n == 0) {
     a.nchunks = 0;
@@ -1215,7 +1308,7 @@ ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView&
                                         const uint32_t* tile_offsets,
                                         int64_t ntiles, int ppt, uint32_t* flags,
                                         int64_t flag_chunks, void* recs, int64_t rec_chunks,
-                                        hipStream_t st) {
+                                        bool reuse_boxes, hipStream_t st) {
   if (pv.n == 0) return NTT_OK;
   FusedArgs a;
   int64_t   nch = 0;
@@ -1226,8 +1319,10 @@ ntt_status launch_push_deposit_periodic(const ParticleView& pv, const FieldView&
       tile_offsets != nullptr && flags != nullptr && nch > 0 && nch <= flag_chunks) {
     a.flags = flags;
     // box records: 16-bit box corners (|cells| below 2^14), one per chunk
-    if (recs != nullptr && nch <= rec_chunks && em.n1 < 16384 && em.n2 < 16384 && em.n3 < 16384)
-      a.recs = static_cast<uint4*>(recs);
+    if (recs != nullptr && nch <= rec_chunks && em.n1 < 16384 && em.n2 < 16384 && em.n3 < 16384) {
+      a.recs  = static_cast<uint4*>(recs);
+      a.reuse = reuse_boxes ? 1 : 0;
+    }
     if (alg == ALG_BORIS) launch_split_o<ALG_BORIS>(order, a, st);
     else launch_split_o<ALG_VAY>(order, a, st);
     return check_launch();

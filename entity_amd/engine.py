@@ -125,12 +125,15 @@ class SRPICEngine:
         # tile offsets of every species (device uint32, ntiles + 1), or None
         self.tile_offsets: list = [None] * len(self.species)
         self.ntiles = _ntiles(self.lib, params)
-        # fixed-point current accumulator used by ntt_sr_step
+        # workspace of ntt_sr_step: the fixed-point current accumulator, and in 3D
+        # the split launches' flags and one box-record set per species, kept from
+        # step to step (zeroed: no valid record before the first push)
         ws = C.c_uint64(0)
         cur = self.fields.abi("cur")
-        _abi.check(self.lib.ntt_sr_step_workspace_size(C.byref(cur), C.byref(ws)),
-                   "ntt_sr_step_workspace_size")
-        self._step_ws = torch.empty(int(ws.value), dtype=torch.uint8, device=self.device)
+        _abi.check(self.lib.ntt_sr_step_workspace_size_species(C.byref(cur), len(self.species),
+                                                               C.byref(ws)),
+                   "ntt_sr_step_workspace_size_species")
+        self._step_ws = torch.zeros(int(ws.value), dtype=torch.uint8, device=self.device)
 
     # -- initial conditions ---------------------------------------------------
     def inject_uniform(self, ppc: int, *, temperature: float, drifts=None, seed=1,

@@ -518,9 +518,20 @@ NTT_API ntt_status ntt_sr_step(ntt_field_t*                em,
 
 /* bytes of the workspace ntt_sr_step uses for a current field of this shape:
  * the accumulator (24 bytes per ghost-extended cell), and in 3D the split
- * launches' chunk flags (one bit pair per cell) and box records (16 bytes per
- * 16 cells) */
+ * launches' chunk flags (one bit pair per cell) and one set of box records
+ * (16 bytes per 16 cells), shared by the species: every push then finds its
+ * chunks' boxes with a pass over the particles */
 NTT_API ntt_status ntt_sr_step_workspace_size(const ntt_field_t* cur, uint64_t* bytes);
+
+/* the same with one set of box records per species (3D): a workspace of this
+ * size, kept from step to step, lets each species' push start from the boxes
+ * its previous push recorded (a superset of the cells, no pass over the
+ * particles).  A record that no longer covers a particle (the particles were
+ * changed, or the workspace is new) only costs time: that particle is pushed
+ * and deposited by the fallback launch.  Layout: accumulator, flags, then the
+ * record sets in species order.  No reference counterpart. */
+NTT_API ntt_status ntt_sr_step_workspace_size_species(const ntt_field_t* cur, int32_t nspecies,
+                                                      uint64_t* bytes);
 
 /* Size of the device array ntt_sr_step_static works in: 32 MiB.  The
  * current accumulator must fit (1,398,101 ghost-extended cells: 2D up to

@@ -546,6 +546,17 @@ def test_em_half_out_is_the_field_the_push_reads(hip, dim, n):
         eng.advance(em_half_out=torch.zeros(3, device="cuda"))
 
 
+def _workspace_parts(n):
+    """bytes of the step workspace's accumulator, chunk flags and one box-record
+    set for a 3D box of n cells (the layout ntt.h documents)"""
+    N1, N2, N3 = (x + 2 * G for x in n)
+    ncell = int(np.prod(n))
+    acc = N1 * N2 * N3 * 3 * 8
+    flags = ((ncell + 1 + 15) // 16 * 4 + 255) // 256 * 256
+    rec = (ncell // 16 + 1 + 15) // 16 * 16 * 16
+    return acc, flags, rec
+
+
 @pytest.mark.parametrize("order,alg", [(1, BORIS), (2, VAY)])
 def test_split_and_fused_paths_agree(hip, order, alg):
     """A 3D sorted species runs as a push launch and a deposit launch when the
@@ -557,16 +568,13 @@ def test_split_and_fused_paths_agree(hip, order, alg):
     from entity_amd.kernels import device_errors
     dim, n, nf = 3, (16, 12, 8), 1
     runs = []
-    for keep in ("all", "flags", "acc"):
+    for keep in ("all", "shared", "flags", "acc"):
         eng = make_engine(dim, n, order, alg, nf, strict=True, theta=0.3, drift=0.6,
                           sort_interval=2)
-        N1, N2, N3 = (x + 2 * G for x in n)
-        acc = N1 * N2 * N3 * 3 * 8
-        ncell = int(np.prod(n))
-        flags = ((ncell + 1 + 15) // 16 * 4 + 255) // 256 * 256
+        acc, flags, rec = _workspace_parts(n)
         full = eng._step_ws.numel()
-        assert full > acc + flags                    # accumulator, flags, box records
-        size = {"all": full, "flags": acc + flags, "acc": acc}[keep]
+        assert full == acc + flags + 2 * rec         # accumulator, flags, a record set per species
+        size = {"all": full, "shared": acc + flags + rec, "flags": acc + flags, "acc": acc}[keep]
         eng._step_ws = eng._step_ws[:size]
         for _ in range(8):
             eng.step_forward()
@@ -578,3 +586,56 @@ def test_split_and_fused_paths_agree(hip, order, alg):
             np.testing.assert_array_equal(fb[k], fa[k], err_msg=k)
         for got, ref in zip(pb, pa):
             np.testing.assert_array_equal(_rows(got, dim), _rows(ref, dim))
+
+
+def _spoil_records(eng, n, mode):
+    """Rewrites every valid box record of both species' sets so that it no longer
+    covers all of its chunk's particles: 'shrink' moves the upper x edge one
+    cell down, 'shift' moves the box one cell up in y (records: uint4 of int16
+    lo | hi << 16 per dimension, rotation bits and the valid bit 31 in w)."""
+    acc, flags, rec = _workspace_parts(n)
+    for k in range(len(eng.species)):
+        off = acc + flags + k * rec
+        r = eng._step_ws[off:off + rec].view(torch.int32).view(-1, 4)
+        valid = r[:, 3] < 0
+        d = 0 if mode == "shrink" else 1
+        x = r[:, d]
+        lo = (x << 16) >> 16
+        hi = x >> 16
+        if mode == "shrink":
+            hi = torch.maximum(hi - 1, lo)
+        else:
+            lo, hi = lo + 1, hi + 1
+        r[:, d] = torch.where(valid, (lo & 0xFFFF) | (hi << 16), x)
+
+
+@pytest.mark.parametrize("mode", ["none", "shrink", "shift"])
+@pytest.mark.parametrize("order,alg", [(1, BORIS), (2, VAY)])
+def test_reused_boxes_agree_with_box_pass(hip, order, alg, mode):
+    """With a box-record set per species each push starts from the boxes its
+    previous push recorded.  Records that no longer cover every particle
+    ('shrink', 'shift': rewritten before every step) leave the uncovered
+    particles to the fallback launch, which pushes and deposits them from
+    HBM.  Either way the fields and the particle multisets equal those of
+    the shared record set, where every push finds its boxes with a pass over
+    the particles (strict-FP build, 8 steps with re-sorts every 2)."""
+    from entity_amd.kernels import device_errors
+    dim, n, nf = 3, (16, 12, 8), 1
+    runs = []
+    for keep in ("shared", "species"):
+        eng = make_engine(dim, n, order, alg, nf, strict=True, theta=0.3, drift=0.6,
+                          sort_interval=2)
+        acc, flags, rec = _workspace_parts(n)
+        if keep == "shared":
+            eng._step_ws = eng._step_ws[:acc + flags + rec]
+        for _ in range(8):
+            if keep == "species" and mode != "none" and eng.step > 0:
+                _spoil_records(eng, n, mode)
+            eng.step_forward()
+        assert device_errors(True) == 0
+        runs.append(snapshot(eng))
+    (pa, fa), (pb, fb) = runs
+    for k in ("em", "cur"):
+        np.testing.assert_array_equal(fb[k], fa[k], err_msg=k)
+    for got, ref in zip(pb, pa):
+        np.testing.assert_array_equal(_rows(got, dim), _rows(ref, dim))
