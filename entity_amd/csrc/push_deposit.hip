@@ -810,7 +810,7 @@ void k_push_deposit_batched(const FusedArgs* __restrict__ table,
 // Each kernel keeps one LDS window and the registers of its half of the work,
 // so both run at six waves per SIMD where the fused kernel runs at four.
 #ifndef NTT_SPLIT_PUSH_WAVES
-#define NTT_SPLIT_PUSH_WAVES 6
+#define NTT_SPLIT_PUSH_WAVES 8
 #endif
 #ifndef NTT_SPLIT_DEP_WAVES
 #define NTT_SPLIT_DEP_WAVES 6

@@ -284,7 +284,8 @@ __global__ __launch_bounds__(kB) void k_copy_batch(ArrayBatch b, const char* __r
 }
 
 // out of place: dst[k][p] = src[k][perm[p]] for up to kDirect arrays per pass
-constexpr int kDirect = 8;
+// (every array of a 3D species without its prev arrays in one pass)
+constexpr int kDirect = 12;
 struct ArrayPairs {
   const void* src[kDirect];
   void*       dst[kDirect];
@@ -307,8 +308,13 @@ __device__ __forceinline__ void scatter_items(void* dst, const void* src, const 
   for (int r = 0; r < kPermItems; ++r)
     if (act[r]) static_cast<T*>(dst)[q[r]] = v[r];
 }
+// The destination is inv[p], or, with prefix given (ranks), the exclusive
+// prefix of the particle's key plus its rank -- k_scatter's arithmetic done in
+// place, so that no pass writes the inverse permutation first.
 __global__ __launch_bounds__(kB) void k_permute_scatter(ArrayPairs b,
                                                         const uint32_t* __restrict__ inv,
+                                                        const uint32_t* __restrict__ prefix,
+                                                        const uint32_t* __restrict__ rank,
                                                         int64_t n) {
   const int64_t p0 = static_cast<int64_t>(blockIdx.x) * (kB * kPermItems) + threadIdx.x;
   int64_t       p[kPermItems];
@@ -318,7 +324,8 @@ __global__ __launch_bounds__(kB) void k_permute_scatter(ArrayPairs b,
   for (int r = 0; r < kPermItems; ++r) {
     p[r]   = p0 + r * kB;
     act[r] = p[r] < n;
-    q[r]   = act[r] ? inv[p[r]] : 0u;
+    q[r]   = 0u;
+    if (act[r]) q[r] = (prefix != nullptr) ? prefix[inv[p[r]]] + rank[p[r]] : inv[p[r]];
   }
 #pragma unroll
   for (int k = 0; k < kDirect; ++k) {
@@ -777,7 +784,10 @@ ntt_status sort_species(ntt_particles_t* src, ntt_particles_t* dst, int32_t dim,
     pv.i1, pv.i2, pv.i3, pv.tag, np, K, keys, rank, cnt);
   s = scan_exclusive(cnt, L.nkeys, scn, st);
   if (s != NTT_OK) return s;
-  k_scatter<<<blocks_for(np), kB, 0, st>>>(keys, np, cnt, L.nkeys, rank, perm, NTT_SORT_SCATTER);
+  // out of place by destination: the permute computes each destination itself
+  const bool fused_dest = dst != nullptr && NTT_SORT_SCATTER && !NTT_SORT_INTERLEAVE;
+  if (!fused_dest)
+    k_scatter<<<blocks_for(np), kB, 0, st>>>(keys, np, cnt, L.nkeys, rank, perm, NTT_SORT_SCATTER);
   if (tile_offsets != nullptr &&
       hipMemcpyAsync(tile_offsets, cnt, 4 * L.nkeys, hipMemcpyDeviceToDevice, st) != hipSuccess)
     return NTT_ERR_HIP;
@@ -797,7 +807,7 @@ ntt_status sort_species(ntt_particles_t* src, ntt_particles_t* dst, int32_t dim,
       if (b.count == kDirect || (k == kSortArrays && b.count > 0)) {
         if (NTT_SORT_SCATTER)
           k_permute_scatter<<<blocks_for((np + kPermItems - 1) / kPermItems), kB, 0, st>>>(
-            b, keys, np);
+            b, keys, fused_dest ? cnt : nullptr, fused_dest ? rank : nullptr, np);
         else k_permute_direct<<<blocks_for(np), kB, 0, st>>>(b, perm, np);
         b = ArrayPairs{};
       }
