@@ -968,10 +968,14 @@ struct DepRecord {
   uint4 a, b;  // a: packed cells, dx1..3; b: dxp1..3, coefficient
 };
 // Order of the deposit list: the particles of one bank class (stencil base
-// slot mod 32) take ranks 0, 1, ... within the stay and the cross group
-// (ranks from kRanks - 1 on share the last level).
-constexpr int kRanks  = 32;
-constexpr int kGroups = 2 * kRanks;  // (stay/cross, rank)
+// slot mod 32) take ranks 0, 1, ... within their class (ranks from the last
+// level of a class on share it).  Classes: the stay group (kRanks levels),
+// then the particles whose stencil shifts along x only, y only, z only, and
+// along several directions (kCrossRanks levels each), so that the wavefronts
+// of the general stencil mostly share one shape and skip the same slots.
+constexpr int kRanks      = 32;
+constexpr int kCrossRanks = 8;
+constexpr int kGroups     = kRanks + 4 * kCrossRanks;
 static_assert(kGroups == 64, "one wavefront scans the groups");
 
 template <int D, int O>
@@ -985,9 +989,9 @@ __device__ __forceinline__ void deposit_chunk(const FusedArgs& a, int64_t chunk)
   __shared__ int                red[4 * 3 + 1][kWaves];
   __shared__ int                gtab[3 * WM];
   __shared__ DepRecord          list[kThreads];
-  // per segment: particles per (stay/cross, bank class), per group, the
-  // groups' fill pointers, the counts
-  __shared__ uint32_t           ctr[2][32];
+  // per segment: particles per (class, bank class), per group, the groups'
+  // fill pointers, the counts
+  __shared__ uint32_t           ctr[5][32];
   __shared__ uint32_t           gsz[kGroups];
   __shared__ uint32_t           gpos[kGroups];
   __shared__ uint32_t           nstay_s, nall_s, heavy;
@@ -1034,15 +1038,16 @@ __device__ __forceinline__ void deposit_chunk(const FusedArgs& a, int64_t chunk)
     // (the previous segment's list and counts are read before the barrier
     // after the next classification, and the group counters are not read
     // by its deposits)
-    for (int k = tid; k < 64 + kGroups; k += kThreads) {
-      if (k < 64) ctr[k >> 5][k & 31] = 0u;
-      else gsz[k - 64] = 0u;
+    for (int k = tid; k < 160 + kGroups; k += kThreads) {
+      if (k < 160) ctr[k >> 5][k & 31] = 0u;
+      else gsz[k - 160] = 0u;
     }
     __syncthreads();
     // load one particle per thread; its record, class and the bank class of
     // its stencil base (the window's 64-bit slot of the lowest cell, mod 32)
     const uint32_t k    = sb + tid;
     bool           live = false, stay = true;
+    uint32_t       shift = 0u;  // bit d: the stencil shifts along d
     DepRecord      r;
     int            grp = 0;
     if (k < cnt && *rel(R.tag, k) == NTT_TAG_ALIVE) {
@@ -1057,7 +1062,7 @@ __device__ __forceinline__ void deposit_chunk(const FusedArgs& a, int64_t chunk)
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         if (!(x[d] >= ZERO && x[d] <= ONE)) live = false;
-        stay = stay && deposit_stays<O>(c[d], x[d], cp[d], xp[d]);
+        if (!deposit_stays<O>(c[d], x[d], cp[d], xp[d])) shift |= 1u << d;
         // the push's frame: taken at the cell before the push, unwrapped
         const int q  = wrap_prev(cp[d], a.n[d]);
         const int fp = (bx.rot[d] && q >= a.n[d] - (a.n[d] >> 1)) ? q - a.n[d] : q;
@@ -1077,10 +1082,14 @@ __device__ __forceinline__ void deposit_chunk(const FusedArgs& a, int64_t chunk)
                        __float_as_uint(x[2]));
       r.b = make_uint4(__float_as_uint(xp[0]), __float_as_uint(xp[1]), __float_as_uint(xp[2]),
                        __float_as_uint(coeff));
+      stay = shift == 0u;
       if (live) {
-        const int      cls  = stay ? 0 : 1;
+        // 0: stay; 1, 2, 3: shifts along x, y, z only; 4: along several
+        const int cls = stay ? 0 : (shift == 1u ? 1 : (shift == 2u ? 2 : (shift == 4u ? 3 : 4)));
         const uint32_t rank = atomicAdd(&ctr[cls][slot & 31], 1u);
-        grp = cls * kRanks + static_cast<int>(min(rank, static_cast<uint32_t>(kRanks - 1)));
+        grp = stay ? static_cast<int>(min(rank, static_cast<uint32_t>(kRanks - 1)))
+                   : kRanks + (cls - 1) * kCrossRanks +
+                       static_cast<int>(min(rank, static_cast<uint32_t>(kCrossRanks - 1)));
         atomicAdd(&gsz[grp], 1u);
       }
     }
@@ -1095,7 +1104,7 @@ __device__ __forceinline__ void deposit_chunk(const FusedArgs& a, int64_t chunk)
         if (lane >= o) incl += t;
       }
       gpos[tid] = incl - v;
-      if (tid == kGroups / 2 - 1) nstay_s = incl;
+      if (tid == kRanks - 1) nstay_s = incl;
       if (tid == 63) nall_s = incl;
     }
     __syncthreads();
