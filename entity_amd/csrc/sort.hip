@@ -83,20 +83,24 @@ __device__ __forceinline__ uint32_t aggregated_increment(uint32_t* counters, uin
 // Ranked histogram: every particle gets its key and its rank among the
 // particles of that key, so that the scatter needs no atomics (destination =
 // exclusive prefix of its key + rank).  A workgroup takes kB * R consecutive
-// particles.  Keys within kHist of a reference key (the workgroup's first key
-// minus kHist / 4: a nearly sorted species moves few particles to lower
-// tiles) are counted in LDS, and the workgroup then reserves each non-empty
-// bin's range with one global atomic; other keys use wavefront-aggregated
-// global atomics.  This keeps the atomics on the few counters of a small
-// species from queuing behind each other.
+// particles.  Keys near the workgroup's first key are counted in LDS, and the
+// workgroup then reserves each non-empty bin's range with one global atomic;
+// other keys use wavefront-aggregated global atomics.  This keeps the atomics
+// on the few counters of a small species from queuing behind each other.
+// The LDS bins: with zs = 0, kHist keys from the first key minus kHist / 4 (a
+// nearly sorted species moves few particles to lower tiles); with zs > 0 (3D
+// tiles, zs = tiles per z layer), three windows of kHist / 4 keys centred on
+// the first key and on the keys one tile layer below and above it, where the
+// particles that crossed a z tile boundary land.
 constexpr int kHist = 1024;
+constexpr int kWin  = kHist / 4;
 
 // base, n: the workgroup's first particle and the particle count (in the
 // numbering of key_of); keys and rank are written at gofs + p.
 template <int R, class KeyOf>
 __device__ __forceinline__ void ranked_keys(int64_t base, int64_t n, int64_t gofs, KeyOf key_of,
                                             uint32_t* keys, uint32_t* rank, uint32_t* counts,
-                                            uint32_t* hist, uint32_t* kref_s) {
+                                            uint32_t* hist, uint32_t* kref_s, uint32_t zs = 0u) {
   for (int b = threadIdx.x; b < kHist; b += kB) hist[b] = 0;
   uint32_t k[R];
   bool     act[R];
@@ -107,27 +111,43 @@ __device__ __forceinline__ void ranked_keys(int64_t base, int64_t n, int64_t gof
     k[r]            = act[r] ? key_of(p) : 0u;
     if (act[r]) keys[gofs + p] = k[r];
   }
-  if (threadIdx.x == 0) *kref_s = k[0] >= kHist / 4 ? k[0] - kHist / 4 : 0u;
+  if (threadIdx.x == 0)
+    *kref_s = (zs == 0u) ? (k[0] >= kHist / 4 ? k[0] - kHist / 4 : 0u) : k[0] - kWin / 2;
   __syncthreads();
   const uint32_t kref = *kref_s;
-  uint32_t       loc[R];
-  bool           win[R];
+  // bin of a key (kHist: none); with zs, window w in {0, 1, 2} holds the keys
+  // kref + (w - 1) zs + [0, kWin) at bins w kWin + [0, kWin) (modulo 2^32)
+  auto bin_of = [&](uint32_t key) -> uint32_t {
+    if (zs == 0u) return key - kref < static_cast<uint32_t>(kHist) ? key - kref : kHist;
+    const uint32_t d = key - kref;
+    if (d < static_cast<uint32_t>(kWin)) return kWin + d;
+    if (d + zs < static_cast<uint32_t>(kWin)) return d + zs;
+    if (d - zs < static_cast<uint32_t>(kWin)) return 2 * kWin + (d - zs);
+    return kHist;
+  };
+  uint32_t loc[R], bin[R];
+  bool     win[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    win[r]   = act[r] && k[r] - kref < static_cast<uint32_t>(kHist);
-    loc[r]   = win[r] ? atomicAdd(hist + (k[r] - kref), 1u) : 0u;
+    bin[r]   = act[r] ? bin_of(k[r]) : static_cast<uint32_t>(kHist);
+    win[r]   = bin[r] < static_cast<uint32_t>(kHist);
+    loc[r]   = win[r] ? atomicAdd(hist + bin[r], 1u) : 0u;
     const uint32_t g = aggregated_increment(counts, k[r], act[r] && !win[r]);
     if (act[r] && !win[r]) rank[gofs + base + r * kB + threadIdx.x] = g;
   }
   __syncthreads();
   for (int b = threadIdx.x; b < kHist; b += kB) {
     const uint32_t c = hist[b];
-    if (c != 0u) hist[b] = atomicAdd(counts + kref + b, c);
+    if (c != 0u) {
+      const uint32_t key =
+        (zs == 0u) ? kref + b : kref + (static_cast<uint32_t>(b / kWin) - 1u) * zs + b % kWin;
+      hist[b] = atomicAdd(counts + key, c);
+    }
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < R; ++r)
-    if (win[r]) rank[gofs + base + r * kB + threadIdx.x] = hist[k[r] - kref] + loc[r];
+    if (win[r]) rank[gofs + base + r * kB + threadIdx.x] = hist[bin[r]] + loc[r];
 }
 
 constexpr int kKeyItems = 4;  // particles per thread of the single-species key pass
@@ -144,8 +164,9 @@ __global__ __launch_bounds__(kB) void k_keys(const int32_t* __restrict__ i1,
   auto key_of = [&](int64_t p) -> uint32_t {
     return tag[p] == NTT_TAG_DEAD ? K.dead_key : K(i1[p], i2[p], K.dim == 3 ? i3[p] : 0);
   };
+  const uint32_t zs = (K.dim == 3) ? static_cast<uint32_t>(K.nt[0]) * K.nt[1] : 0u;
   ranked_keys<kKeyItems>(static_cast<int64_t>(blockIdx.x) * (kB * kKeyItems), n, 0, key_of, keys,
-                         rank, counts, hist, &kref);
+                         rank, counts, hist, &kref, zs);
 }
 
 // block-level exclusive scan of kScanTile elements; writes block totals

@@ -390,6 +390,47 @@ def test_sort_ranked_histogram_paths(hip, layout):
     assert off[ntiles] == alive
 
 
+def test_sort_ranked_histogram_3d_layers(hip):
+    """In 3D the key pass also bins the keys one tile layer below and above a
+    workgroup's first key (where particles that crossed a z tile boundary
+    land).  A tile-sorted species with every 5th particle moved one cell down
+    or up in z (periodic) and every 7th one cell in y must sort to a tile-ordered
+    permutation of the same particles with consistent offsets (16 x 16 x 4
+    tiles: 256 per layer, so the layer windows do not overlap)."""
+    from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
+    n, t = (64, 64, 16), (4, 4, 4)
+    prm = SRParams(dim=3, n=n, tile=t)
+    eng = SRPICEngine(prm, [SpeciesSpec("e", 1.0, -1.0)], int(np.prod(n)) * 2, with_payload=True)
+    eng.inject_uniform(2, temperature=0.1)
+    sp = eng.species[0]
+    npart = sp.npart
+    sp.pld_i[:npart] = torch.arange(npart, device="cuda")
+    sl = slice(0, npart, 5)
+    step = torch.where(torch.arange(npart, device="cuda")[sl] % 2 == 0, 1, -1).to(torch.int32)
+    sp.i3[sl] = (sp.i3[sl] + step) % n[2]
+    sl = slice(3, npart, 7)
+    sp.i2[sl] = (sp.i2[sl] + 1) % n[1]
+    before = sp.to_host()
+    eng.sort(compact=True)
+    after = sp.to_host()
+    off = eng.tile_offsets[0].cpu().numpy().astype(np.int64)
+    torch.cuda.synchronize()
+    assert sp.npart == npart
+    inv = np.empty(npart, dtype=np.int64)
+    inv[before["pld_i"]] = np.arange(npart)
+    src = inv[after["pld_i"]]
+    np.testing.assert_array_equal(np.sort(src), np.arange(npart))
+    for name, arr in before.items():
+        np.testing.assert_array_equal(after[name], arr[src], err_msg=name)
+    nt = [a // b for a, b in zip(n, t)]
+    key = (after["i1"] // t[0]) + nt[0] * ((after["i2"] // t[1]) + nt[1] * (after["i3"] // t[2]))
+    assert (np.diff(key) >= 0).all()
+    ntiles = int(np.prod(nt))
+    counts = np.bincount(key, minlength=ntiles)
+    np.testing.assert_array_equal(off[:ntiles], np.concatenate(([0], np.cumsum(counts)[:-1])))
+    assert off[ntiles] == npart
+
+
 def test_generator_statistics(hip):
     from entity_amd.engine import SRParams, SpeciesSpec, SRPICEngine
     prm = SRParams(dim=2, n=(64, 64), tile=(8, 8))
