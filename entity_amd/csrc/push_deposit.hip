@@ -896,10 +896,13 @@ __device__ __forceinline__ void push_chunk(const FusedArgs& a, int64_t chunk) {
       continue;
     }
     s.tag   = NTT_TAG_ALIVE;
-    bool ok = true;
+    bool ok = true, in = true;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       if (!(s.dx[d] >= ZERO && s.dx[d] <= ONE)) ok = false;
+      // a cell outside [0, n) (the box pass sends its chunk to the HBM
+      // path, which reports it) is left to the fallback with a reused box
+      if (static_cast<uint32_t>(s.i[d]) >= static_cast<uint32_t>(a.n[d])) in = false;
       if (bx.rot[d] && s.i[d] >= a.n[d] - (a.n[d] >> 1)) s.i[d] -= a.n[d];
     }
     if (!ok) {
@@ -907,7 +910,6 @@ __device__ __forceinline__ void push_chunk(const FusedArgs& a, int64_t chunk) {
       continue;
     }
     if (reuse) {
-      bool in = true;
 #pragma unroll
       for (int d = 0; d < D; ++d)
         if (s.i[d] < bx.lo[d] || s.i[d] > bx.hi[d]) in = false;

@@ -520,6 +520,28 @@ def test_heavy_particles_take_exact_path(hip, oracle):
     assert device_errors() & _abi.DEVERR_CURRENT_RANGE
 
 
+@pytest.mark.parametrize("records", ["species", "shared"])
+def test_cell_out_of_range_reported_by_split_push(hip, records):
+    """A live particle whose cell lies outside [0, n) is reported
+    (NTT_DEVERR_OUT_OF_BOUNDS) by the 3D split launches whether the push finds
+    its chunk boxes with a pass over the particles (shared record set) or
+    reuses its species' boxes of the previous step (a record set per species):
+    the particle goes to the HBM fallback, which checks its stencil."""
+    from entity_amd import _abi
+    from entity_amd.kernels import device_errors
+    n = (16, 12, 8)
+    eng = make_engine(3, n, 2, VAY, 1, theta=0.3, sort_interval=8)
+    if records == "shared":
+        acc, flags, rec = _workspace_parts(n)
+        eng._step_ws = eng._step_ws[:acc + flags + rec]
+    eng.step_forward()
+    eng.step_forward()
+    assert device_errors() == 0
+    eng.species[0].i1[5:6].fill_(n[0])
+    eng.step_forward()
+    assert device_errors() & _abi.DEVERR_OUT_OF_BOUNDS
+
+
 def test_double_buffered_sort_matches_in_place(hip):
     """ntt_sort_particles_to (out of place, arrays swapped afterwards) gives the
     same tile order, tile offsets, compaction and particle set as the in-place
