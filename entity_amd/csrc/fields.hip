@@ -419,37 +419,65 @@ __global__ __launch_bounds__(BX* BY) void k_filter_periodic(FieldView Dst, Field
 constexpr int FTX = 64, FTY = 4, FKC = 16;
 constexpr int FHX = FTX + 2, FHY = FTY + 2, FPL = FHX * FHY;
 
+// Each thread moves FLD fixed elements of every source plane (component,
+// row, column: offsets computed once); the next plane's loads are issued into
+// registers before the current plane's stencil is computed, so their latency
+// overlaps it, and are written to the ring after.
+constexpr int FLD = (3 * FPL + FTX * FTY - 1) / (FTX * FTY);
+
 __global__ __launch_bounds__(FTX* FTY) void k_filter_periodic_3d(FieldView Dst, FieldView Src) {
-  __shared__ float ring[3][3][FPL];
+  __shared__ float ring[3][3 * FPL];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * FTX + tx;
   const int G = Src.ngh;
   const int i0 = blockIdx.x * FTX, j0 = blockIdx.y * FTY, k0 = blockIdx.z * FKC;
   const int kend = min(k0 + FKC, Src.n3);
-  // loads the source plane kk (active index, may be -1 or n3: a ghost plane)
-  // with its halo into ring slot `slot`
-  auto load_plane = [&](int kk, int slot) {
-    for (int e = tid; e < 3 * FPL; e += FTX * FTY) {
+  // this thread's elements of a plane: offset in the source (within a plane
+  // of ghost-extended index 0; -1: outside the array, loaded as zero)
+  int64_t goff[FLD];
+#pragma unroll
+  for (int u = 0; u < FLD; ++u) {
+    const int e = tid + u * FTX * FTY;
+    goff[u]     = -1;
+    if (e < 3 * FPL) {
       const int c = e / FPL, r = e - c * FPL;
       const int hy = r / FHX, hx = r - hy * FHX;
       const int gi = i0 + hx - 1 + G, gj = j0 + hy - 1 + G;
-      float v = ZERO;
-      if (gi < Src.N1 && gj < Src.N2) v = Src(gi, gj, kk + G, c);
-      ring[slot][c][r] = v;
+      if (gi < Src.N1 && gj < Src.N2) goff[u] = Src.idx(gi, gj, 0, c);
+    }
+  }
+  const int64_t pstride = static_cast<int64_t>(Src.N1) * Src.N2;
+  float pre[FLD];
+  // source plane kk (active index, may be -1 or n3: a ghost plane) into pre
+  auto fetch = [&](int kk) {
+    const float* base = Src.d + static_cast<int64_t>(kk + G) * pstride;
+#pragma unroll
+    for (int u = 0; u < FLD; ++u) pre[u] = (goff[u] >= 0) ? base[goff[u]] : ZERO;
+  };
+  auto commit = [&](int slot) {
+#pragma unroll
+    for (int u = 0; u < FLD; ++u) {
+      const int e = tid + u * FTX * FTY;
+      if (e < 3 * FPL) ring[slot][e] = pre[u];
     }
   };
-  load_plane(k0 - 1, (k0 + 2) % 3);
-  load_plane(k0, k0 % 3);
+  fetch(k0 - 1);
+  commit((k0 + 2) % 3);
+  fetch(k0);
+  commit(k0 % 3);
+  fetch(k0 + 1);
   const int i = i0 + tx, j = j0 + ty;
   const bool active = i < Src.n1 && j < Src.n2;
   const int  o = (ty + 1) * FHX + (tx + 1);
   for (int k = k0; k < kend; ++k) {
-    load_plane(k + 1, (k + 1) % 3);
+    commit((k + 1) % 3);
     __syncthreads();
+    if (k + 2 <= kend) fetch(k + 2);
     if (active) {
       const int sm = (k + 2) % 3, s0 = k % 3, sp = (k + 1) % 3;
 #pragma unroll
       for (int comp = 0; comp < 3; ++comp) {
-        const float* pl[3] = { ring[sm][comp], ring[s0][comp], ring[sp][comp] };
+        const float* pl[3] = { ring[sm] + comp * FPL, ring[s0] + comp * FPL,
+                               ring[sp] + comp * FPL };
         float b[3][3][3];
 #pragma unroll
         for (int z = 0; z < 3; ++z)
