@@ -60,7 +60,8 @@ def parse():
     ap.add_argument("--gr-full-steps", type=int, default=20)
     ap.add_argument("--adjoint", type=int, default=1, help="also time the adjoint (gradient) run")
     ap.add_argument("--autotune", type=int, default=1, help="also run the launch autotuner")
-    ap.add_argument("--c0", type=int, default=1, help="also run the 2D 256^2 CPU-runnable case")
+    ap.add_argument("--beams2d", type=int, default=1,
+                    help="also run the 2D 256^2 CPU-runnable case")
     ap.add_argument("--parity", type=int, default=1,
                     help="also count the one-step differences from the oracle on a small box")
     ap.add_argument("--functor", type=int, default=1,
@@ -165,7 +166,7 @@ MI355X_CUS, MI355X_CLOCK_HZ = 256, 2.4e9
 
 
 def parity_leg(args):
-    """The C1 step's physics (order 2, Vay, 2 filter passes, B0 z plus the
+    """The 3D benchmark step's physics (order 2, Vay, 2 filter passes, B0 z plus the
     random-phase modes, theta = 0.3) on a 32 x 16 x 16 box, one ntt_sr_step
     against one step of the oracle from the same state: the particles whose
     cell index or tag differs (all within 4 float ulps of a cell face, which
@@ -713,37 +714,38 @@ def adjoint_leg(args):
                        "an FP64 forward's differences at h = 1e-3 is checked in the GPU tests"}
 
 
-C0_N, C0_PPC = (256, 256), 16
-C0_DRIFTS = [(0.57735, 0, 0), (0.57735, 0, 0), (-0.57735, 0, 0), (-0.57735, 0, 0)]
-C0_CHARGES = (-1.0, 1.0, -1.0, 1.0)
+BEAMS2D_N, BEAMS2D_PPC = (256, 256), 16
+BEAMS2D_DRIFTS = [(0.57735, 0, 0), (0.57735, 0, 0), (-0.57735, 0, 0), (-0.57735, 0, 0)]
+BEAMS2D_CHARGES = (-1.0, 1.0, -1.0, 1.0)
 
 
-def c0_params(tile=None, sort_interval=4, particles_per_thread=0):
+def beams2d_params(tile=None, sort_interval=4, particles_per_thread=0):
     from entity_amd.engine import SRParams
-    return SRParams(dim=2, n=C0_N, cfl=0.5, shape_order=1, n_filters=1, ppc0=4.0 * C0_PPC,
+    return SRParams(dim=2, n=BEAMS2D_N, cfl=0.5, shape_order=1, n_filters=1,
+                    ppc0=4.0 * BEAMS2D_PPC,
                     larmor0=8.0, skindepth0=8.0, sort_interval=sort_interval, tile=tile,
                     particles_per_thread=particles_per_thread)
 
 
 def autotune_leg(args):
-    """Autotuner on the C0 box (2D periodic 256 x 256, e-/e+ beams u_x =
+    """Autotuner on the 2D beam box (2D periodic 256 x 256, e-/e+ beams u_x =
     +-0.57735 over theta 0.1, 4 species x 16 ppc, order 1, Boris, 1 filter
     pass): every candidate (tile shape x re-sort cadence, and the unsorted
     layout x chunk size) runs 16 timed steps after 32 steps that age the
     layout; reports the fastest and its gain over 8x8 tiles re-sorted every 4
-    steps.  The C0 leg then runs with the pick."""
+    steps.  The 2D beam leg then runs with the pick."""
     import torch
     from entity_amd.autotune import Candidate, candidates_for, engine_runner, tune
     from entity_amd.engine import SpeciesSpec, SRPICEngine
-    n, ppc = C0_N, C0_PPC
-    sp = [SpeciesSpec(f"s{k}", 1.0, q) for k, q in enumerate(C0_CHARGES)]
-    ref = SRPICEngine(c0_params(tile=(8, 8)), sp, n[0] * n[1] * ppc)
-    ref.inject_uniform(ppc, temperature=0.1, drifts=C0_DRIFTS, seed=101)
+    n, ppc = BEAMS2D_N, BEAMS2D_PPC
+    sp = [SpeciesSpec(f"s{k}", 1.0, q) for k, q in enumerate(BEAMS2D_CHARGES)]
+    ref = SRPICEngine(beams2d_params(tile=(8, 8)), sp, n[0] * n[1] * ppc)
+    ref.inject_uniform(ppc, temperature=0.1, drifts=BEAMS2D_DRIFTS, seed=101)
     prts = [s_.to_host() for s_ in ref.species]
     del ref
 
     def make(c):
-        eng = SRPICEngine(c0_params(c.tile, c.sort_interval, c.particles_per_thread), sp,
+        eng = SRPICEngine(beams2d_params(c.tile, c.sort_interval, c.particles_per_thread), sp,
                           n[0] * n[1] * ppc)
         for s_, h in zip(eng.species, prts):
             s_.load(**h)
@@ -755,7 +757,7 @@ def autotune_leg(args):
     default = Candidate((8, 8), 4)
     t_def = res.ms_per_step.get(default)
     torch.cuda.synchronize()
-    return {"workload": "the C0 box: 2D periodic 256x256, e-/e+ beams, 4 species x 16 ppc "
+    return {"workload": "the 2D beam box: 2D periodic 256x256, e-/e+ beams, 4 species x 16 ppc "
                         "(4,194,304 particles), order 1, Boris, 1 filter pass",
             "best": {"tile": list(res.best.tile), "sort_interval": res.best.sort_interval,
                      "particles_per_thread": res.best.particles_per_thread},
@@ -764,7 +766,7 @@ def autotune_leg(args):
             "candidates": len(res.ms_per_step)}
 
 
-def c0_leg(args, tuned=None):
+def beams2d_leg(args, tuned=None):
     """The reference's CPU-runnable case on the GPU: 2D periodic 256 x 256,
     dx = 1, CFL 0.5, e-/e+ beams at u_x = +-0.57735 (beta 0.5) over
     theta = 0.1, 16 ppc per species x 4 species (4,194,304 particles), zero
@@ -779,18 +781,18 @@ def c0_leg(args, tuned=None):
     import torch
     from entity_amd import _abi
     from entity_amd.engine import SpeciesSpec, SRPICEngine
-    n, ppc, nsteps = C0_N, C0_PPC, 100
-    drifts = C0_DRIFTS
+    n, ppc, nsteps = BEAMS2D_N, BEAMS2D_PPC, 100
+    drifts = BEAMS2D_DRIFTS
     if tuned is not None:
         b = tuned["best"]
-        prm = c0_params(tuple(b["tile"]) if b["tile"] else None, b["sort_interval"],
+        prm = beams2d_params(tuple(b["tile"]) if b["tile"] else None, b["sort_interval"],
                         b["particles_per_thread"])
         layout = dict(b, source="autotune leg")
     else:
-        prm = c0_params()      # default tile: 16 x 8 at 256^2, re-sort every 4
+        prm = beams2d_params()      # default tile: 16 x 8 at 256^2, re-sort every 4
         layout = {"tile": "default", "sort_interval": 4, "particles_per_thread": 1,
                   "source": "defaults"}
-    sp = [SpeciesSpec(f"s{k}", 1.0, q) for k, q in enumerate(C0_CHARGES)]
+    sp = [SpeciesSpec(f"s{k}", 1.0, q) for k, q in enumerate(BEAMS2D_CHARGES)]
     eng = SRPICEngine(prm, sp, n[0] * n[1] * ppc)
     eng.inject_uniform(ppc, temperature=0.1, drifts=drifts, seed=101)
     for _ in range(4):
@@ -821,11 +823,11 @@ def c0_leg(args, tuned=None):
            "fused_kernel": {"avg_launch_ms": avg, "achieved_gbs": achieved,
                             "frac": achieved / HBM_PEAK_GBS}}
     if args.cpu_baseline:
-        out["cpu_baseline"] = c0_cpu_baseline(eng, n, ppc, drifts)
+        out["cpu_baseline"] = beams2d_cpu_baseline(eng, n, ppc, drifts)
     return out
 
 
-def c0_cpu_baseline(eng, n, ppc, drifts):
+def beams2d_cpu_baseline(eng, n, ppc, drifts):
     """The oracle's C++ restatement (FP32, on OpenMP threads) on 20 steps of
     the same box, from the GPU engine's state copied to the host."""
     import numpy as np
@@ -988,8 +990,8 @@ def main():
         out["adjoint"] = adjoint_leg(args)
     if args.autotune:
         out["autotune"] = autotune_leg(args)
-    if args.c0:
-        out["c0"] = c0_leg(args, out.get("autotune"))
+    if args.beams2d:
+        out["beams_2d"] = beams2d_leg(args, out.get("autotune"))
     if args.cpu_baseline:
         ccells = tuple(int(x) for x in args.cpu_cells.split(","))
         out["cpu_baseline"] = cpu_baseline(args, ccells, args.order, pusher, args.filters)
