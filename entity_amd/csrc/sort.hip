@@ -150,7 +150,7 @@ __device__ __forceinline__ void ranked_keys(int64_t base, int64_t n, int64_t gof
     if (win[r]) rank[gofs + base + r * kB + threadIdx.x] = hist[bin[r]] + loc[r];
 }
 
-constexpr int kKeyItems = 4;  // particles per thread of the single-species key pass
+constexpr int kKeyItems = 8;  // particles per thread of the single-species key pass
 
 __global__ __launch_bounds__(kB) void k_keys(const int32_t* __restrict__ i1,
                                              const int32_t* __restrict__ i2,
@@ -317,7 +317,7 @@ struct ArrayPairs {
 // the same by destination: dst[k][inv[p]] = src[k][p] (coalesced reads),
 // kPermItems particles per thread, so that each array's loads of a thread are
 // in flight together
-constexpr int kPermItems = 4;
+constexpr int kPermItems = 8;
 template <typename T>
 __device__ __forceinline__ void scatter_items(void* dst, const void* src, const int64_t* p,
                                               const uint32_t* q, const bool* act) {
