@@ -9,9 +9,10 @@
 // particles get the key one past the last tile.
 //
 // Steps: key + ranked histogram -> exclusive scan of the histogram -> each
-// particle's destination (prefix of its key + its rank) -> gather of the SoA
-// arrays, four per pass, through a scratch buffer.  All scratch lives in the
-// caller's workspace.
+// particle's destination (prefix of its key + its rank) -> the SoA arrays
+// moved there: in place, four per pass through a scratch buffer; out of place
+// (into a second set of arrays), up to twelve per pass, each destination
+// computed by the pass itself.  All scratch lives in the caller's workspace.
 #include "internal.hpp"
 
 namespace ntt {
