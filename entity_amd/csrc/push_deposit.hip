@@ -808,7 +808,9 @@ void k_push_deposit_batched(const FusedArgs* __restrict__ table,
 
 // ---- split path: push and deposit in separate launches ----------------------
 // Each kernel keeps one LDS window and the registers of its half of the work,
-// so both run at six waves per SIMD where the fused kernel runs at four.
+// so the push runs at eight waves per SIMD (64 registers) and the deposit at
+// six (its LDS window and list admit three workgroups per CU), where the fused
+// kernel runs at four.
 #ifndef NTT_SPLIT_PUSH_WAVES
 #define NTT_SPLIT_PUSH_WAVES 8
 #endif
